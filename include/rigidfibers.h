@@ -1,0 +1,177 @@
+/*
+ * rigidfibers.h -- C ABI of librigidfibers.so, the MI355X (gfx950) rigid-fiber
+ * Stokes-flow timestep.
+ *
+ * The reference has no library ABI: its timestep lives in the C++ class
+ * Simulation (src/simulation.h:38-98) behind a process boundary
+ * (runner/cuda_runner.py:154-163, `fibers <layout>`).  Every entry point below
+ * replaces one member or free function of that class, cited beside it, so a
+ * host (the Python runner here, or a ctypes/cgo/JNI binding) can drive the
+ * same assemble -> solve -> update velocities -> update fibers sequence.
+ *
+ * Conventions
+ *  - All functions return rf_status (0 = RF_OK).  rf_last_error() gives the
+ *    message of the last failure on the calling thread.
+ *  - Fiber vectors are host arrays of 3*M doubles (x0,y0,z0,x1,...).  The
+ *    system has n = 3*N*M unknowns; A is returned column-major, row
+ *    3N*i + 3k + a, column 3N*j + 3k + b (src/kernels/assemble_system_2D.cu:144-160).
+ *  - The caller owns every host array; the library copies what it needs.
+ *    The library owns all device memory and frees it in rf_destroy.
+ *  - One host thread per rf_simulation; each simulation runs on its own HIP
+ *    stream on the device that was current when rf_create was called.
+ */
+#ifndef RIGIDFIBERS_H_
+#define RIGIDFIBERS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RF_API __attribute__((visibility("default")))
+
+typedef enum rf_status {
+  RF_OK = 0,
+  RF_ERR_INVALID_ARGUMENT = 1,
+  RF_ERR_UNSUPPORTED = 2,
+  RF_ERR_OUT_OF_MEMORY = 3,
+  RF_ERR_DEVICE = 4,
+  RF_ERR_NONFINITE = 5,    /* NaN/Inf in the solution or velocities (e.g. coincident fibers) */
+  RF_ERR_SINGULAR = 6,     /* zero pivot in the direct solver */
+  RF_WARN_NOT_CONVERGED = 100 /* step completed, iterative solver hit its iteration limit */
+} rf_status;
+
+/* --magma (direct LU), --gmres, --bicgstab of fibers_runner.py:101-104, plus
+ * the mixed-precision forms. */
+typedef enum rf_solver {
+  RF_SOLVER_GMRES = 0,
+  RF_SOLVER_BICGSTAB = 1,
+  RF_SOLVER_DIRECT = 2,            /* FP32 LU with partial pivoting (magma_sgesv_gpu / sgesv) */
+  RF_SOLVER_DIRECT_MIXED_BF16 = 3, /* BF16-MFMA LU + FP64 iterative refinement */
+  RF_SOLVER_DIRECT_MIXED_FP16 = 4  /* FP16-MFMA LU + FP64 iterative refinement */
+} rf_solver;
+
+/* --numerical / --analytical (fibers_runner.py:105-108). */
+typedef enum rf_integration { RF_INTEGRATION_NUMERICAL = 0, RF_INTEGRATION_ANALYTICAL = 1 } rf_integration;
+
+/* Inner integral of the velocity update.  The shipped Fortran and CUDA
+ * (fortran/update_velocities.incl, src/kernels/update_velocities_2D.cu) always
+ * use the quadrature; the reference's golden vectors for --analytical
+ * (tests/reference/100_analytic_*) integrate it exactly.  FOLLOW uses the
+ * system's integration. */
+typedef enum rf_velocity_integration {
+  RF_VELOCITY_FOLLOW = 0,
+  RF_VELOCITY_NUMERICAL = 1,
+  RF_VELOCITY_ANALYTICAL = 2
+} rf_velocity_integration;
+
+/* Initial guess of the iterative solvers.  The shipped Fortran and ViennaCL
+ * start from zero (fortran/solve_system.incl:16-17); the reference's golden
+ * vectors for --gmres (tests/reference/100_*_gmres) start each step from the
+ * previous step's solution, which also saves iterations. */
+typedef enum rf_initial_guess { RF_GUESS_ZERO = 0, RF_GUESS_PREVIOUS = 1 } rf_initial_guess;
+
+/* The keys of the reference's configuration file (tests/config.ini:18-37,
+ * read by runner/cuda_runner.py:18-47), as run-time values. */
+typedef struct rf_config {
+  int32_t number_of_fibers;                           /* M */
+  int32_t number_of_terms_in_force_expansion;         /* N, 1..8 */
+  int32_t number_of_quadrature_intervals;             /* 4, 8 or 16 */
+  int32_t number_of_quadrature_points_per_interval;   /* 3 (the only rule the reference implements) */
+  double timestep;
+  double slenderness;
+  double external_force[3];                           /* F; the reference carries F/2 = (0,0,-0.5) */
+  int32_t solver;                                     /* rf_solver */
+  int32_t integration;                                /* rf_integration */
+  int32_t velocity_integration;                       /* rf_velocity_integration */
+  int32_t gmres_restart;
+  int32_t gmres_max_iterations;
+  double gmres_tolerance;                             /* relative residual ||b - Ax|| / ||b|| */
+  int32_t bicgstab_max_iterations;
+  double bicgstab_tolerance;
+  double refinement_tolerance;                        /* mixed direct solvers: FP64 residual target */
+  int32_t refinement_max_iterations;
+  int32_t initial_guess;                              /* rf_initial_guess */
+  int32_t reserved[7];
+} rf_config;
+
+typedef struct rf_solver_stats {
+  int32_t iterations;          /* Krylov iterations (matvecs) or refinement sweeps of the last solve */
+  int32_t converged;           /* 1 if the tolerance was met */
+  double relative_residual;    /* ||b - A x|| / ||b|| of the returned x (FP64) */
+} rf_solver_stats;
+
+typedef struct rf_timings {
+  /* milliseconds of the last step, HIP events on the simulation's stream;
+   * names of src/performance.cc:110-134 */
+  double assemble_system;
+  double solve_system;
+  double update_velocities;
+  double update_fibers;
+  double total;
+} rf_timings;
+
+typedef struct rf_simulation rf_simulation;
+
+/* Fills *cfg with the values of the reference's tests/config.ini. */
+RF_API rf_status rf_config_init(rf_config* cfg);
+
+/* Simulation::Simulation(Configuration) (src/simulation.cu:44-57): allocates
+ * device memory (initializeGPUMemory, :64-110), uploads the initial state
+ * (writeFiberStateToDevice, :112-118) and the constants
+ * (precomputeLegendrePolynomials, :153-284). */
+RF_API rf_status rf_create(const rf_config* cfg, const double* positions, const double* orientations,
+                           rf_simulation** out);
+
+/* Simulation::~Simulation (src/simulation.cu:59-62), which in the reference
+ * never frees device memory. */
+RF_API void rf_destroy(rf_simulation* sim);
+
+/* Simulation::step(size_t) (src/simulation.cu:286-329): assemble, solve,
+ * update velocities, update fibers (forward Euler when current_timestep == 0),
+ * then rotate the state buffers. */
+RF_API rf_status rf_step(rf_simulation* sim, int64_t current_timestep);
+
+/* The four phases of Simulation::step, for VALIDATE-style dumps between them. */
+RF_API rf_status rf_assemble_system(rf_simulation* sim);   /* assembleSystem, :331-399 */
+RF_API rf_status rf_solve_system(rf_simulation* sim);      /* solveSystem, :401-445 */
+RF_API rf_status rf_update_velocities(rf_simulation* sim); /* updateVelocities, :447-489 */
+RF_API rf_status rf_update_fibers(rf_simulation* sim, int32_t first_timestep); /* updateFibers, :491-528 */
+RF_API rf_status rf_advance(rf_simulation* sim);           /* the buffer swaps, :307-311 */
+
+/* Current positions/orientations (what saveFibers writes, :530-566); either
+ * pointer may be NULL. */
+RF_API rf_status rf_get_state(rf_simulation* sim, double* positions, double* orientations);
+/* Positions/orientations produced by the last rf_update_fibers (dumpFibers,
+ * :606-646, and the Fortran _POS/_ORIENT dumps, fortran/fibers.f95:287-303). */
+RF_API rf_status rf_get_next_state(rf_simulation* sim, double* positions, double* orientations);
+/* Velocities of the last rf_update_velocities (dumpVelocities, :770-822). */
+RF_API rf_status rf_get_velocities(rf_simulation* sim, double* translational, double* rotational);
+/* Replaces the state.  With history != 0 the previous positions,
+ * orientations and velocities are set too, so the next step may use the
+ * multistep update (a resume that the reference's state files cannot do). */
+RF_API rf_status rf_set_state(rf_simulation* sim, const double* positions, const double* orientations,
+                              int32_t history, const double* previous_positions,
+                              const double* previous_orientations, const double* previous_translational,
+                              const double* previous_rotational);
+/* System of the last assembly (dumpLinearSystem, :648-745) and solution
+ * (dumpSolutionSystem, :747-768).  a_matrix: n*n floats column-major, or NULL;
+ * b_vector, x_vector: n doubles each, or NULL. */
+RF_API rf_status rf_get_system(rf_simulation* sim, float* a_matrix, double* b_vector, double* x_vector);
+/* Number of unknowns n = 3*N*M. */
+RF_API int64_t rf_system_size(const rf_simulation* sim);
+
+RF_API rf_status rf_get_solver_stats(rf_simulation* sim, rf_solver_stats* stats);
+RF_API rf_status rf_get_timings(rf_simulation* sim, rf_timings* timings);
+/* Device bytes held by the simulation (admission accounting). */
+RF_API int64_t rf_device_bytes(const rf_simulation* sim);
+
+RF_API const char* rf_last_error(void);
+RF_API const char* rf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIGIDFIBERS_H_ */

@@ -1,0 +1,316 @@
+// rf_assemble.hip -- assembly of the dense fiber-fiber system A x = b.
+//
+// Reference: fortran/assemble_matrix.incl:1-161 (block formulas),
+// fortran/compute_inner_integral_numerically.incl:1-40 and
+// fortran/compute_inner_integral_analytically.incl:1-338 (inner integrals),
+// src/kernels/assemble_system_2D.cu:8-367 (the GPU layout of the same system).
+//
+// Work decomposition.  One workgroup = PAIRS consecutive target fibers i times
+// Q target nodes q_i (192 lanes), looping over a group of source fibers j.
+//   phase 1  lane (i, q_i) forms G_m(q_i) = sum_{q_j} w P_m(s_{q_j}) K(X_i(q_i) - X_j(q_j))
+//            for all modes m = 0..N at once (the reference recomputes K for each
+//            source mode), pairing q_j with its mirror -s_{q_j} so even modes use
+//            K(q) + K(-q) and odd modes K(q) - K(-q).  Analytical mode forms the
+//            same G_m exactly from FP64 moments, once per (i, j, q_i) for all m.
+//   phase 2  jobs (i, m, column b) read G_m(.)[(a,b)] from LDS, project over q_i
+//            (Theta_{k_i} = sum_{q_i} w P_{k_i+1}(s_{q_i}) G) and write the
+//            15-row column segment of A; jobs (i, m = 0) accumulate the RHS.
+// A is column-major with leading dimension lda; the workgroup's 8 target
+// fibers own 8*15 consecutive rows, so each column segment it writes is
+// contiguous.  Diagonal blocks are written as the identity, so A needs no
+// memset.  RHS partial sums go to bpart[group][row] (FP64), summed afterwards
+// in a fixed order: no atomics, bitwise reproducible.
+#include "rf_internal.h"
+
+namespace rf {
+
+constexpr int kPairThreads = 192;
+
+template <int N, int NQI, int M_>
+__device__ __forceinline__ void analytic_modes(const double R0[3], const double tj[3], double eps2,
+                                               const double* I1, const double* I3, const double* I5,
+                                               float acc[N + 1][6]) {
+  if constexpr (M_ <= N) {
+    double G[6];
+    analytic_g<M_>(R0, tj, eps2, I1, I3, I5, G);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[M_][c] = static_cast<float>(G[c]);
+    analytic_modes<N, NQI, M_ + 1>(R0, tj, eps2, I1, I3, I5, acc);
+  }
+}
+
+template <int N, int NQI, bool ANALYTIC>
+__global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t lda, const double* __restrict__ pos,
+                                                                const double* __restrict__ ori,
+                                                                float* __restrict__ A, double* __restrict__ bpart,
+                                                                int jgroup, PhysConsts pc) {
+  constexpr Quad<N, NQI> quad{};
+  constexpr int Q = 3 * NQI;
+  constexpr int PAIRS = kPairThreads / Q;
+  constexpr int NM = N + 1;
+  constexpr int ROWS = 3 * N;
+  static_assert(kPairThreads % Q == 0, "Q must divide the workgroup");
+  static_assert(Q % 2 == 0, "mirror pairing needs an even node count");
+
+  __shared__ float lds[PAIRS][NM][6][Q];
+  __shared__ float ti_lds[PAIRS][3];
+
+  const int tid = threadIdx.x;
+  const int p = tid / Q;
+  const int qi = tid - p * Q;
+  const int i = blockIdx.x * PAIRS + p;
+  const bool ivalid = i < M;
+  const int j0 = blockIdx.y * jgroup;
+  const int j1 = min(M, j0 + jgroup);
+  const int64_t n = static_cast<int64_t>(ROWS) * M;
+
+  double xi[3] = {0, 0, 0}, tid_[3] = {0, 0, 0};
+  if (ivalid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      xi[a] = pos[3 * i + a];
+      tid_[a] = ori[3 * i + a];
+    }
+  }
+  if (qi == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ti_lds[p][a] = static_cast<float>(tid_[a]);
+  }
+  const double sq_d = node_fp64<NQI>(qi);
+  const float sq = static_cast<float>(sq_d);
+  const float ti[3] = {static_cast<float>(tid_[0]), static_cast<float>(tid_[1]), static_cast<float>(tid_[2])};
+
+  // RHS accumulators of the jobs (i, m = 0); thread tid < PAIRS owns pair tid.
+  double bacc[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) bacc[r] = 0.0;
+
+  constexpr int kJobs = PAIRS * (3 * N + 1);
+
+  for (int j = j0; j < j1; ++j) {
+    // ---------------- phase 1: inner integrals G_m(q_i) ----------------
+    if (ivalid && i != j) {
+      const double xj0 = pos[3 * j + 0], xj1 = pos[3 * j + 1], xj2 = pos[3 * j + 2];
+      const double tj0 = ori[3 * j + 0], tj1 = ori[3 * j + 1], tj2 = ori[3 * j + 2];
+      float acc[NM][6];
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc[m][c] = 0.0f;
+      if constexpr (!ANALYTIC) {
+        // R(q_i, q_j) = (x_i - x_j) + s_i t_i - s_j t_j; the centre difference
+        // is formed in FP64 and rounded once.
+        const float ax = fmaf(sq, ti[0], static_cast<float>(xi[0] - xj0));
+        const float ay = fmaf(sq, ti[1], static_cast<float>(xi[1] - xj1));
+        const float az = fmaf(sq, ti[2], static_cast<float>(xi[2] - xj2));
+        const float tx = static_cast<float>(tj0), ty = static_cast<float>(tj1), tz = static_cast<float>(tj2);
+#pragma unroll
+        for (int h = 0; h < Q / 2; ++h) {
+          const float s = quad.s[h];
+          const float ux = s * tx, uy = s * ty, uz = s * tz;
+          float Kp[6], Km[6];
+          stokes_kernel(ax - ux, ay - uy, az - uz, pc.eps2, Kp);  // node h (s)
+          stokes_kernel(ax + ux, ay + uy, az + uz, pc.eps2, Km);  // node Q-1-h (-s)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) {
+            const float E = Kp[c] + Km[c];
+            const float O = Kp[c] - Km[c];
+#pragma unroll
+            for (int m = 0; m < NM; ++m) acc[m][c] = fmaf(quad.W[m][h], (m & 1) ? O : E, acc[m][c]);
+          }
+        }
+      } else {
+        const double R0[3] = {xi[0] - xj0 + sq_d * tid_[0], xi[1] - xj1 + sq_d * tid_[1],
+                              xi[2] - xj2 + sq_d * tid_[2]};
+        const double tjd[3] = {tj0, tj1, tj2};
+        const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
+        const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
+        double I1[N + 3], I3[N + 3], I5[N + 3];
+        moments<N>(bb, ll, I1, I3, I5);
+        analytic_modes<N, NQI, 0>(R0, tjd, pc.eps2_d, I1, I3, I5, acc);
+      }
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lds[p][m][c][qi] = acc[m][c];
+    }
+    __syncthreads();
+
+    // ---------------- phase 2: projections and stores ----------------
+    for (int job = tid; job < kJobs; job += kPairThreads) {
+      int p2, m, bcol;
+      if (job < PAIRS) {
+        p2 = job;
+        m = 0;
+        bcol = 0;
+      } else {
+        const int jj = job - PAIRS;
+        p2 = jj % PAIRS;
+        const int rest = jj / PAIRS;
+        m = 1 + rest / 3;
+        bcol = rest - 3 * (rest / 3);
+      }
+      const int i2 = blockIdx.x * PAIRS + p2;
+      if (i2 >= M) continue;
+      const float t0 = ti_lds[p2][0], t1 = ti_lds[p2][1], t2 = ti_lds[p2][2];
+      if (m == 0) {
+        if (i2 == j) continue;
+        // TF_{k_i} = sum_{q_i} w P_{k_i+1} (G_0(q_i) F/2)
+        float TF[N][3];
+#pragma unroll
+        for (int k = 0; k < N; ++k) TF[k][0] = TF[k][1] = TF[k][2] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float g0 = lds[p2][0][0][q], g1 = lds[p2][0][1][q], g2 = lds[p2][0][2][q];
+          const float g3 = lds[p2][0][3][q], g4 = lds[p2][0][4][q], g5 = lds[p2][0][5][q];
+          const float f0 = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
+          const float f1 = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
+          const float f2 = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
+#pragma unroll
+          for (int k = 0; k < N; ++k) {
+            TF[k][0] = fmaf(quad.W[k + 1][q], f0, TF[k][0]);
+            TF[k][1] = fmaf(quad.W[k + 1][q], f1, TF[k][1]);
+            TF[k][2] = fmaf(quad.W[k + 1][q], f2, TF[k][2]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const float QF = TF[k][0] * t0 + TF[k][1] * t1 + TF[k][2] * t2;
+          const float tt[3] = {t0, t1, t2};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float v = (k == 0) ? pc.D1 * tt[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * tt[a] * QF);
+            bacc[3 * k + a] -= static_cast<double>(v);
+          }
+        }
+        continue;
+      }
+      const int kj = m - 1;
+      const int64_t col = static_cast<int64_t>(ROWS) * j + 3 * kj + bcol;
+      float* out = A + col * lda + static_cast<int64_t>(ROWS) * i2;
+      if (i2 == j) {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) out[r] = (r == 3 * kj + bcol) ? 1.0f : 0.0f;
+        continue;
+      }
+      // Theta_{k_i}[(a, bcol)] for a = 0..2
+      const int c0 = sym_index(0, bcol), c1 = sym_index(1, bcol), c2 = sym_index(2, bcol);
+      float T[N][3];
+#pragma unroll
+      for (int k = 0; k < N; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float g0 = lds[p2][m][c0][q], g1 = lds[p2][m][c1][q], g2 = lds[p2][m][c2][q];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          T[k][0] = fmaf(quad.W[k + 1][q], g0, T[k][0]);
+          T[k][1] = fmaf(quad.W[k + 1][q], g1, T[k][1]);
+          T[k][2] = fmaf(quad.W[k + 1][q], g2, T[k][2]);
+        }
+      }
+      const float tt[3] = {t0, t1, t2};
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        // (Theta t_i)_bcol, Theta symmetric
+        const float Qb = T[k][0] * t0 + T[k][1] * t1 + T[k][2] * t2;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float v = (k == 0) ? pc.D1 * tt[a] * Qb : pc.gamma[k] * (T[k][a] - pc.eigen[k] * tt[a] * Qb);
+          out[3 * k + a] = v;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (tid < PAIRS) {
+    const int i2 = blockIdx.x * PAIRS + tid;
+    if (i2 < M) {
+      double* dst = bpart + static_cast<int64_t>(blockIdx.y) * n + static_cast<int64_t>(ROWS) * i2;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) dst[r] = bacc[r];
+    }
+  }
+}
+
+__global__ void sum_groups_kernel(int groups, int64_t len, const double* __restrict__ part, double* __restrict__ out) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= len) return;
+  double s = 0.0;
+  for (int g = 0; g < groups; ++g) s += part[static_cast<int64_t>(g) * len + r];
+  out[r] = s;
+}
+
+PairGrid pair_grid(int M, int pairs_per_tile) {
+  PairGrid g;
+  g.tiles = (M + pairs_per_tile - 1) / pairs_per_tile;
+  // aim at ~16 workgroups per CU over 256 CUs
+  int groups = (4096 + g.tiles - 1) / g.tiles;
+  if (groups > M) groups = M;
+  if (groups < 1) groups = 1;
+  g.jgroup = (M + groups - 1) / groups;
+  g.groups = (M + g.jgroup - 1) / g.jgroup;
+  return g;
+}
+
+int max_pair_groups(int M) {
+  int best = 1;
+  for (int pairs : {4, 8, 16}) {
+    const PairGrid g = pair_grid(M, pairs);
+    if (g.groups > best) best = g.groups;
+  }
+  return best;
+}
+
+template <int N, int NQI, bool AN>
+static hipError_t launch_asm_t(int M, int64_t lda, const double* pos, const double* ori, float* A, double* bpart,
+                               const PhysConsts& pc, hipStream_t s) {
+  constexpr int PAIRS = kPairThreads / (3 * NQI);
+  const PairGrid g = pair_grid(M, PAIRS);
+  hipLaunchKernelGGL((assemble_kernel<N, NQI, AN>), dim3(g.tiles, g.groups), dim3(kPairThreads), 0, s, M, lda, pos,
+                     ori, A, bpart, g.jgroup, pc);
+  return hipGetLastError();
+}
+
+template <int N, int NQI>
+static hipError_t launch_asm_n(bool an, int M, int64_t lda, const double* pos, const double* ori, float* A,
+                               double* bpart, const PhysConsts& pc, hipStream_t s) {
+  return an ? launch_asm_t<N, NQI, true>(M, lda, pos, ori, A, bpart, pc, s)
+            : launch_asm_t<N, NQI, false>(M, lda, pos, ori, A, bpart, pc, s);
+}
+
+bool assemble_supported(int N, int NQI) { return N >= 1 && N <= kMaxTerms && (NQI == 4 || NQI == 8 || NQI == 16); }
+
+#define RF_ASM_CASE(NN)                                                                      \
+  case NN:                                                                                   \
+    if (NQI == 4) return launch_asm_n<NN, 4>(analytical, M, lda, pos, ori, A, bpart, pc, s); \
+    if (NQI == 8) return launch_asm_n<NN, 8>(analytical, M, lda, pos, ori, A, bpart, pc, s); \
+    if (NQI == 16) return launch_asm_n<NN, 16>(analytical, M, lda, pos, ori, A, bpart, pc, s); \
+    return hipErrorInvalidValue;
+
+hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, const double* pos,
+                           const double* ori, float* A, double* bpart, const PhysConsts& pc, hipStream_t s) {
+  switch (N) {
+    RF_ASM_CASE(1)
+    RF_ASM_CASE(2)
+    RF_ASM_CASE(3)
+    RF_ASM_CASE(4)
+    RF_ASM_CASE(5)
+    RF_ASM_CASE(6)
+    RF_ASM_CASE(7)
+    RF_ASM_CASE(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+#undef RF_ASM_CASE
+
+hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double* out, hipStream_t s) {
+  const int threads = 256;
+  const int64_t blocks = (len + threads - 1) / threads;
+  hipLaunchKernelGGL(sum_groups_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, s, groups, len, part,
+                     out);
+  return hipGetLastError();
+}
+
+}  // namespace rf

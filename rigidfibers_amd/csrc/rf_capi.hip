@@ -1,0 +1,706 @@
+// rf_capi.hip -- the C ABI of include/rigidfibers.h: simulation state on the
+// device, step orchestration (src/simulation.cu:286-329) and the iterative
+// solvers (src/simulation.cu:401-445).
+//
+// Solvers.  GMRES(m) follows the reference semantics (x0 = 0, no
+// preconditioner, restart m, stop on ||b - A x|| / ||b|| <= tol with the true
+// residual recomputed at every restart: fortran/solve_system.incl:18-57,
+// fortran/sgmres.f:1240-1358; lib/viennacl/linalg/gmres.hpp:169-336).  The
+// Krylov basis, all vectors and the matvec accumulation are FP64 on the device
+// (A stays FP32); the basis is orthogonalised with classical Gram-Schmidt
+// applied twice, which spans the same space as the reference's MGS/Householder
+// variants, and the (m+1) x m Hessenberg least-squares problem is solved on
+// the host, one device->host copy per iteration.  BiCGStab follows
+// lib/viennacl/linalg/bicgstab.hpp:92-160.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rigidfibers.h"
+#include "rf_internal.h"
+#include "rf_lu.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+rf_status fail(rf_status code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define RF_HIP(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? RF_ERR_OUT_OF_MEMORY : RF_ERR_DEVICE,                    \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                                  \
+  } while (0)
+
+#define RF_TRY(expr)             \
+  do {                           \
+    rf_status s_ = (expr);       \
+    if (s_ != RF_OK) return s_;  \
+  } while (0)
+
+}  // namespace
+
+struct rf_simulation {
+  rf_config cfg{};
+  int M = 0, N = 0, NQI = 0;
+  int64_t n = 0, lda = 0;
+  bool analytical = false, vel_analytical = false;
+  rf::PhysConsts pc{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int device = 0;
+
+  double* pos[3] = {nullptr, nullptr, nullptr};
+  double* ori[3] = {nullptr, nullptr, nullptr};
+  double* tv[2] = {nullptr, nullptr};
+  double* rv[2] = {nullptr, nullptr};
+  int cur = 0, prev = 1, next = 2, vcur = 0, vprev = 1;
+  int vlast = 0;  // buffer written by the last rf_update_velocities
+  int plast = 0;  // buffer written by the last rf_update_fibers
+  bool have_history = false;
+  bool have_next = false;
+  bool have_solution = false;  // x holds a previous solution (initial guess)
+
+  float* A = nullptr;
+  double* b = nullptr;
+  double* x = nullptr;
+  double* bpart = nullptr;
+  float* wf = nullptr;
+  double* fm = nullptr;
+  double* vpart = nullptr;
+
+  // Krylov / refinement workspace
+  int kdim = 0;           // basis vectors allocated
+  double* V = nullptr;    // kdim x n
+  double* work = nullptr; // 6 x n (BiCGStab vectors, residuals)
+  double* gpart = nullptr;
+  double* dpart = nullptr;
+  double* hdev = nullptr;
+  int* flag = nullptr;
+  rf::GemvPlan plan{};
+  rf::LuWorkspace lu{};
+
+  rf_solver_stats stats{};
+  rf_timings timings{};
+  int64_t device_bytes = 0;
+  std::vector<void*> allocations;
+
+  template <typename T>
+  rf_status alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = count * sizeof(T) > 0 ? count * sizeof(T) : 16;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RF_ERR_OUT_OF_MEMORY, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                            hipGetErrorString(e));
+    }
+    allocations.push_back(q);
+    device_bytes += static_cast<int64_t>(bytes);
+    *p = static_cast<T*>(q);
+    return RF_OK;
+  }
+};
+
+namespace {
+
+rf_status validate_config(const rf_config* c) {
+  if (!c) return fail(RF_ERR_INVALID_ARGUMENT, "config is NULL");
+  if (c->number_of_fibers < 1) return fail(RF_ERR_INVALID_ARGUMENT, "NUMBER_OF_FIBERS must be >= 1");
+  if (c->number_of_terms_in_force_expansion < 1 || c->number_of_terms_in_force_expansion > rf::kMaxTerms)
+    return fail(RF_ERR_UNSUPPORTED, "NUMBER_OF_TERMS_IN_FORCE_EXPANSION must be in [1..8]");
+  if (c->number_of_quadrature_points_per_interval != 3)
+    return fail(RF_ERR_UNSUPPORTED, "NUMBER_OF_QUADRATURE_POINTS_PER_INTERVAL must be 3");
+  if (!rf::assemble_supported(c->number_of_terms_in_force_expansion, c->number_of_quadrature_intervals))
+    return fail(RF_ERR_UNSUPPORTED, "NUMBER_OF_QUADRATURE_INTERVALS must be 4, 8 or 16");
+  if (!(c->slenderness > 0.0) || !std::isfinite(c->slenderness))
+    return fail(RF_ERR_INVALID_ARGUMENT, "SLENDERNESS must be > 0");
+  const double d = -std::log(c->slenderness * c->slenderness * std::exp(1.0));
+  if (!(d > 2.0)) return fail(RF_ERR_INVALID_ARGUMENT, "SLENDERNESS too large: -ln(eps^2 e) must exceed 2");
+  if (!std::isfinite(c->timestep)) return fail(RF_ERR_INVALID_ARGUMENT, "TIMESTEP must be finite");
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(c->external_force[a])) return fail(RF_ERR_INVALID_ARGUMENT, "external force must be finite");
+  if (c->solver < RF_SOLVER_GMRES || c->solver > RF_SOLVER_DIRECT_MIXED_FP16)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown solver");
+  if (c->integration != RF_INTEGRATION_NUMERICAL && c->integration != RF_INTEGRATION_ANALYTICAL)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
+  if (c->velocity_integration < RF_VELOCITY_FOLLOW || c->velocity_integration > RF_VELOCITY_ANALYTICAL)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity integration");
+  if (c->solver == RF_SOLVER_GMRES) {
+    if (c->gmres_restart < 1 || c->gmres_restart > 1024)
+      return fail(RF_ERR_INVALID_ARGUMENT, "GMRES RESTART must be in [1..1024]");
+    if (c->gmres_max_iterations < 1) return fail(RF_ERR_INVALID_ARGUMENT, "GMRES MAX_ITERATIONS must be >= 1");
+    if (!(c->gmres_tolerance > 0.0)) return fail(RF_ERR_INVALID_ARGUMENT, "GMRES TOLERANCE must be > 0");
+  }
+  if (c->solver == RF_SOLVER_BICGSTAB) {
+    if (c->bicgstab_max_iterations < 1) return fail(RF_ERR_INVALID_ARGUMENT, "BICGSTAB MAX_ITERATIONS must be >= 1");
+    if (!(c->bicgstab_tolerance > 0.0)) return fail(RF_ERR_INVALID_ARGUMENT, "BICGSTAB TOLERANCE must be > 0");
+  }
+  if (c->solver == RF_SOLVER_DIRECT_MIXED_BF16 || c->solver == RF_SOLVER_DIRECT_MIXED_FP16) {
+    if (!(c->refinement_tolerance > 0.0)) return fail(RF_ERR_INVALID_ARGUMENT, "refinement tolerance must be > 0");
+    if (c->refinement_max_iterations < 1)
+      return fail(RF_ERR_INVALID_ARGUMENT, "refinement max iterations must be >= 1");
+  }
+  return RF_OK;
+}
+
+void make_consts(const rf_config& c, rf::PhysConsts* pc) {
+  std::memset(pc, 0, sizeof(*pc));
+  // fortran/fibers.f95:141-145 and :353-440 (lambda, eigen in FP64, then FP32)
+  const double eps = c.slenderness;
+  const double d = -std::log(eps * eps * std::exp(1.0));
+  const double e = 2.0;
+  double lambda = 2.0;
+  for (int k = 0; k < c.number_of_terms_in_force_expansion; ++k) {
+    if (k > 0) lambda += 2.0 / (k + 1);
+    const double eigen = ((d - e - lambda) / 2.0) / (d - lambda);
+    pc->eigen[k] = static_cast<float>(eigen);
+    // gamma_k = 0.5 (2(k+1) + 1) / (d + e - lambda_k), fortran/assemble_matrix.incl:101
+    pc->gamma[k] = (k == 0) ? 0.0f : static_cast<float>(0.5 * (2.0 * (k + 1) + 1.0) / (d + e - lambda));
+  }
+  pc->D1 = static_cast<float>(0.75 / (d - 2.0));
+  pc->eps2 = static_cast<float>(eps * eps);
+  pc->eps2_d = eps * eps;
+  pc->d = d;
+  for (int a = 0; a < 3; ++a) {
+    pc->Fh_d[a] = 0.5 * c.external_force[a];
+    pc->Fh[a] = static_cast<float>(pc->Fh_d[a]);
+  }
+}
+
+int solver_kdim(const rf_config& c) { return (c.solver == RF_SOLVER_GMRES) ? c.gmres_restart + 1 : 0; }
+
+rf_status check_nonfinite(rf_simulation* s, const double* v, int64_t len, const char* what) {
+  RF_HIP(hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
+  RF_HIP(rf::nonfinite_count(len, v, s->flag, s->stream));
+  int h = 0;
+  RF_HIP(hipMemcpyAsync(&h, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  if (h) return fail(RF_ERR_NONFINITE, std::string("non-finite values in ") + what);
+  return RF_OK;
+}
+
+// ||v||_2 on the device, returned to the host (synchronises).
+rf_status device_norm(rf_simulation* s, const double* v, double* out) {
+  RF_HIP(rf::mdot(s->n, 1, v, s->n, v, s->dpart, s->hdev, s->stream));
+  double h = 0.0;
+  RF_HIP(hipMemcpyAsync(&h, s->hdev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  *out = std::sqrt(h);
+  return RF_OK;
+}
+
+rf_status device_dot(rf_simulation* s, const double* u, const double* v, double* out) {
+  RF_HIP(rf::mdot(s->n, 1, u, s->n, v, s->dpart, s->hdev, s->stream));
+  RF_HIP(hipMemcpyAsync(out, s->hdev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
+// Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
+rf_status solve_gmres(rf_simulation* s) {
+  const int64_t n = s->n;
+  const int m = s->cfg.gmres_restart;
+  const int maxit = s->cfg.gmres_max_iterations;
+  const double tol = s->cfg.gmres_tolerance;
+  double* V = s->V;
+  auto Vk = [&](int k) { return V + static_cast<int64_t>(k) * n; };
+
+  double bnorm = 0.0;
+  RF_TRY(device_norm(s, s->b, &bnorm));
+  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  if (!warm) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
+  s->stats = rf_solver_stats{0, 0, 0.0};
+  if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
+  if (bnorm == 0.0) {
+    RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
+    s->stats.converged = 1;
+    return RF_OK;
+  }
+  std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
+  std::vector<double> hbuf(2 * (m + 1) + 2);
+  int it = 0;
+  bool first = true, converged = false;
+  double res = bnorm;
+  while (true) {
+    double beta;
+    if (first && !warm) {
+      RF_HIP(rf::axpby(n, 1.0, s->b, 0.0, Vk(0), s->stream));
+      beta = bnorm;
+      first = false;
+    } else {
+      first = false;
+      RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, Vk(0), s->b, 1, s->stream));
+      RF_TRY(device_norm(s, Vk(0), &beta));
+    }
+    res = beta;
+    if (!std::isfinite(beta)) return fail(RF_ERR_NONFINITE, "non-finite residual in GMRES");
+    if (beta / bnorm <= tol) {
+      converged = true;
+      break;
+    }
+    if (it >= maxit) break;
+    RF_HIP(rf::axpby(n, 1.0 / beta, Vk(0), 0.0, Vk(0), s->stream));
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k_used = 0;
+    for (int k = 0; k < m && it < maxit; ++k) {
+      ++it;
+      double* w = Vk(k + 1);
+      RF_HIP(rf::gemv(s->plan, s->A, Vk(k), s->gpart, w, nullptr, 0, s->stream));
+      // h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||^2
+      double* h1 = s->hdev;
+      double* h2 = s->hdev + (m + 1);
+      double* ww = s->hdev + 2 * (m + 1);
+      RF_HIP(rf::mdot(n, k + 1, V, n, w, s->dpart, h1, s->stream));
+      RF_HIP(rf::axpy_multi(n, k + 1, V, n, h1, -1.0, w, s->stream));
+      RF_HIP(rf::mdot(n, k + 1, V, n, w, s->dpart, h2, s->stream));
+      RF_HIP(rf::axpy_multi(n, k + 1, V, n, h2, -1.0, w, s->stream));
+      RF_HIP(rf::mdot(n, 1, w, n, w, s->dpart, ww, s->stream));
+      RF_HIP(hipMemcpyAsync(hbuf.data(), s->hdev, sizeof(double) * (2 * (m + 1) + 1), hipMemcpyDeviceToHost,
+                            s->stream));
+      RF_HIP(hipStreamSynchronize(s->stream));
+      const double hn = std::sqrt(hbuf[2 * (m + 1)]);
+      if (!std::isfinite(hn)) return fail(RF_ERR_NONFINITE, "non-finite Arnoldi vector in GMRES");
+      for (int i = 0; i <= k; ++i) H[i + static_cast<size_t>(k) * (m + 1)] = hbuf[i] + hbuf[(m + 1) + i];
+      H[(k + 1) + static_cast<size_t>(k) * (m + 1)] = hn;
+      if (hn > 0.0) RF_HIP(rf::axpby(n, 1.0 / hn, w, 0.0, w, s->stream));
+      for (int i = 0; i < k; ++i) {
+        const double a0 = H[i + static_cast<size_t>(k) * (m + 1)], a1 = H[i + 1 + static_cast<size_t>(k) * (m + 1)];
+        H[i + static_cast<size_t>(k) * (m + 1)] = cs[i] * a0 + sn[i] * a1;
+        H[i + 1 + static_cast<size_t>(k) * (m + 1)] = -sn[i] * a0 + cs[i] * a1;
+      }
+      const double a0 = H[k + static_cast<size_t>(k) * (m + 1)], a1 = H[k + 1 + static_cast<size_t>(k) * (m + 1)];
+      const double rr = std::hypot(a0, a1);
+      cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
+      sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
+      H[k + static_cast<size_t>(k) * (m + 1)] = rr;
+      H[k + 1 + static_cast<size_t>(k) * (m + 1)] = 0.0;
+      g[k + 1] = -sn[k] * g[k];
+      g[k] = cs[k] * g[k];
+      k_used = k + 1;
+      res = std::fabs(g[k + 1]);
+      if (res / bnorm <= tol || hn == 0.0) break;
+    }
+    for (int i = k_used - 1; i >= 0; --i) {
+      double acc = g[i];
+      for (int j = i + 1; j < k_used; ++j) acc -= H[i + static_cast<size_t>(j) * (m + 1)] * y[j];
+      y[i] = acc / H[i + static_cast<size_t>(i) * (m + 1)];
+    }
+    RF_HIP(hipMemcpyAsync(s->hdev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s->stream));
+    RF_HIP(rf::axpy_multi(n, k_used, V, n, s->hdev, 1.0, s->x, s->stream));
+    // the loop head recomputes the true residual b - A x
+  }
+  s->stats.iterations = it;
+  s->stats.converged = converged ? 1 : 0;
+  s->stats.relative_residual = res / bnorm;
+  return RF_OK;
+}
+
+// Unpreconditioned BiCGStab, x0 = 0 (lib/viennacl/linalg/bicgstab.hpp:92-160).
+rf_status solve_bicgstab(rf_simulation* s) {
+  const int64_t n = s->n;
+  const int maxit = s->cfg.bicgstab_max_iterations;
+  const double tol = s->cfg.bicgstab_tolerance;
+  double* r = s->work;
+  double* r0 = s->work + n;
+  double* p = s->work + 2 * n;
+  double* v = s->work + 3 * n;
+  double* sv = s->work + 4 * n;
+  double* t = s->work + 5 * n;
+  double bnorm = 0.0;
+  RF_TRY(device_norm(s, s->b, &bnorm));
+  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  s->stats = rf_solver_stats{0, 0, 0.0};
+  if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
+  if (bnorm == 0.0 || !warm) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
+  if (bnorm == 0.0) {
+    s->stats.converged = 1;
+    return RF_OK;
+  }
+  double rho, res;
+  if (warm) {
+    RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, r, s->b, 1, s->stream));
+    RF_TRY(device_norm(s, r, &res));
+    rho = res * res;
+  } else {
+    RF_HIP(rf::axpby(n, 1.0, s->b, 0.0, r, s->stream));
+    rho = bnorm * bnorm;
+    res = bnorm;
+  }
+  RF_HIP(rf::axpby(n, 1.0, r, 0.0, r0, s->stream));
+  RF_HIP(rf::axpby(n, 1.0, r, 0.0, p, s->stream));
+  int it = 0;
+  bool converged = res / bnorm <= tol;
+  while (it < maxit && !converged) {
+    ++it;
+    RF_HIP(rf::gemv(s->plan, s->A, p, s->gpart, v, nullptr, 0, s->stream));
+    double r0v;
+    RF_TRY(device_dot(s, r0, v, &r0v));
+    if (r0v == 0.0 || !std::isfinite(r0v)) break;
+    const double alpha = rho / r0v;
+    RF_HIP(rf::axpby(n, 1.0, r, 0.0, sv, s->stream));
+    RF_HIP(rf::axpby(n, -alpha, v, 1.0, sv, s->stream));
+    double snorm;
+    RF_TRY(device_norm(s, sv, &snorm));
+    if (!std::isfinite(snorm)) return fail(RF_ERR_NONFINITE, "non-finite residual in BiCGStab");
+    if (snorm / bnorm <= tol) {
+      RF_HIP(rf::axpby(n, alpha, p, 1.0, s->x, s->stream));
+      res = snorm;
+      converged = true;
+      break;
+    }
+    RF_HIP(rf::gemv(s->plan, s->A, sv, s->gpart, t, nullptr, 0, s->stream));
+    double ts, tt;
+    RF_TRY(device_dot(s, t, sv, &ts));
+    RF_TRY(device_dot(s, t, t, &tt));
+    const double omega = (tt == 0.0) ? 0.0 : ts / tt;
+    RF_HIP(rf::axpby(n, alpha, p, 1.0, s->x, s->stream));
+    RF_HIP(rf::axpby(n, omega, sv, 1.0, s->x, s->stream));
+    RF_HIP(rf::axpby(n, 1.0, sv, 0.0, r, s->stream));
+    RF_HIP(rf::axpby(n, -omega, t, 1.0, r, s->stream));
+    RF_TRY(device_norm(s, r, &res));
+    if (res / bnorm <= tol) {
+      converged = true;
+      break;
+    }
+    if (omega == 0.0) break;
+    double rho_new;
+    RF_TRY(device_dot(s, r0, r, &rho_new));
+    const double beta = (rho_new / rho) * (alpha / omega);
+    rho = rho_new;
+    // p = r + beta (p - omega v)
+    RF_HIP(rf::axpby(n, -omega, v, 1.0, p, s->stream));
+    RF_HIP(rf::axpby(n, 1.0, r, beta, p, s->stream));
+  }
+  // report the true residual of the returned x
+  RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, r, s->b, 1, s->stream));
+  RF_TRY(device_norm(s, r, &res));
+  s->stats.iterations = it;
+  s->stats.converged = (converged && res / bnorm <= 10.0 * tol) ? 1 : 0;
+  s->stats.relative_residual = res / bnorm;
+  return RF_OK;
+}
+
+rf_status solve_direct(rf_simulation* s) {
+  const int mode = s->cfg.solver;
+  rf::LuResult r{};
+  hipError_t e = rf::lu_solve(s->lu, mode == RF_SOLVER_DIRECT ? rf::LuMode::kFP32
+                                     : mode == RF_SOLVER_DIRECT_MIXED_BF16 ? rf::LuMode::kBF16
+                                                                          : rf::LuMode::kFP16,
+                              s->A, s->lda, s->n, s->b, s->x, s->gpart, s->plan, s->work, s->dpart, s->hdev,
+                              s->cfg.refinement_tolerance, s->cfg.refinement_max_iterations, s->stream, &r);
+  if (e != hipSuccess) return fail(RF_ERR_DEVICE, std::string("LU: ") + hipGetErrorString(e));
+  if (r.info != 0) return fail(RF_ERR_SINGULAR, "zero pivot at column " + std::to_string(r.info));
+  s->stats.iterations = r.refinement_iterations;
+  s->stats.relative_residual = r.relative_residual;
+  s->stats.converged = r.converged ? 1 : 0;
+  if (!std::isfinite(r.relative_residual)) return fail(RF_ERR_NONFINITE, "non-finite solution in LU solve");
+  return RF_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0f;
+  return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_config_init(rf_config* cfg) {
+  if (!cfg) return fail(RF_ERR_INVALID_ARGUMENT, "config is NULL");
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->number_of_fibers = 100;
+  cfg->number_of_terms_in_force_expansion = 5;
+  cfg->number_of_quadrature_intervals = 8;
+  cfg->number_of_quadrature_points_per_interval = 3;
+  cfg->timestep = 0.1;
+  cfg->slenderness = 0.01;
+  cfg->external_force[0] = 0.0;
+  cfg->external_force[1] = 0.0;
+  cfg->external_force[2] = -1.0;
+  cfg->solver = RF_SOLVER_GMRES;
+  cfg->integration = RF_INTEGRATION_NUMERICAL;
+  cfg->velocity_integration = RF_VELOCITY_FOLLOW;
+  cfg->gmres_restart = 10;
+  cfg->gmres_max_iterations = 1000;
+  cfg->gmres_tolerance = 1e-5;
+  cfg->bicgstab_max_iterations = 1000;
+  cfg->bicgstab_tolerance = 1e-5;
+  cfg->refinement_tolerance = 1e-10;
+  cfg->refinement_max_iterations = 50;
+  cfg->initial_guess = RF_GUESS_PREVIOUS;
+  return RF_OK;
+}
+
+rf_status rf_create(const rf_config* cfg, const double* positions, const double* orientations, rf_simulation** out) {
+  if (!out) return fail(RF_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  RF_TRY(validate_config(cfg));
+  if (!positions || !orientations) return fail(RF_ERR_INVALID_ARGUMENT, "positions/orientations are NULL");
+  const int M = cfg->number_of_fibers;
+  for (int64_t k = 0; k < 3 * static_cast<int64_t>(M); ++k)
+    if (!std::isfinite(positions[k]) || !std::isfinite(orientations[k]))
+      return fail(RF_ERR_INVALID_ARGUMENT, "non-finite initial state");
+  rf_simulation* s = new (std::nothrow) rf_simulation();
+  if (!s) return fail(RF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  auto cleanup = [&](rf_status st) {
+    rf_destroy(s);
+    return st;
+  };
+  s->cfg = *cfg;
+  s->M = M;
+  s->N = cfg->number_of_terms_in_force_expansion;
+  s->NQI = cfg->number_of_quadrature_intervals;
+  s->n = 3LL * s->N * M;
+  s->lda = (s->n + 63) / 64 * 64;
+  s->analytical = cfg->integration == RF_INTEGRATION_ANALYTICAL;
+  s->vel_analytical = (cfg->velocity_integration == RF_VELOCITY_FOLLOW) ? s->analytical
+                                                                        : cfg->velocity_integration == RF_VELOCITY_ANALYTICAL;
+  make_consts(*cfg, &s->pc);
+  if (hipGetDevice(&s->device) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "no HIP device"));
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(RF_ERR_DEVICE, "hipStreamCreate failed"));
+  for (auto& e : s->ev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
+
+  const int64_t n = s->n;
+  const size_t fiber3 = 3 * static_cast<size_t>(M);
+  rf_status st = RF_OK;
+  for (int k = 0; k < 3 && st == RF_OK; ++k) {
+    st = s->alloc(&s->pos[k], fiber3);
+    if (st == RF_OK) st = s->alloc(&s->ori[k], fiber3);
+  }
+  for (int k = 0; k < 2 && st == RF_OK; ++k) {
+    st = s->alloc(&s->tv[k], fiber3);
+    if (st == RF_OK) st = s->alloc(&s->rv[k], fiber3);
+  }
+  if (st == RF_OK) st = s->alloc(&s->A, static_cast<size_t>(s->lda) * static_cast<size_t>(n));
+  if (st == RF_OK) st = s->alloc(&s->b, static_cast<size_t>(n));
+  if (st == RF_OK) st = s->alloc(&s->x, static_cast<size_t>(n));
+  if (st == RF_OK) st = s->alloc(&s->bpart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
+  if (st == RF_OK) st = s->alloc(&s->vpart, static_cast<size_t>(rf::max_pair_groups(M)) * M * 6);
+  if (st == RF_OK) st = s->alloc(&s->wf, static_cast<size_t>(M) * 3 * 3 * s->NQI);
+  if (st == RF_OK) st = s->alloc(&s->fm, static_cast<size_t>(M) * 3 * (s->N + 1));
+  s->plan = rf::gemv_plan(n, s->lda);
+  if (st == RF_OK) st = s->alloc(&s->gpart, rf::gemv_partial_elems(s->plan));
+  s->kdim = solver_kdim(*cfg);
+  if (st == RF_OK && s->kdim > 0) st = s->alloc(&s->V, static_cast<size_t>(s->kdim) * n);
+  if (st == RF_OK) st = s->alloc(&s->work, 6 * static_cast<size_t>(n));
+  const int kmax = (s->kdim > 2 ? s->kdim : 2) + 1;
+  if (st == RF_OK) st = s->alloc(&s->dpart, static_cast<size_t>(rf::mdot_blocks(n)) * kmax);
+  if (st == RF_OK) st = s->alloc(&s->hdev, static_cast<size_t>(2 * kmax + 4));
+  if (st == RF_OK) st = s->alloc(&s->flag, 1);
+  if (st != RF_OK) return cleanup(st);
+  if (cfg->solver == RF_SOLVER_DIRECT || cfg->solver == RF_SOLVER_DIRECT_MIXED_BF16 ||
+      cfg->solver == RF_SOLVER_DIRECT_MIXED_FP16) {
+    const rf::LuMode mode = cfg->solver == RF_SOLVER_DIRECT ? rf::LuMode::kFP32
+                            : cfg->solver == RF_SOLVER_DIRECT_MIXED_BF16 ? rf::LuMode::kBF16
+                                                                          : rf::LuMode::kFP16;
+    hipError_t e = rf::lu_workspace_create(mode, n, s->lda, &s->lu);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return cleanup(fail(e == hipErrorOutOfMemory ? RF_ERR_OUT_OF_MEMORY : RF_ERR_DEVICE,
+                          std::string("LU workspace: ") + hipGetErrorString(e)));
+    }
+    s->device_bytes += rf::lu_workspace_bytes(s->lu);
+  }
+  if (hipMemcpyAsync(s->pos[s->cur], positions, fiber3 * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess ||
+      hipMemcpyAsync(s->ori[s->cur], orientations, fiber3 * sizeof(double), hipMemcpyHostToDevice, s->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess)
+    return cleanup(fail(RF_ERR_DEVICE, "initial upload failed"));
+  *out = s;
+  return RF_OK;
+}
+
+void rf_destroy(rf_simulation* s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  rf::lu_workspace_destroy(&s->lu);
+  for (void* p : s->allocations) (void)hipFree(p);
+  for (auto& e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+rf_status rf_assemble_system(rf_simulation* s) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
+                             s->bpart, s->pc, s->stream));
+  const rf::PairGrid g = rf::pair_grid(s->M, 192 / (3 * s->NQI));
+  RF_HIP(rf::launch_sum_groups(g.groups, s->n, s->bpart, s->b, s->stream));
+  return RF_OK;
+}
+
+rf_status rf_solve_system(rf_simulation* s) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  switch (s->cfg.solver) {
+    case RF_SOLVER_GMRES:
+      RF_TRY(solve_gmres(s));
+      break;
+    case RF_SOLVER_BICGSTAB:
+      RF_TRY(solve_bicgstab(s));
+      break;
+    default:
+      RF_TRY(solve_direct(s));
+      break;
+  }
+  s->have_solution = true;
+  return RF_OK;
+}
+
+rf_status rf_update_velocities(rf_simulation* s) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  RF_HIP(rf::launch_velocities(s->N, s->NQI, s->vel_analytical, s->M, s->pos[s->cur], s->ori[s->cur], s->x, s->wf,
+                               s->fm, s->vpart, s->tv[s->vcur], s->rv[s->vcur], s->pc, s->stream));
+  s->vlast = s->vcur;
+  return RF_OK;
+}
+
+rf_status rf_update_fibers(rf_simulation* s, int32_t first_timestep) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const bool first = first_timestep != 0 || !s->have_history;
+  RF_HIP(rf::launch_update(first, s->M, s->cfg.timestep, s->pos[s->prev], s->pos[s->cur], s->ori[s->prev],
+                           s->ori[s->cur], s->tv[s->vprev], s->tv[s->vcur], s->rv[s->vprev], s->rv[s->vcur],
+                           s->pos[s->next], s->ori[s->next], s->stream));
+  s->plast = s->next;
+  s->have_next = true;
+  return RF_OK;
+}
+
+rf_status rf_advance(rf_simulation* s) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  if (!s->have_next) return fail(RF_ERR_INVALID_ARGUMENT, "rf_advance before rf_update_fibers");
+  const int old_prev = s->prev;
+  s->prev = s->cur;
+  s->cur = s->next;
+  s->next = old_prev;
+  std::swap(s->vcur, s->vprev);
+  s->have_history = true;
+  s->have_next = false;
+  return RF_OK;
+}
+
+rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  RF_HIP(hipEventRecord(s->ev[0], s->stream));
+  RF_TRY(rf_assemble_system(s));
+  RF_HIP(hipEventRecord(s->ev[1], s->stream));
+  RF_TRY(rf_solve_system(s));
+  RF_HIP(hipEventRecord(s->ev[2], s->stream));
+  RF_TRY(rf_update_velocities(s));
+  RF_HIP(hipEventRecord(s->ev[3], s->stream));
+  RF_TRY(check_nonfinite(s, s->tv[s->vcur], 3LL * s->M, "translational velocities"));
+  RF_TRY(rf_update_fibers(s, current_timestep == 0 ? 1 : 0));
+  RF_HIP(hipEventRecord(s->ev[4], s->stream));
+  RF_HIP(hipEventSynchronize(s->ev[4]));
+  s->timings.assemble_system = elapsed(s->ev[0], s->ev[1]);
+  s->timings.solve_system = elapsed(s->ev[1], s->ev[2]);
+  s->timings.update_velocities = elapsed(s->ev[2], s->ev[3]);
+  s->timings.update_fibers = elapsed(s->ev[3], s->ev[4]);
+  s->timings.total = elapsed(s->ev[0], s->ev[4]);
+  RF_TRY(rf_advance(s));
+  if (!s->stats.converged) {
+    g_last_error = "solver did not reach its tolerance (relative residual " +
+                   std::to_string(s->stats.relative_residual) + ")";
+    return RF_WARN_NOT_CONVERGED;
+  }
+  return RF_OK;
+}
+
+rf_status rf_get_state(rf_simulation* s, double* positions, double* orientations) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const size_t bytes = 3 * static_cast<size_t>(s->M) * sizeof(double);
+  if (positions) RF_HIP(hipMemcpyAsync(positions, s->pos[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  if (orientations) RF_HIP(hipMemcpyAsync(orientations, s->ori[s->cur], bytes, hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
+rf_status rf_get_next_state(rf_simulation* s, double* positions, double* orientations) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const int k = s->plast;
+  const size_t bytes = 3 * static_cast<size_t>(s->M) * sizeof(double);
+  if (positions) RF_HIP(hipMemcpyAsync(positions, s->pos[k], bytes, hipMemcpyDeviceToHost, s->stream));
+  if (orientations) RF_HIP(hipMemcpyAsync(orientations, s->ori[k], bytes, hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
+rf_status rf_get_velocities(rf_simulation* s, double* translational, double* rotational) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const int k = s->vlast;
+  const size_t bytes = 3 * static_cast<size_t>(s->M) * sizeof(double);
+  if (translational) RF_HIP(hipMemcpyAsync(translational, s->tv[k], bytes, hipMemcpyDeviceToHost, s->stream));
+  if (rotational) RF_HIP(hipMemcpyAsync(rotational, s->rv[k], bytes, hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
+rf_status rf_set_state(rf_simulation* s, const double* positions, const double* orientations, int32_t history,
+                       const double* pp, const double* po, const double* pt, const double* pr) {
+  if (!s || !positions || !orientations) return fail(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (history && (!pp || !po || !pt || !pr)) return fail(RF_ERR_INVALID_ARGUMENT, "history arrays are NULL");
+  const size_t bytes = 3 * static_cast<size_t>(s->M) * sizeof(double);
+  RF_HIP(hipMemcpyAsync(s->pos[s->cur], positions, bytes, hipMemcpyHostToDevice, s->stream));
+  RF_HIP(hipMemcpyAsync(s->ori[s->cur], orientations, bytes, hipMemcpyHostToDevice, s->stream));
+  if (history) {
+    RF_HIP(hipMemcpyAsync(s->pos[s->prev], pp, bytes, hipMemcpyHostToDevice, s->stream));
+    RF_HIP(hipMemcpyAsync(s->ori[s->prev], po, bytes, hipMemcpyHostToDevice, s->stream));
+    RF_HIP(hipMemcpyAsync(s->tv[s->vprev], pt, bytes, hipMemcpyHostToDevice, s->stream));
+    RF_HIP(hipMemcpyAsync(s->rv[s->vprev], pr, bytes, hipMemcpyHostToDevice, s->stream));
+  }
+  RF_HIP(hipStreamSynchronize(s->stream));
+  s->have_history = history != 0;
+  s->have_next = false;
+  return RF_OK;
+}
+
+rf_status rf_get_system(rf_simulation* s, float* a_matrix, double* b_vector, double* x_vector) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const int64_t n = s->n;
+  if (a_matrix)
+    RF_HIP(hipMemcpy2DAsync(a_matrix, n * sizeof(float), s->A, s->lda * sizeof(float), n * sizeof(float), n,
+                            hipMemcpyDeviceToHost, s->stream));
+  if (b_vector) RF_HIP(hipMemcpyAsync(b_vector, s->b, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  if (x_vector) RF_HIP(hipMemcpyAsync(x_vector, s->x, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
+int64_t rf_system_size(const rf_simulation* s) { return s ? s->n : 0; }
+
+rf_status rf_get_solver_stats(rf_simulation* s, rf_solver_stats* st) {
+  if (!s || !st) return fail(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *st = s->stats;
+  return RF_OK;
+}
+
+rf_status rf_get_timings(rf_simulation* s, rf_timings* t) {
+  if (!s || !t) return fail(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *t = s->timings;
+  return RF_OK;
+}
+
+int64_t rf_device_bytes(const rf_simulation* s) { return s ? s->device_bytes : 0; }
+
+const char* rf_last_error(void) { return g_last_error.c_str(); }
+
+const char* rf_version(void) { return "rigidfibers_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

@@ -1,0 +1,228 @@
+// rf_device.h -- device-side building blocks shared by the rigid-fiber kernels.
+//
+// * Quadrature and Legendre tables are compile-time constants for each
+//   (N, intervals) instantiation, so every table lookup in the unrolled inner
+//   loops becomes an immediate operand (no loads).  Values are computed in FP64
+//   and rounded to FP32 exactly as the reference does
+//   (fortran/fibers.f95:353-440; src/simulation.cu:153-284).
+// * The Stokeslet+doublet kernel K(R) of
+//   fortran/compute_inner_integral_numerically.incl:18-23, written as
+//   K = a I + c R R^T with a = 1/r + 2 eps^2/r^3, c = 1/r^3 - 6 eps^2/r^5.
+// * The moment recurrences of fortran/compute_inner_integral_analytically.incl:12-127,
+//   evaluated in FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rf {
+
+constexpr int kMaxTerms = 8;
+// sqrt(15)/5, the outer node of the 3-point Gauss-Legendre rule.
+constexpr double kGaussNode = 0.77459666924148337703585307995648;
+
+// Monomial coefficients of P_0..P_8 (coef[f][p] multiplies s^p).
+struct LegendreCoef {
+  double c[9][9];
+};
+constexpr LegendreCoef kLeg = {{
+    {1, 0, 0, 0, 0, 0, 0, 0, 0},
+    {0, 1, 0, 0, 0, 0, 0, 0, 0},
+    {-0.5, 0, 1.5, 0, 0, 0, 0, 0, 0},
+    {0, -1.5, 0, 2.5, 0, 0, 0, 0, 0},
+    {3.0 / 8, 0, -30.0 / 8, 0, 35.0 / 8, 0, 0, 0, 0},
+    {0, 15.0 / 8, 0, -70.0 / 8, 0, 63.0 / 8, 0, 0, 0},
+    {-5.0 / 16, 0, 105.0 / 16, 0, -315.0 / 16, 0, 231.0 / 16, 0, 0},
+    {0, -35.0 / 16, 0, 315.0 / 16, 0, -693.0 / 16, 0, 429.0 / 16, 0},
+    {35.0 / 128, 0, -1260.0 / 128, 0, 6930.0 / 128, 0, -12012.0 / 128, 0, 6435.0 / 128},
+}};
+
+__host__ __device__ constexpr double legendre_value(int f, double s) {
+  double acc = 0.0, p = 1.0;
+  for (int k = 0; k <= f; ++k) {
+    acc += kLeg.c[f][k] * p;
+    p *= s;
+  }
+  return acc;
+}
+
+// Composite 3-point Gauss-Legendre rule on NQI equal intervals of [-1, 1] and
+// the weighted Legendre table W[m][q] = w_q * P_m(s_q), m = 0..N.
+template <int N, int NQI>
+struct Quad {
+  static constexpr int Q = 3 * NQI;
+  float s[Q];                 // nodes (FP64 -> FP32, as the reference tables)
+  float w[Q];                 // weights
+  float P[N + 1][Q];          // P_m(s_q), m = 0..N
+  float W[N + 1][Q];          // w_q * P_m(s_q)
+  double sd[Q];               // nodes in FP64 (analytical path)
+  constexpr Quad() : s(), w(), P(), W(), sd() {
+    const double h = 2.0 / NQI;
+    const double gp[3] = {-kGaussNode, 0.0, kGaussNode};
+    const double gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+    for (int iv = 0; iv < NQI; ++iv) {
+      const double lo = -1.0 + iv * h;
+      for (int k = 0; k < 3; ++k) {
+        const double node = (2.0 * lo + h + gp[k] * h) / 2.0;
+        s[3 * iv + k] = static_cast<float>(node);
+        sd[3 * iv + k] = node;
+        w[3 * iv + k] = static_cast<float>(gw[k] / NQI);
+      }
+    }
+    for (int m = 0; m <= N; ++m)
+      for (int q = 0; q < Q; ++q) {
+        P[m][q] = static_cast<float>(legendre_value(m, sd[q]));
+        W[m][q] = w[q] * P[m][q];
+      }
+  }
+};
+
+// Node s_q in FP64 and weight w_q in FP32 for a runtime node index (the
+// per-lane target node), computed exactly as Quad<> does.
+template <int NQI>
+__device__ __forceinline__ double node_fp64(int q) {
+  const double h = 2.0 / NQI;
+  const int iv = q / 3, k = q - 3 * (q / 3);
+  const double lo = -1.0 + iv * h;
+  const double gp = (k == 0) ? -kGaussNode : ((k == 1) ? 0.0 : kGaussNode);
+  return (2.0 * lo + h + gp * h) / 2.0;
+}
+template <int NQI>
+__device__ __forceinline__ float quad_weight_fp32(int q) {
+  const int k = q - 3 * (q / 3);
+  return static_cast<float>(((k == 1) ? 8.0 / 9.0 : 5.0 / 9.0) / NQI);
+}
+
+// Per-run scalars that are not compile-time: d = -ln(eps^2 e), D1, the mode
+// factors gamma_k and eigen_k, eps^2 and F/2 (fortran/fibers.f95:141-149,
+// fortran/assemble_matrix.incl:101).
+struct PhysConsts {
+  float D1;
+  float eps2;
+  float gamma[kMaxTerms];   // gamma_k, k >= 1 (index 0 unused)
+  float eigen[kMaxTerms];
+  float Fh[3];              // F/2
+  double eps2_d;
+  double Fh_d[3];
+  double d;                 // -ln(eps^2 e)
+};
+
+// K(R) components (xx, yy, zz, xy, xz, yz) in FP32.
+__device__ __forceinline__ void stokes_kernel(float rx, float ry, float rz, float eps2, float K[6]) {
+  const float r2 = fmaf(rx, rx, fmaf(ry, ry, rz * rz));
+  const float inv = rsqrtf(r2);
+  const float inv2 = inv * inv;
+  const float inv3 = inv2 * inv;
+  const float a = fmaf(2.0f * eps2, inv3, inv);                 // 1/r + 2eps^2/r^3
+  const float c = inv3 * fmaf(-6.0f * eps2, inv2, 1.0f);        // 1/r^3 - 6eps^2/r^5
+  const float cx = c * rx, cy = c * ry, cz = c * rz;
+  K[0] = fmaf(cx, rx, a);
+  K[1] = fmaf(cy, ry, a);
+  K[2] = fmaf(cz, rz, a);
+  K[3] = cx * ry;
+  K[4] = cx * rz;
+  K[5] = cy * rz;
+}
+
+// Moments I1_n, I3_n, I5_n (n = 0..N+2) of int_{-1}^{1} s^n (s^2 + b s + l)^{-m/2} ds
+// as the reference computes them: closed forms for n = 0, 1 (collinear branch
+// when l - b^2/4 < 1e-7), forward recurrence when l < 10, else a backward
+// recurrence started from zeros at n = 27 keeping n < N + 3.
+template <int N>
+__device__ __forceinline__ void moments(double b, double l, double I1[N + 3], double I3[N + 3], double I5[N + 3]) {
+  const double m = l - 0.25 * b * b;
+  const double uu = sqrt(1.0 + b + l);
+  const double ul = sqrt(1.0 - b + l);
+  const double pu = 2.0 + b, pl = -2.0 + b;
+  I1[0] = log(fabs(pu + 2.0 * uu)) - log(fabs(pl + 2.0 * ul));
+  I1[1] = uu - ul - 0.5 * b * I1[0];
+  if (m < 1e-7) {
+    I3[0] = (-2.0 / (pu * pu)) - (-2.0 / (pl * pl));
+    I5[0] = (-4.0 / (pu * pu * pu * pu)) - (-4.0 / (pl * pl * pl * pl));
+    I5[1] = (-8.0 / (3.0 * pu * pu * pu)) - (-8.0 / (3.0 * pl * pl * pl)) - 0.5 * b * I5[0];
+  } else {
+    const double im = 1.0 / m;
+    const double iu3 = 1.0 / (uu * uu * uu), il3 = 1.0 / (ul * ul * ul);
+    I3[0] = 0.5 * im * (pu / uu - pl / ul);
+    I5[0] = (im / 6.0) * (pu * iu3 - pl * il3) + (2.0 / 3.0) * im * I3[0];
+    I5[1] = (im / 6.0) * (-(b + 2.0 * l) * iu3 + (-b + 2.0 * l) * il3) - (b / 3.0) * im * I3[0];
+  }
+  I3[1] = (-1.0 / uu) + (1.0 / ul) - 0.5 * b * I3[0];
+  if (l < 10.0) {
+#pragma unroll
+    for (int n = 2; n <= N + 2; ++n) {
+      const double rn = n;
+      // s^(n-1) at s = +1 and s = -1
+      const double su = 1.0, sl = (n & 1) ? 1.0 : -1.0;
+      I1[n] = (su * uu - sl * ul) / rn + ((1.0 - 2.0 * rn) * b) / (2.0 * rn) * I1[n - 1] -
+              ((rn - 1.0) * l) / rn * I1[n - 2];
+      I3[n] = I1[n - 2] - b * I3[n - 1] - l * I3[n - 2];
+      I5[n] = I3[n - 2] - b * I5[n - 1] - l * I5[n - 2];
+    }
+  } else {
+    const double il = 1.0 / l;
+    double i1n1 = 0.0, i1n2 = 0.0, i3n1 = 0.0, i3n2 = 0.0, i5n1 = 0.0, i5n2 = 0.0;
+#pragma unroll
+    for (int n = 27; n >= 2; --n) {
+      const double rn = n;
+      // s^(n+1) at s = +1 and s = -1
+      const double sl = ((n + 1) & 1) ? -1.0 : 1.0;
+      const double i1n0 = (rn + 2.0) / (rn + 1.0) * il *
+                          ((uu - sl * ul) / (rn + 2.0) + ((1.0 - 2.0 * (rn + 2.0)) / (2.0 * (rn + 2.0))) * b * i1n1 - i1n2);
+      const double i3n0 = il * (i1n0 - b * i3n1 - i3n2);
+      const double i5n0 = il * (i3n0 - b * i5n1 - i5n2);
+      if (n < N + 3) {
+        I1[n] = i1n0;
+        I3[n] = i3n0;
+        I5[n] = i5n0;
+      }
+      i1n2 = i1n1;
+      i1n1 = i1n0;
+      i3n2 = i3n1;
+      i3n1 = i3n0;
+      i5n2 = i5n1;
+      i5n1 = i5n0;
+    }
+  }
+}
+
+// G_f = L01 I + L03 R0R0^T - L13 (t R0^T + R0 t^T) + L23 t t^T
+//     + 2 eps^2 (L03 I - 3 L05 R0R0^T + 3 L15 (t R0^T + R0 t^T) - 3 L25 t t^T)
+// (fortran/compute_inner_integral_analytically.incl:207-260), FP64.
+template <int F>
+__device__ __forceinline__ void analytic_g(const double R0[3], const double t[3], double eps2, const double* I1,
+                                           const double* I3, const double* I5, double G[6]) {
+  double L01 = 0, L03 = 0, L05 = 0, L13 = 0, L15 = 0, L23 = 0, L25 = 0;
+#pragma unroll
+  for (int p = 0; p <= F; ++p) {
+    constexpr LegendreCoef lc = kLeg;
+    const double cf = lc.c[F][p];
+    if (cf != 0.0) {
+      L01 += cf * I1[p];
+      L03 += cf * I3[p];
+      L05 += cf * I5[p];
+      L13 += cf * I3[p + 1];
+      L15 += cf * I5[p + 1];
+      L23 += cf * I3[p + 2];
+      L25 += cf * I5[p + 2];
+    }
+  }
+  const double two_e2 = 2.0 * eps2;
+  const double iso = L01 + two_e2 * L03;           // coefficient of I
+  const double crr = L03 - 3.0 * two_e2 * L05;     // of R0 R0^T
+  const double ctr = -L13 + 3.0 * two_e2 * L15;    // of (t R0^T + R0 t^T)
+  const double ctt = L23 - 3.0 * two_e2 * L25;     // of t t^T
+  constexpr int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int a = ia[c], bb = ib[c];
+    G[c] = (a == bb ? iso : 0.0) + crr * R0[a] * R0[bb] + ctr * (t[a] * R0[bb] + R0[a] * t[bb]) + ctt * t[a] * t[bb];
+  }
+}
+
+// Component index of the symmetric 3x3 pair (a, b) in (xx, yy, zz, xy, xz, yz).
+__host__ __device__ constexpr int sym_index(int a, int b) {
+  return (a == b) ? a : ((a + b == 1) ? 3 : ((a + b == 2) ? 4 : 5));
+}
+
+}  // namespace rf

@@ -1,0 +1,229 @@
+// rf_linalg.hip -- HBM-streaming GEMV and the vector kernels of the Krylov
+// solvers.
+//
+// Reference: the GMRES/BiCGStab matvec is ViennaCL's vec_mul_col_kernel
+// (lib/viennacl/linalg/cuda/matrix_operations_col.hpp:1251-1278, one thread per
+// row over a column-major FP32 matrix) and sgemv in fortran/solve_system.incl:36.
+//
+// gemv: A is column-major FP32 with a leading dimension lda that is a multiple
+// of 64, so every column starts 256-byte aligned.  A workgroup owns 1024 rows
+// (4 per lane, one 16-byte load per lane per column: a wave reads 1 KiB
+// contiguous per column) and a chunk of columns; x is read with scalar loads.
+// Products accumulate in FP64 (the FP64 VALU rate is far above what 8 TB/s of
+// FP32 matrix data can feed), chunks are summed afterwards in a fixed order.
+#include "rf_internal.h"
+
+namespace rf {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kGemvThreads = 256;
+constexpr int kGemvRows = 4 * kGemvThreads;
+
+__global__ __launch_bounds__(kGemvThreads) void gemv_f32_kernel(int64_t n, int64_t lda, const float* __restrict__ A,
+                                                                const double* __restrict__ x,
+                                                                double* __restrict__ part, int cols_per_chunk) {
+  const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kGemvRows + 4 * threadIdx.x;
+  const int64_t c0 = static_cast<int64_t>(blockIdx.y) * cols_per_chunk;
+  int64_t c1 = c0 + cols_per_chunk;
+  if (c1 > n) c1 = n;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (r0 < n) {
+    // rows r0..r0+3 < lda (lda % 4 == 0), so the 16-byte load stays inside the column
+    const float* p = A + c0 * lda + r0;
+    int64_t c = c0;
+    for (; c + 4 <= c1; c += 4) {
+      const f4 v0 = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+      const f4 v1 = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p + lda));
+      const f4 v2 = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p + 2 * lda));
+      const f4 v3 = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p + 3 * lda));
+      const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2], x3 = x[c + 3];
+      a0 = fma(static_cast<double>(v0.x), x0, a0);
+      a1 = fma(static_cast<double>(v0.y), x0, a1);
+      a2 = fma(static_cast<double>(v0.z), x0, a2);
+      a3 = fma(static_cast<double>(v0.w), x0, a3);
+      a0 = fma(static_cast<double>(v1.x), x1, a0);
+      a1 = fma(static_cast<double>(v1.y), x1, a1);
+      a2 = fma(static_cast<double>(v1.z), x1, a2);
+      a3 = fma(static_cast<double>(v1.w), x1, a3);
+      a0 = fma(static_cast<double>(v2.x), x2, a0);
+      a1 = fma(static_cast<double>(v2.y), x2, a1);
+      a2 = fma(static_cast<double>(v2.z), x2, a2);
+      a3 = fma(static_cast<double>(v2.w), x2, a3);
+      a0 = fma(static_cast<double>(v3.x), x3, a0);
+      a1 = fma(static_cast<double>(v3.y), x3, a1);
+      a2 = fma(static_cast<double>(v3.z), x3, a2);
+      a3 = fma(static_cast<double>(v3.w), x3, a3);
+      p += 4 * lda;
+    }
+    for (; c < c1; ++c) {
+      const f4 v = __builtin_nontemporal_load(reinterpret_cast<const f4*>(p));
+      const double xc = x[c];
+      a0 = fma(static_cast<double>(v.x), xc, a0);
+      a1 = fma(static_cast<double>(v.y), xc, a1);
+      a2 = fma(static_cast<double>(v.z), xc, a2);
+      a3 = fma(static_cast<double>(v.w), xc, a3);
+      p += lda;
+    }
+    double* out = part + static_cast<int64_t>(blockIdx.y) * n + r0;
+    if (r0 + 3 < n) {
+      out[0] = a0;
+      out[1] = a1;
+      out[2] = a2;
+      out[3] = a3;
+    } else {
+      out[0] = a0;
+      if (r0 + 1 < n) out[1] = a1;
+      if (r0 + 2 < n) out[2] = a2;
+    }
+  }
+}
+
+// y[r] = sum_k part[k][r]  (mode 0)   or   y[r] = b[r] - sum_k part[k][r]  (mode 1)
+__global__ void gemv_reduce_kernel(int64_t n, int chunks, const double* __restrict__ part, double* __restrict__ y,
+                                   const double* __restrict__ b, int mode) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < chunks; ++k) s += part[static_cast<int64_t>(k) * n + r];
+  y[r] = (mode == 1) ? b[r] - s : s;
+}
+
+GemvPlan gemv_plan(int64_t n, int64_t lda) {
+  GemvPlan p;
+  p.n = n;
+  p.lda = lda;
+  p.row_tiles = static_cast<int>((n + kGemvRows - 1) / kGemvRows);
+  // ~2048 workgroups over 256 CUs; at least 64 columns per chunk.
+  int64_t chunks = (2048 + p.row_tiles - 1) / p.row_tiles;
+  const int64_t max_chunks = (n + 63) / 64;
+  if (chunks > max_chunks) chunks = max_chunks;
+  if (chunks < 1) chunks = 1;
+  p.cols_per_chunk = static_cast<int>((n + chunks - 1) / chunks);
+  p.chunks = static_cast<int>((n + p.cols_per_chunk - 1) / p.cols_per_chunk);
+  return p;
+}
+
+size_t gemv_partial_elems(const GemvPlan& p) { return static_cast<size_t>(p.chunks) * static_cast<size_t>(p.n); }
+
+hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part, double* y, const double* b,
+                int mode, hipStream_t s) {
+  hipLaunchKernelGGL(gemv_f32_kernel, dim3(p.row_tiles, p.chunks), dim3(kGemvThreads), 0, s, p.n, p.lda, A, x, part,
+                     p.cols_per_chunk);
+  hipLaunchKernelGGL(gemv_reduce_kernel, dim3(static_cast<unsigned>((p.n + 255) / 256)), dim3(256), 0, s, p.n,
+                     p.chunks, part, y, b, mode);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Multi-dot: out[k] = <V_k, w>, k < K.  Grid (blocks, K); block (b, k) reduces
+// rows [b*rpb, (b+1)*rpb) of vector k; a second kernel sums the blocks.
+
+constexpr int kDotThreads = 256;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) t += sh[w];
+  return t;
+}
+
+int mdot_blocks(int64_t n) {
+  int64_t b = (n + 2047) / 2048;
+  if (b > 256) b = 256;
+  if (b < 1) b = 1;
+  return static_cast<int>(b);
+}
+
+__global__ __launch_bounds__(kDotThreads) void mdot_partial_kernel(int64_t n, int64_t rpb,
+                                                                   const double* __restrict__ V, int64_t ldv,
+                                                                   const double* __restrict__ w,
+                                                                   double* __restrict__ part) {
+  __shared__ double sh[kDotThreads / 64];
+  const int k = blockIdx.y;
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * rpb;
+  int64_t hi = lo + rpb;
+  if (hi > n) hi = n;
+  const double* v = V + static_cast<int64_t>(k) * ldv;
+  double s = 0.0;
+  for (int64_t r = lo + threadIdx.x; r < hi; r += kDotThreads) s = fma(v[r], w[r], s);
+  const double t = block_sum(s, sh);
+  if (threadIdx.x == 0) part[static_cast<int64_t>(k) * gridDim.x + blockIdx.x] = t;
+}
+
+__global__ void mdot_final_kernel(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) s += part[static_cast<int64_t>(k) * nb + b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (threadIdx.x == 0) out[k] = s;
+}
+
+hipError_t mdot(int64_t n, int K, const double* V, int64_t ldv, const double* w, double* part, double* out,
+                hipStream_t s) {
+  if (K <= 0) return hipSuccess;
+  const int nb = mdot_blocks(n);
+  const int64_t rpb = (n + nb - 1) / nb;
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, K), dim3(kDotThreads), 0, s, n, rpb, V, ldv, w, part);
+  hipLaunchKernelGGL(mdot_final_kernel, dim3(K), dim3(64), 0, s, nb, part, out);
+  return hipGetLastError();
+}
+
+__global__ void axpy_multi_kernel(int64_t n, int K, const double* __restrict__ V, int64_t ldv,
+                                  const double* __restrict__ h, double alpha, double* __restrict__ w) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < K; ++k) s = fma(V[static_cast<int64_t>(k) * ldv + r], h[k], s);
+  w[r] = fma(alpha, s, w[r]);
+}
+
+hipError_t axpy_multi(int64_t n, int K, const double* V, int64_t ldv, const double* h, double alpha, double* w,
+                      hipStream_t s) {
+  if (K <= 0) return hipSuccess;
+  hipLaunchKernelGGL(axpy_multi_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, K, V, ldv,
+                     h, alpha, w);
+  return hipGetLastError();
+}
+
+__global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__ x, double beta,
+                             double* __restrict__ y) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  y[r] = (beta == 0.0) ? alpha * x[r] : fma(alpha, x[r], beta * y[r]);
+}
+
+hipError_t axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(axpby_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, alpha, x, beta,
+                     y);
+  return hipGetLastError();
+}
+
+__global__ void fill_kernel(int64_t n, double v, double* __restrict__ y) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < n) y[r] = v;
+}
+
+hipError_t fill(int64_t n, double value, double* y, hipStream_t s) {
+  hipLaunchKernelGGL(fill_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, value, y);
+  return hipGetLastError();
+}
+
+__global__ void nonfinite_kernel(int64_t n, const double* __restrict__ x, int* __restrict__ flag) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < n && !isfinite(x[r])) flag[0] = 1;  // benign race: any writer stores 1
+}
+
+hipError_t nonfinite_count(int64_t n, const double* x, int* flag_dev, hipStream_t s) {
+  hipLaunchKernelGGL(nonfinite_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, x,
+                     flag_dev);
+  return hipGetLastError();
+}
+
+}  // namespace rf

@@ -1,0 +1,219 @@
+"""GPU parity: the HIP path (through the C ABI) against the reference's golden
+vectors and the FP64 oracle.
+
+Tolerances are norm-wise relative errors (max |gpu - ref| / max |ref|):
+  * 1e-5 for the state (positions, orientations) -- the FP32 drop-in bar;
+  * 1e-5 for b, x and the velocities (FP32 assembly of a system with
+    condition number ~4.5; observed errors are ~1e-7).
+"""
+import numpy as np
+import pytest
+
+from helpers import FIELDS, VARIANTS, golden, random_scene, ref100, ref_config, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _sim(params, pos, ori, **kw):
+    from rigidfibers_amd.simulation import Simulation
+
+    return Simulation(params, pos, ori, **kw)
+
+
+def _variant_kwargs(variant):
+    integration = "analytical" if variant.startswith("analytic") else "numerical"
+    solver = "direct" if variant.endswith("direct") else "gmres"
+    return dict(solver=solver, integration=integration)
+
+
+def _run_recorded(sim, steps):
+    out = []
+    for t in range(steps):
+        sim.assemble_system()
+        _, b, _ = sim.system()
+        sim.solve_system()
+        _, _, x = sim.system()
+        sim.update_velocities()
+        v, w = sim.velocities()
+        sim.update_fibers(t == 0)
+        p, o = sim.next_state()
+        sim.advance()
+        out.append({"BVec": b, "XVec": x, "TRANSVel": v, "ROTVel": w, "POS": p, "ORIENT": o})
+    return out
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_golden_vectors(variant):
+    """Two validated timesteps of the reference's M=100 scene, every dumped
+    quantity against tests/reference/100_<variant> (runner/cuda_runner.py:165-215)."""
+    M, pos, ori = ref100()
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    with _sim(params, pos, ori, **_variant_kwargs(variant)) as sim:
+        rec = _run_recorded(sim, 2)
+        stats = sim.solver_stats()
+    worst = {}
+    for step in range(2):
+        for f in FIELDS:
+            e = rel_err(rec[step][f], golden(variant, step, f))
+            worst[f"{step}_{f}"] = e
+    print(variant, {k: f"{v:.1e}" for k, v in worst.items()}, stats)
+    for k, e in worst.items():
+        assert e < TOL, f"{variant} {k}: relative error {e:.2e}"
+
+
+def test_teacher_forced_second_step():
+    """Step 1 started from the golden step-0 state (multistep history set
+    through rf_set_state) reproduces the golden step-1 vectors."""
+    variant = "numeric_direct"
+    M, pos0, ori0 = ref100()
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    p1, o1 = golden(variant, 0, "POS"), golden(variant, 0, "ORIENT")
+    v0, w0 = golden(variant, 0, "TRANSVel"), golden(variant, 0, "ROTVel")
+    with _sim(params, pos0, ori0, **_variant_kwargs(variant)) as sim:
+        sim.set_state(p1, o1, previous=(pos0, ori0, v0, w0))
+        sim.step(1)
+        p, o = sim.state()
+        v, w = sim.velocities()
+    for f, a in (("POS", p), ("ORIENT", o), ("TRANSVel", v), ("ROTVel", w)):
+        assert rel_err(a, golden(variant, 1, f)) < TOL, f
+
+
+@pytest.mark.parametrize("N,nqi,analytical", [(5, 8, False), (5, 8, True), (1, 8, False), (3, 4, True),
+                                              (8, 16, False), (8, 8, True), (2, 16, True), (6, 4, False)])
+def test_assembly_matches_oracle(oracle64, oracle32, N, nqi, analytical):
+    """A entrywise and b against the FP64 oracle for other expansion orders and
+    quadrature sizes (random 20-fiber scene with near pairs).  The bound is the
+    larger of 2e-6 and 3x the error the reference's own FP32 arithmetic (the
+    FP32 build of the oracle) makes on the same scene: near-touching fibers
+    amplify FP32 rounding in the numerical integrals, and the reference's FP32
+    analytical recurrences can lose every digit (the path here uses FP64)."""
+    from oracle.pyoracle import make_params
+
+    M = 20
+    pos, ori = random_scene(M, side=3.0, seed=N * 100 + nqi, min_dist=0.3)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.NUMBER_OF_TERMS_IN_FORCE_EXPANSION = N
+    params.NUMBER_OF_QUADRATURE_INTERVALS = nqi
+    with _sim(params, pos, ori, solver="gmres", integration="analytical" if analytical else "numerical") as sim:
+        sim.assemble_system()
+        A, b, _ = sim.system(with_matrix=True)
+    p = make_params(M, N=N, nqi=nqi)
+    A_ref, b_ref = oracle64.assemble(p, pos, ori, analytical)
+    A_32, b_32 = oracle32.assemble(p, pos, ori, analytical)
+    ea, eb = rel_err(A, A_ref), rel_err(b, b_ref)
+    ra, rb = rel_err(A_32, A_ref), rel_err(b_32, b_ref)
+    print(f"N={N} nqi={nqi} analytical={analytical}: A {ea:.1e} (ref FP32 {ra:.1e}) b {eb:.1e} (ref FP32 {rb:.1e})")
+    assert ea < max(2e-6, 3 * ra) and eb < max(2e-6, 3 * rb)
+    if analytical:  # FP64 recurrences: no FP32 blow-up
+        assert ea < 2e-6 and eb < 2e-6
+
+
+@pytest.mark.parametrize("analytical", [False, True])
+def test_velocities_match_oracle(oracle64, analytical):
+    """Velocities from the GPU's own solution against the oracle's velocity
+    routine fed the same solution (isolates the velocity kernel)."""
+    from oracle.pyoracle import make_params
+
+    M = 40
+    pos, ori = random_scene(M, side=5.0, seed=11, min_dist=0.4)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-10
+    with _sim(params, pos, ori, solver="gmres", integration="analytical" if analytical else "numerical") as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        _, _, x = sim.system()
+        sim.update_velocities()
+        v, w = sim.velocities()
+    v_ref, w_ref = oracle64.velocities(make_params(M), pos, ori, x, analytical)
+    assert rel_err(v, v_ref) < 1e-6 and rel_err(w, w_ref) < 1e-6
+
+
+def test_single_fiber_known_answer():
+    """M = 1: A = I, b = 0, x = 0, v = (0.5/d)((d+2) F + (d-2)(t.F) t), w = 0
+    (src/kernels/reset_velocities.cu:22-38)."""
+    t = np.array([0.3, -0.4, 0.5])
+    t /= np.linalg.norm(t)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = 1
+    with _sim(params, [1.0, 2.0, 3.0], t) as sim:
+        sim.step(0)
+        A, b, x = sim.system(with_matrix=True)
+        v, w = sim.velocities()
+        p, o = sim.state()
+    d = -np.log(0.01 ** 2 * np.e)
+    F = np.array([0.0, 0.0, -1.0])
+    v_ref = (0.5 / d) * ((d + 2) * F + (d - 2) * (t @ F) * t)
+    assert np.array_equal(A, np.eye(15, dtype=np.float32))
+    assert np.all(b == 0) and np.all(x == 0)
+    assert np.allclose(v, v_ref, rtol=1e-12, atol=1e-14) and np.all(w == 0)
+    assert np.allclose(p, np.array([1.0, 2.0, 3.0]) + 0.1 * v_ref, rtol=1e-12)
+    assert np.allclose(o, t, rtol=1e-12)
+
+
+def test_ring_symmetry():
+    """16 vertical fibers on a ring (the reference's tests/XcT_ring16.in): by
+    symmetry every fiber has the same speed and the same vertical velocity."""
+    import os
+
+    from helpers import GOLDEN
+    from rigidfibers_amd.config import read_fibers
+
+    M, pos, ori = read_fibers(os.path.join(GOLDEN, "XcT_ring16.in"))
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-10
+    with _sim(params, pos, ori) as sim:
+        sim.step(0)
+        v, w = sim.velocities()
+    v = v.reshape(M, 3)
+    speed = np.linalg.norm(v, axis=1)
+    assert np.ptp(speed) < 1e-9 * speed.max()
+    assert np.ptp(v[:, 2]) < 1e-9 * np.abs(v[:, 2]).max()
+
+
+@pytest.mark.parametrize("solver", ["gmres", "bicgstab", "direct"])
+def test_solvers_agree_with_oracle(oracle64, solver):
+    """x of each solver against the oracle's LU solution of the oracle's system."""
+    from oracle.pyoracle import make_params
+
+    M = 60
+    pos, ori = random_scene(M, side=5.0, seed=3, min_dist=0.4)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-9
+    params.BICGSTAB_TOLERANCE = 1e-9
+    with _sim(params, pos, ori, solver=solver) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        _, _, x = sim.system()
+        st = sim.solver_stats()
+    A, b = oracle64.assemble(make_params(M), pos, ori)
+    x_ref = oracle64.solve_direct(A, b)
+    print(solver, st, rel_err(x, x_ref))
+    assert st["converged"]
+    assert rel_err(x, x_ref) < 1e-6
+
+
+def test_ten_steps_against_oracle(oracle64):
+    """Free-running 10 steps of the reference scene: positions/orientations
+    within 1e-5 of the FP64 oracle (GMRES at 1e-8 vs oracle LU)."""
+    from oracle.pyoracle import make_params
+
+    M, pos, ori = ref100()
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-8
+    with _sim(params, pos, ori) as sim:
+        for t in range(10):
+            sim.step(t)
+        p, o = sim.state()
+    p_ref, o_ref = oracle64.simulate(make_params(M), pos, ori, 10, solver="direct")
+    ep, eo = rel_err(p, p_ref), rel_err(o, o_ref)
+    print(f"10 steps: pos {ep:.1e} ori {eo:.1e}")
+    assert ep < 1e-5 and eo < 1e-5
