@@ -110,6 +110,9 @@ typedef struct rf_timings {
   double update_velocities;
   double update_fibers;
   double total;
+  double gemv;             /* summed duration of the matrix-vector kernels of the solve */
+  int32_t gemv_launches;   /* how many there were */
+  int32_t reserved;
 } rf_timings;
 
 typedef struct rf_simulation rf_simulation;

@@ -67,6 +67,10 @@ class Oracle:
         lib.rfo_moments.restype = None
         lib.rfo_stokes_kernel.argtypes = [P, C.c_double, P]
         lib.rfo_stokes_kernel.restype = None
+        lib.rfo_assemble_rows.argtypes = [C.POINTER(Params), C.c_int, P, P, C.c_int, C.c_int, P, P]
+        lib.rfo_velocities_rows.argtypes = [C.POINTER(Params), C.c_int, P, P, P, C.c_int, C.c_int, P, P]
+        lib.rfo_matvec_rows.argtypes = [C.c_int, C.c_int, P, P, P]
+        lib.rfo_matvec_rows.restype = None
         assert lib.rfo_real_size() == (8 if bits == 64 else 4)
 
     # -- helpers -------------------------------------------------------------
@@ -144,6 +148,34 @@ class Oracle:
         pn, on = np.zeros_like(arrs[1]), np.zeros_like(arrs[3])
         self.lib.rfo_update(C.byref(params), *(self._ptr(a) for a in arrs + [pn, on]))
         return pn, on
+
+    # -- row samples (CPU-baseline timing) ---------------------------------------
+    def assemble_rows(self, params, pos, ori, i0, i1, analytical=False):
+        n = 3 * params.N * params.M
+        rows = 3 * params.N * (i1 - i0)
+        pos, ori = self._arr(pos), self._arr(ori)
+        A = np.zeros((rows, n), dtype=self.dtype, order="F")
+        b = np.zeros(rows, dtype=self.dtype)
+        if self.lib.rfo_assemble_rows(C.byref(params), int(bool(analytical)), self._ptr(pos), self._ptr(ori), i0, i1,
+                                      self._ptr(A), self._ptr(b)):
+            raise ValueError("rfo_assemble_rows failed")
+        return A, b
+
+    def velocities_rows(self, params, pos, ori, x, i0, i1, analytical=False):
+        pos, ori, x = self._arr(pos), self._arr(ori), self._arr(x)
+        v = np.zeros(3 * (i1 - i0), dtype=self.dtype)
+        w = np.zeros_like(v)
+        if self.lib.rfo_velocities_rows(C.byref(params), int(bool(analytical)), self._ptr(pos), self._ptr(ori),
+                                        self._ptr(x), i0, i1, self._ptr(v), self._ptr(w)):
+            raise ValueError("rfo_velocities_rows failed")
+        return v, w
+
+    def matvec_rows(self, A_rows, x):
+        A_rows = np.asfortranarray(A_rows, dtype=self.dtype)
+        x = self._arr(x)
+        y = np.zeros(A_rows.shape[0], dtype=self.dtype)
+        self.lib.rfo_matvec_rows(A_rows.shape[0], A_rows.shape[1], self._ptr(A_rows), self._ptr(x), self._ptr(y))
+        return y
 
     def moments(self, b, l, nfwd, nkeep):
         I1 = np.zeros(nkeep + 4, dtype=self.dtype)

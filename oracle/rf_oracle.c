@@ -330,22 +330,27 @@ static void inner_analytical(const tables_t *t, const real xi[3], const real ti[
   }
 }
 
-/* System assembly, fortran/assemble_matrix.incl:1-161. A is column-major. */
-int rfo_assemble(const rfo_params *p, int analytical, const real *pos, const real *ori, real *A, real *bvec) {
+/* System assembly, fortran/assemble_matrix.incl:1-161, for the rows of the
+ * target fibers [i0, i1): A holds those 3N(i1-i0) rows (leading dimension
+ * lda, all n columns, column-major), bvec the same rows of b. */
+static int assemble_rows(const rfo_params *p, int analytical, const real *pos, const real *ori, int i0, int i1,
+                         real *A, size_t lda, real *bvec) {
   tables_t t;
   if (make_tables(p, &t)) return -1;
   const int M = p->M, N = p->N, Q = t.Q;
   const size_t n = (size_t)3 * N * M;
+  const size_t roff = (size_t)3 * N * i0, nrows = (size_t)3 * N * (i1 - i0);
+  if (i0 < 0 || i1 > M || i0 > i1 || lda < nrows) return -1;
   set_threads(p);
-  memset(A, 0, n * n * sizeof(real));
-  memset(bvec, 0, n * sizeof(real));
+  for (size_t c = 0; c < n; ++c) memset(A + c * lda, 0, nrows * sizeof(real));
+  memset(bvec, 0, nrows * sizeof(real));
 #pragma omp parallel for schedule(dynamic, 1)
-  for (int i = 0; i < M; ++i) {
+  for (int i = i0; i < i1; ++i) {
     real G[RFO_MAX_Q * 6], GF[RFO_MAX_Q * 3];
     const real *xi = &pos[3 * i], *ti = &ori[3 * i];
     for (int j = 0; j < M; ++j) {
       if (i == j) {
-        for (size_t r = (size_t)3 * N * i; r < (size_t)3 * N * (i + 1); ++r) A[r + r * n] = 1;
+        for (size_t r = (size_t)3 * N * i; r < (size_t)3 * N * (i + 1); ++r) A[(r - roff) + r * lda] = 1;
         continue;
       }
       const real *xj = &pos[3 * j], *tj = &ori[3 * j];
@@ -367,16 +372,16 @@ int rfo_assemble(const rfo_params *p, int analytical, const real *pos, const rea
                               T[3] * ti[0] + T[1] * ti[1] + T[5] * ti[2],
                               T[4] * ti[0] + T[5] * ti[1] + T[2] * ti[2]};
           const real Tm[3][3] = {{T[0], T[3], T[4]}, {T[3], T[1], T[5]}, {T[4], T[5], T[2]}};
-          const size_t row0 = (size_t)3 * N * i + 3 * ki;
+          const size_t row0 = (size_t)3 * N * i + 3 * ki - roff;
           const size_t col0 = (size_t)3 * N * j + 3 * kj;
           if (ki == 0) {
             for (int a = 0; a < 3; ++a)
-              for (int bb = 0; bb < 3; ++bb) A[(row0 + a) + (col0 + bb) * n] = t.D1 * ti[a] * Qv[bb];
+              for (int bb = 0; bb < 3; ++bb) A[(row0 + a) + (col0 + bb) * lda] = t.D1 * ti[a] * Qv[bb];
           } else {
             const real gamma = (real)0.5 * ((real)2.0 * (ki + 1) + (real)1.0) / (t.d + t.e - t.lambda[ki]);
             for (int a = 0; a < 3; ++a)
               for (int bb = 0; bb < 3; ++bb)
-                A[(row0 + a) + (col0 + bb) * n] = gamma * (Tm[a][bb] - t.eigen[ki] * ti[a] * Qv[bb]);
+                A[(row0 + a) + (col0 + bb) * lda] = gamma * (Tm[a][bb] - t.eigen[ki] * ti[a] * Qv[bb]);
           }
           if (kj == 0) {
             const real QF = TF[0] * ti[0] + TF[1] * ti[1] + TF[2] * ti[2];
@@ -392,6 +397,38 @@ int rfo_assemble(const rfo_params *p, int analytical, const real *pos, const rea
     }
   }
   return 0;
+}
+
+int rfo_assemble(const rfo_params *p, int analytical, const real *pos, const real *ori, real *A, real *bvec) {
+  const size_t n = (size_t)3 * p->N * p->M;
+  return assemble_rows(p, analytical, pos, ori, 0, p->M, A, n, bvec);
+}
+
+/* Rows of the target fibers [i0, i1) only (a bounded sample for timing). */
+int rfo_assemble_rows(const rfo_params *p, int analytical, const real *pos, const real *ori, int i0, int i1,
+                      real *A, real *bvec) {
+  return assemble_rows(p, analytical, pos, ori, i0, i1, A, (size_t)3 * p->N * (i1 - i0), bvec);
+}
+
+/* y = A_rows x for a block of rows stored column-major (nrows x n), swept
+ * column by column as sgemv('N') does (fortran/solve_system.incl:36). */
+void rfo_matvec_rows(int nrows, int n_, const real *A, const real *x, real *y) {
+  const size_t n = (size_t)n_, nr = (size_t)nrows;
+  for (size_t r = 0; r < nr; ++r) y[r] = 0;
+#pragma omp parallel
+  {
+#ifdef _OPENMP
+    const int nt = omp_get_num_threads(), id = omp_get_thread_num();
+#else
+    const int nt = 1, id = 0;
+#endif
+    const size_t lo = nr * id / nt, hi = nr * (id + 1) / nt;
+    for (size_t c = 0; c < n; ++c) {
+      const real xc = x[c];
+      const real *col = A + c * nr;
+      for (size_t r = lo; r < hi; ++r) y[r] += col[r] * xc;
+    }
+  }
 }
 
 /* Dense LU with partial pivoting and the two triangular solves (the semantics
@@ -600,14 +637,16 @@ static void velocity_inner_analytical(const tables_t *t, const real R0[3], const
   }
 }
 
-int rfo_velocities(const rfo_params *p, int analytical, const real *pos, const real *ori, const real *x, real *v,
-                   real *w) {
+/* Velocities of the target fibers [i0, i1) (v, w hold 3(i1-i0) values). */
+int rfo_velocities_rows(const rfo_params *p, int analytical, const real *pos, const real *ori, const real *x,
+                        int i0, int i1, real *v, real *w) {
   tables_t t;
   if (make_tables(p, &t)) return -1;
   const int M = p->M, N = p->N, Q = t.Q;
+  if (i0 < 0 || i1 > M || i0 > i1) return -1;
   set_threads(p);
 #pragma omp parallel for schedule(dynamic, 1)
-  for (int i = 0; i < M; ++i) {
+  for (int i = i0; i < i1; ++i) {
     const real *xi = &pos[3 * i], *ti = &ori[3 * i];
     real F2[3], vi[3], wi[3] = {0, 0, 0};
     for (int a = 0; a < 3; ++a) F2[a] = (real)2.0 * t.Fh[a];
@@ -657,11 +696,16 @@ int rfo_velocities(const rfo_params *p, int analytical, const real *pos, const r
       for (int a = 0; a < 3; ++a) wi[a] += A1[a] - ti[a] * tA1;
     }
     for (int a = 0; a < 3; ++a) {
-      v[3 * i + a] = vi[a] * ((real)0.5 / t.d);
-      w[3 * i + a] = wi[a] * ((real)1.5 / t.d);
+      v[3 * (i - i0) + a] = vi[a] * ((real)0.5 / t.d);
+      w[3 * (i - i0) + a] = wi[a] * ((real)1.5 / t.d);
     }
   }
   return 0;
+}
+
+int rfo_velocities(const rfo_params *p, int analytical, const real *pos, const real *ori, const real *x, real *v,
+                   real *w) {
+  return rfo_velocities_rows(p, analytical, pos, ori, x, 0, p->M, v, w);
 }
 
 static void normalize3(real *o) {

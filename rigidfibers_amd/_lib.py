@@ -69,6 +69,9 @@ class RfTimings(C.Structure):
         ("update_velocities", C.c_double),
         ("update_fibers", C.c_double),
         ("total", C.c_double),
+        ("gemv", C.c_double),
+        ("gemv_launches", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -92,6 +92,10 @@ struct rf_simulation {
 
   rf_solver_stats stats{};
   rf_timings timings{};
+  // event pairs around the matvec kernels of one solve (timings.gemv)
+  static constexpr int kGemvEvents = 256;
+  hipEvent_t gev[2 * kGemvEvents] = {};
+  int ngemv = 0;
   int64_t device_bytes = 0;
   std::vector<void*> allocations;
 
@@ -180,6 +184,18 @@ void make_consts(const rf_config& c, rf::PhysConsts* pc) {
 
 int solver_kdim(const rf_config& c) { return (c.solver == RF_SOLVER_GMRES) ? c.gmres_restart + 1 : 0; }
 
+// y = A x or y = b - A x on the simulation's stream, timed with an event pair
+// while the pool lasts.
+hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, const double* b, int mode) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s->ngemv < rf_simulation::kGemvEvents) {
+    e0 = s->gev[2 * s->ngemv];
+    e1 = s->gev[2 * s->ngemv + 1];
+    ++s->ngemv;
+  }
+  return rf::gemv(s->plan, s->A, x, part, y, b, mode, s->stream, e0, e1);
+}
+
 rf_status check_nonfinite(rf_simulation* s, const double* v, int64_t len, const char* what) {
   RF_HIP(hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
   RF_HIP(rf::nonfinite_count(len, v, s->flag, s->stream));
@@ -240,7 +256,7 @@ rf_status solve_gmres(rf_simulation* s) {
       first = false;
     } else {
       first = false;
-      RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, Vk(0), s->b, 1, s->stream));
+      RF_HIP(sim_gemv(s, s->x, s->gpart, Vk(0), s->b, 1));
       RF_TRY(device_norm(s, Vk(0), &beta));
     }
     res = beta;
@@ -257,7 +273,7 @@ rf_status solve_gmres(rf_simulation* s) {
     for (int k = 0; k < m && it < maxit; ++k) {
       ++it;
       double* w = Vk(k + 1);
-      RF_HIP(rf::gemv(s->plan, s->A, Vk(k), s->gpart, w, nullptr, 0, s->stream));
+      RF_HIP(sim_gemv(s, Vk(k), s->gpart, w, nullptr, 0));
       // h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||^2
       double* h1 = s->hdev;
       double* h2 = s->hdev + (m + 1);
@@ -330,7 +346,7 @@ rf_status solve_bicgstab(rf_simulation* s) {
   }
   double rho, res;
   if (warm) {
-    RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, r, s->b, 1, s->stream));
+    RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
     RF_TRY(device_norm(s, r, &res));
     rho = res * res;
   } else {
@@ -344,7 +360,7 @@ rf_status solve_bicgstab(rf_simulation* s) {
   bool converged = res / bnorm <= tol;
   while (it < maxit && !converged) {
     ++it;
-    RF_HIP(rf::gemv(s->plan, s->A, p, s->gpart, v, nullptr, 0, s->stream));
+    RF_HIP(sim_gemv(s, p, s->gpart, v, nullptr, 0));
     double r0v;
     RF_TRY(device_dot(s, r0, v, &r0v));
     if (r0v == 0.0 || !std::isfinite(r0v)) break;
@@ -360,7 +376,7 @@ rf_status solve_bicgstab(rf_simulation* s) {
       converged = true;
       break;
     }
-    RF_HIP(rf::gemv(s->plan, s->A, sv, s->gpart, t, nullptr, 0, s->stream));
+    RF_HIP(sim_gemv(s, sv, s->gpart, t, nullptr, 0));
     double ts, tt;
     RF_TRY(device_dot(s, t, sv, &ts));
     RF_TRY(device_dot(s, t, t, &tt));
@@ -384,7 +400,7 @@ rf_status solve_bicgstab(rf_simulation* s) {
     RF_HIP(rf::axpby(n, 1.0, r, beta, p, s->stream));
   }
   // report the true residual of the returned x
-  RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, r, s->b, 1, s->stream));
+  RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
   RF_TRY(device_norm(s, r, &res));
   s->stats.iterations = it;
   s->stats.converged = (converged && res / bnorm <= 10.0 * tol) ? 1 : 0;
@@ -475,6 +491,8 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     return cleanup(fail(RF_ERR_DEVICE, "hipStreamCreate failed"));
   for (auto& e : s->ev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
+  for (auto& e : s->gev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
 
   const int64_t n = s->n;
   const size_t fiber3 = 3 * static_cast<size_t>(M);
@@ -534,6 +552,8 @@ void rf_destroy(rf_simulation* s) {
   for (void* p : s->allocations) (void)hipFree(p);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : s->gev)
+    if (e) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -549,6 +569,7 @@ rf_status rf_assemble_system(rf_simulation* s) {
 
 rf_status rf_solve_system(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  s->ngemv = 0;
   switch (s->cfg.solver) {
     case RF_SOLVER_GMRES:
       RF_TRY(solve_gmres(s));
@@ -614,6 +635,9 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.update_velocities = elapsed(s->ev[2], s->ev[3]);
   s->timings.update_fibers = elapsed(s->ev[3], s->ev[4]);
   s->timings.total = elapsed(s->ev[0], s->ev[4]);
+  s->timings.gemv = 0.0;
+  for (int k = 0; k < s->ngemv; ++k) s->timings.gemv += elapsed(s->gev[2 * k], s->gev[2 * k + 1]);
+  s->timings.gemv_launches = s->ngemv;
   RF_TRY(rf_advance(s));
   if (!s->stats.converged) {
     g_last_error = "solver did not reach its tolerance (relative residual " +

@@ -41,9 +41,10 @@ struct GemvPlan {
 };
 GemvPlan gemv_plan(int64_t n, int64_t lda);
 size_t gemv_partial_elems(const GemvPlan& p);
-// y = A x  (mode 0) or y = b - A x (mode 1); A FP32 column-major.
+// y = A x  (mode 0) or y = b - A x (mode 1); A FP32 column-major.  Optional
+// events are recorded around the streaming kernel (not the chunk reduction).
 hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part, double* y, const double* b,
-                int mode, hipStream_t s);
+                int mode, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // out[k] = <V_k, w> for k < K (V column stride ldv), partial buffer of
 // mdot_blocks(n) * K doubles.
 int mdot_blocks(int64_t n);

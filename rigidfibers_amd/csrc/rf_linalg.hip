@@ -107,9 +107,11 @@ GemvPlan gemv_plan(int64_t n, int64_t lda) {
 size_t gemv_partial_elems(const GemvPlan& p) { return static_cast<size_t>(p.chunks) * static_cast<size_t>(p.n); }
 
 hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part, double* y, const double* b,
-                int mode, hipStream_t s) {
+                int mode, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
   hipLaunchKernelGGL(gemv_f32_kernel, dim3(p.row_tiles, p.chunks), dim3(kGemvThreads), 0, s, p.n, p.lda, A, x, part,
                      p.cols_per_chunk);
+  if (ev_end) (void)hipEventRecord(ev_end, s);
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3(static_cast<unsigned>((p.n + 255) / 256)), dim3(256), 0, s, p.n,
                      p.chunks, part, y, b, mode);
   return hipGetLastError();

@@ -30,6 +30,10 @@ struct LuResult {
   bool converged = false;
 };
 
+// C -= A B, column-major FP32 blocks, products on MFMA in the given mode.
+hipError_t gemm_sub(LuMode mode, int64_t m, int64_t n, int64_t k, const float* A, int64_t lda, const float* B,
+                    int64_t ldb, float* C, int64_t ldc, hipStream_t s);
+
 hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws);
 int64_t lu_workspace_bytes(const LuWorkspace& ws);
 void lu_workspace_destroy(LuWorkspace* ws);

@@ -25,7 +25,8 @@
 
 namespace rf {
 
-constexpr int kLuNb = 64;
+constexpr int kLuNb = 64;      // inner panel (diagonal block in LDS)
+constexpr int kLuOuter = 256;  // outer panel: the K of the trailing MFMA update
 
 // ---------------------------------------------------------------------------
 // 1. diagonal block: GEPP inside one workgroup.
@@ -114,18 +115,18 @@ __global__ void lu_swap_kernel(int64_t n, int64_t lda, float* __restrict__ F, in
   }
 }
 
-// 3a. L21 = A21 U11^-1: lane per row r >= k0+kb, forward over the block's columns.
+// 3a. L21 = A21 U11^-1: lane per row r in [r0, r1), forward over the block's columns.
 template <int NB>
-__global__ __launch_bounds__(256) void lu_trsm_l21_kernel(int64_t n, int64_t lda, float* __restrict__ F, int64_t k0,
-                                                          int kb) {
+__global__ __launch_bounds__(256) void lu_trsm_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
+                                                          int64_t k0, int kb) {
   __shared__ float U[NB][NB + 1];  // [col][row]
   for (int e = threadIdx.x; e < kb * kb; e += 256) {
     const int c = e / kb, r = e - c * kb;
     U[c][r] = F[(k0 + c) * lda + k0 + r];
   }
   __syncthreads();
-  const int64_t r = k0 + kb + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (r >= n) return;
+  const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (r >= r1) return;
   float x[NB];
 #pragma unroll
   for (int c = 0; c < NB; ++c) x[c] = (c < kb) ? F[(k0 + c) * lda + r] : 0.0f;
@@ -144,18 +145,18 @@ __global__ __launch_bounds__(256) void lu_trsm_l21_kernel(int64_t n, int64_t lda
     if (c < kb) F[(k0 + c) * lda + r] = x[c];
 }
 
-// 3b. U12 = L11^-1 A12: lane per column c >= k0+kb (unit lower L11).
+// 3b. U12 = L11^-1 A12: lane per column c in [c0, c1) (unit lower L11).
 template <int NB>
-__global__ __launch_bounds__(256) void lu_trsm_u12_kernel(int64_t n, int64_t lda, float* __restrict__ F, int64_t k0,
-                                                          int kb) {
+__global__ __launch_bounds__(256) void lu_trsm_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
+                                                          int64_t k0, int kb) {
   __shared__ float L[NB][NB + 1];  // [col][row]
   for (int e = threadIdx.x; e < kb * kb; e += 256) {
     const int c = e / kb, r = e - c * kb;
     L[c][r] = F[(k0 + c) * lda + k0 + r];
   }
   __syncthreads();
-  const int64_t c = k0 + kb + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (c >= n) return;
+  const int64_t c = c0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (c >= c1) return;
   float* col = F + c * lda + k0;
   float x[NB];
 #pragma unroll
@@ -173,53 +174,6 @@ __global__ __launch_bounds__(256) void lu_trsm_u12_kernel(int64_t n, int64_t lda
 #pragma unroll
   for (int r = 0; r < NB; ++r)
     if (r < kb) col[r] = x[r];
-}
-
-// 4. C -= A B with A = F[r0:, k0:k0+kb], B = F[k0:k0+kb, c0:], C = F[r0:, c0:].
-// 64x64 tile per workgroup, 4x4 outputs per lane, FP32.
-__global__ __launch_bounds__(256) void lu_gemm_simt_kernel(int64_t n, int64_t lda, float* __restrict__ F, int64_t k0,
-                                                           int kb) {
-  __shared__ float As[16][64 + 4];
-  __shared__ float Bs[16][64 + 4];
-  const int64_t r0 = k0 + kb + static_cast<int64_t>(blockIdx.x) * 64;
-  const int64_t c0 = k0 + kb + static_cast<int64_t>(blockIdx.y) * 64;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  float acc[4][4] = {};
-  for (int kk = 0; kk < kb; kk += 16) {
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      const int k = e >> 6, i = e & 63;
-      const int64_t r = r0 + i;
-      As[k][i] = (kk + k < kb && r < n) ? F[(k0 + kk + k) * lda + r] : 0.0f;
-      const int kc = e & 15, j = e >> 4;
-      const int64_t c = c0 + j;
-      Bs[kc][j] = (kk + kc < kb && c < n) ? F[c * lda + k0 + kk + kc] : 0.0f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      float a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[u] = As[k][ty * 4 + u];
-        b[u] = Bs[k][tx * 4 + u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = fmaf(a[u], b[v], acc[u][v]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int64_t c = c0 + tx * 4 + v;
-    if (c >= n) continue;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int64_t r = r0 + ty * 4 + u;
-      if (r < n) F[c * lda + r] -= acc[u][v];
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -270,20 +224,47 @@ __global__ void lu_panel_gemv_kernel(int64_t lda, const float* __restrict__ F, i
   y[r] -= s;
 }
 
-static hipError_t lu_factor(LuWorkspace& ws, hipStream_t s) {
+// Two-level right-looking factorisation: inner panels of kLuNb columns are
+// factored inside an outer panel of kLuOuter columns (their updates touch only
+// the outer panel's columns); the outer panel's rows of U are then solved for
+// and the trailing matrix receives one rank-kLuOuter update on MFMA.
+static hipError_t lu_factor(LuWorkspace& ws, LuMode mode, hipStream_t s) {
   const int64_t n = ws.n, lda = ws.lda;
-  for (int64_t k0 = 0; k0 < n; k0 += kLuNb) {
-    const int kb = static_cast<int>(std::min<int64_t>(kLuNb, n - k0));
-    hipLaunchKernelGGL((lu_diag_kernel<kLuNb>), dim3(1), dim3(256), 0, s, n, lda, ws.F, k0, kb, ws.ipiv, ws.info);
-    hipLaunchKernelGGL(lu_swap_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, lda, ws.F,
-                       k0, kb, ws.ipiv);
-    const int64_t rest = n - k0 - kb;
-    if (rest > 0) {
-      const unsigned g = static_cast<unsigned>((rest + 255) / 256);
-      hipLaunchKernelGGL((lu_trsm_l21_kernel<kLuNb>), dim3(g), dim3(256), 0, s, n, lda, ws.F, k0, kb);
-      hipLaunchKernelGGL((lu_trsm_u12_kernel<kLuNb>), dim3(g), dim3(256), 0, s, n, lda, ws.F, k0, kb);
-      const unsigned t = static_cast<unsigned>((rest + 63) / 64);
-      hipLaunchKernelGGL(lu_gemm_simt_kernel, dim3(t, t), dim3(256), 0, s, n, lda, ws.F, k0, kb);
+  float* F = ws.F;
+  auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
+  auto blocks = [](int64_t len) { return static_cast<unsigned>((len + 255) / 256); };
+  hipError_t e = hipSuccess;
+  for (int64_t K0 = 0; K0 < n; K0 += kLuOuter) {
+    const int64_t KE = std::min<int64_t>(n, K0 + kLuOuter);
+    for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
+      const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
+      const int64_t k1 = k0 + kb;
+      hipLaunchKernelGGL((lu_diag_kernel<kLuNb>), dim3(1), dim3(256), 0, s, n, lda, F, k0, kb, ws.ipiv, ws.info);
+      hipLaunchKernelGGL(lu_swap_kernel, dim3(blocks(n)), dim3(256), 0, s, n, lda, F, k0, kb, ws.ipiv);
+      if (k1 < n)
+        hipLaunchKernelGGL((lu_trsm_l21_kernel<kLuNb>), dim3(blocks(n - k1)), dim3(256), 0, s, k1, n, lda, F, k0, kb);
+      if (k1 < KE) {
+        hipLaunchKernelGGL((lu_trsm_u12_kernel<kLuNb>), dim3(blocks(KE - k1)), dim3(256), 0, s, k1, KE, lda, F, k0,
+                           kb);
+        if ((e = gemm_sub(mode, n - k1, KE - k1, kb, at(k1, k0), lda, at(k0, k1), lda, at(k1, k1), lda, s)) !=
+            hipSuccess)
+          return e;
+      }
+    }
+    if (KE < n) {
+      // rows K0..KE of U for the trailing columns: U12 = L11^-1 A12, blockwise
+      for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
+        const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
+        const int64_t k1 = k0 + kb;
+        hipLaunchKernelGGL((lu_trsm_u12_kernel<kLuNb>), dim3(blocks(n - KE)), dim3(256), 0, s, KE, n, lda, F, k0, kb);
+        if (k1 < KE &&
+            (e = gemm_sub(mode, KE - k1, n - KE, kb, at(k1, k0), lda, at(k0, KE), lda, at(k1, KE), lda, s)) !=
+                hipSuccess)
+          return e;
+      }
+      if ((e = gemm_sub(mode, n - KE, n - KE, KE - K0, at(KE, K0), lda, at(K0, KE), lda, at(KE, KE), lda, s)) !=
+          hipSuccess)
+        return e;
     }
   }
   return hipGetLastError();
@@ -340,14 +321,13 @@ void lu_workspace_destroy(LuWorkspace* ws) {
 hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const float* A, int64_t lda, int64_t n, const double* b,
                     double* x, double* gpart, const GemvPlan& plan, double* work, double* dpart, double* hdev,
                     double tol, int maxit, hipStream_t s, LuResult* result) {
-  (void)mode;
   hipError_t e;
   *result = LuResult{};
   if ((e = hipMemcpy2DAsync(ws.F, lda * sizeof(float), A, lda * sizeof(float), n * sizeof(float), n,
                             hipMemcpyDeviceToDevice, s)) != hipSuccess)
     return e;
   if ((e = hipMemsetAsync(ws.info, 0, sizeof(int), s)) != hipSuccess) return e;
-  if ((e = lu_factor(ws, s)) != hipSuccess) return e;
+  if ((e = lu_factor(ws, mode, s)) != hipSuccess) return e;
   int info = 0;
   if ((e = hipMemcpyAsync(&info, ws.info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;

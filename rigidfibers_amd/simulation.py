@@ -148,7 +148,7 @@ class Simulation:
     def timings(self):
         t = _lib.RfTimings()
         _lib.check(self._lib.rf_get_timings(self._h, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in _lib.RfTimings._fields_}
+        return {k: getattr(t, k) for k, _ in _lib.RfTimings._fields_ if k != "reserved"}
 
     def device_bytes(self):
         return int(self._lib.rf_device_bytes(self._h))

@@ -158,7 +158,8 @@ def test_single_fiber_known_answer():
 
 def test_ring_symmetry():
     """16 vertical fibers on a ring (the reference's tests/XcT_ring16.in): by
-    symmetry every fiber has the same speed and the same vertical velocity."""
+    symmetry every fiber has the same speed and the same vertical velocity, to
+    the precision of the ring's 8-decimal coordinates and the FP32 system."""
     import os
 
     from helpers import GOLDEN
@@ -173,17 +174,20 @@ def test_ring_symmetry():
         v, w = sim.velocities()
     v = v.reshape(M, 3)
     speed = np.linalg.norm(v, axis=1)
-    assert np.ptp(speed) < 1e-9 * speed.max()
-    assert np.ptp(v[:, 2]) < 1e-9 * np.abs(v[:, 2]).max()
+    assert np.ptp(speed) < 1e-6 * speed.max()
+    assert np.ptp(v[:, 2]) < 1e-6 * np.abs(v[:, 2]).max()
 
 
-@pytest.mark.parametrize("solver", ["gmres", "bicgstab", "direct"])
-def test_solvers_agree_with_oracle(oracle64, solver):
-    """x of each solver against the oracle's LU solution of the oracle's system."""
+@pytest.mark.parametrize("solver,M", [("gmres", 60), ("bicgstab", 60), ("direct", 60), ("direct", 230),
+                                      ("mixed-bf16", 60), ("mixed-fp16", 60), ("mixed-bf16", 230),
+                                      ("mixed-fp16", 230)])
+def test_solvers_agree_with_oracle(oracle64, solver, M):
+    """x of each solver against the oracle's LU solution of the oracle's system
+    (n = 900 and 3450: one and several 256-column outer LU panels with ragged
+    edges)."""
     from oracle.pyoracle import make_params
 
-    M = 60
-    pos, ori = random_scene(M, side=5.0, seed=3, min_dist=0.4)
+    pos, ori = random_scene(M, side=5.0 * (M / 60) ** (1 / 3), seed=3, min_dist=0.4)
     params = ref_config()
     params.NUMBER_OF_FIBERS = M
     params.GMRES_TOLERANCE = 1e-9
