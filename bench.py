@@ -14,9 +14,10 @@ run untimed, then exactly K steps are timed between two device
 synchronisations.  value = M * K / elapsed.
 
 Also reported:
-  roofline      the GEMV of the solver (the streaming matvec, HBM-bound): A's
-                n^2 * 4 bytes per launch / mean launch duration (HIP events on
-                the library's stream), against 8 TB/s;
+  roofline      the matvec of the solver (HBM-bound): its algorithmic bytes
+                per launch (theta form: M * M * N * (3 + 6(N-1)) * 4, the FP32
+                numbers that define A; dense: n^2 * 4) / mean launch duration
+                (HIP events on the library's stream), against 8 TB/s;
   kernels       per-phase times and the assembly kernel's rate;
   cpu_baseline  the FP32 OpenMP port of the reference Fortran (oracle build
                 librf_oracle32.so) on a bounded sample: the rows of S target
@@ -52,7 +53,8 @@ def run_gpu(args, pos, ori):
     params = Parameters(NUMBER_OF_FIBERS=M, TIMESTEP=0.1, SLENDERNESS=0.01, NUMBER_OF_TERMS_IN_FORCE_EXPANSION=5,
                         NUMBER_OF_QUADRATURE_INTERVALS=8, GMRES_RESTART=args.restart,
                         GMRES_MAX_ITERATIONS=1000, GMRES_TOLERANCE=args.tol, BICGSTAB_TOLERANCE=args.tol)
-    sim = Simulation(params, pos, ori, solver=args.solver, integration=args.integration)
+    sim = Simulation(params, pos, ori, solver=args.solver, integration=args.integration,
+                     matrix_format=args.matrix_format)
     try:
         for t in range(args.warmup):
             sim.step(t)
@@ -134,6 +136,7 @@ def main():
     ap.add_argument("--integration", default="analytical", choices=["analytical", "numerical"])
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--restart", type=int, default=10)
+    ap.add_argument("--matrix-format", default="auto", choices=["auto", "dense", "theta"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -147,7 +150,15 @@ def main():
     M, K, n = args.fibers, args.steps, g["n"]
     value = M * K / g["elapsed"]
     gemv_avg_ms = g["gemv_ms"] / max(g["gemv_n"], 1)
-    gemv_bytes = float(n) * n * 4.0
+    iterative = args.solver in ("gmres", "bicgstab")
+    theta = args.matrix_format == "theta" or (args.matrix_format == "auto" and iterative)
+    N = 5
+    if theta:  # 3 + 6(N-1) FP32 numbers per (target, source, source mode)
+        gemv_bytes = float(M) * (3 + 6 * (N - 1)) * M * N * 4.0
+        gemv_kernel = "gemv_theta_kernel"
+    else:      # n^2 FP32 entries
+        gemv_bytes = float(n) * n * 4.0
+        gemv_kernel = "gemv_f32_kernel"
     achieved = gemv_bytes / (gemv_avg_ms * 1e-3) / 1e9
     asm_ms = g["phases"]["assemble_system"] / K
     # algorithmic count of the analytical assembly: per ordered pair and target
@@ -169,8 +180,9 @@ def main():
         "config": {"workload": f"rigid-fiber timestep, M={M} fibers in a ball of radius {args.radius:g}, "
                                f"min centre distance 0.4, N=5, 8x3 quadrature, {args.integration} integrals, "
                                f"{args.solver} tol {args.tol:g} (restart {args.restart}), dt=0.1",
-                   "fibers": M, "unknowns": n, "solver": args.solver, "integration": args.integration},
-        "roofline": {"bound": "hbm", "kernel": "gemv_f32_kernel", "achieved": round(achieved, 1),
+                   "fibers": M, "unknowns": n, "solver": args.solver, "integration": args.integration,
+                   "matrix_format": "theta" if theta else "dense"},
+        "roofline": {"bound": "hbm", "kernel": gemv_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": gemv_bytes, "avg_launch_ms": round(gemv_avg_ms, 4)},
         "kernels": {k: round(v / K, 3) for k, v in g["phases"].items()},

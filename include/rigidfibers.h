@@ -72,6 +72,15 @@ typedef enum rf_velocity_integration {
  * previous step's solution, which also saves iterations. */
 typedef enum rf_initial_guess { RF_GUESS_ZERO = 0, RF_GUESS_PREVIOUS = 1 } rf_initial_guess;
 
+/* Storage of A on the device.  DENSE: n x n FP32 column-major, the
+ * reference's layout (src/simulation.cu:64-110).  THETA: each 3x3 block kept
+ * as the 3 + 6(N-1) numbers that determine a block column (Q0 and the
+ * symmetric projections Theta, fortran/assemble_matrix.incl:59-137), 40% fewer
+ * bytes for N = 5; the matrix-vector product rebuilds the blocks on the fly.
+ * AUTO: THETA for the iterative solvers, DENSE for the direct ones.  Dumps
+ * (rf_get_system) always return the dense matrix. */
+typedef enum rf_matrix_format { RF_MATRIX_AUTO = 0, RF_MATRIX_DENSE = 1, RF_MATRIX_THETA = 2 } rf_matrix_format;
+
 /* The keys of the reference's configuration file (tests/config.ini:18-37,
  * read by runner/cuda_runner.py:18-47), as run-time values. */
 typedef struct rf_config {
@@ -93,7 +102,8 @@ typedef struct rf_config {
   double refinement_tolerance;                        /* mixed direct solvers: FP64 residual target */
   int32_t refinement_max_iterations;
   int32_t initial_guess;                              /* rf_initial_guess */
-  int32_t reserved[7];
+  int32_t matrix_format;                              /* rf_matrix_format */
+  int32_t reserved[6];
 } rf_config;
 
 typedef struct rf_solver_stats {

@@ -24,6 +24,7 @@ SOLVERS = {"gmres": 0, "bicgstab": 1, "direct": 2, "magma": 2, "mixed-bf16": 3, 
 INTEGRATIONS = {"numerical": 0, "analytical": 1}
 VELOCITY_INTEGRATIONS = {"follow": 0, "numerical": 1, "analytical": 2}
 INITIAL_GUESSES = {"zero": 0, "previous": 1}
+MATRIX_FORMATS = {"auto": 0, "dense": 1, "theta": 2}
 
 # Every symbol include/rigidfibers.h declares.
 EXPORTS = (
@@ -54,7 +55,8 @@ class RfConfig(C.Structure):
         ("refinement_tolerance", C.c_double),
         ("refinement_max_iterations", C.c_int32),
         ("initial_guess", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("matrix_format", C.c_int32),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

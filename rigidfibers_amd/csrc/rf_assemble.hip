@@ -39,6 +39,52 @@ __device__ __forceinline__ void analytic_modes(const double R0[3], const double 
   }
 }
 
+// G_m(q_i) for m = 0..N of one ordered pair (i, j) at the lane's target node:
+// quadrature over q_j (mirror-paired nodes) or the exact FP64 integral.
+template <int N, int NQI, bool ANALYTIC>
+__device__ __forceinline__ void inner_integrals(const double xi[3], const double tid_[3], const float ti[3], double sq_d,
+                                                float sq, double xj0, double xj1, double xj2, double tj0, double tj1,
+                                                double tj2, const PhysConsts& pc, float acc[N + 1][6]) {
+  constexpr Quad<N, NQI> quad{};
+  constexpr int Q = 3 * NQI;
+  constexpr int NM = N + 1;
+#pragma unroll
+  for (int m = 0; m < NM; ++m)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) acc[m][c] = 0.0f;
+  if constexpr (!ANALYTIC) {
+    // R(q_i, q_j) = (x_i - x_j) + s_i t_i - s_j t_j; the centre difference is
+    // formed in FP64 and rounded once.
+    const float ax = fmaf(sq, ti[0], static_cast<float>(xi[0] - xj0));
+    const float ay = fmaf(sq, ti[1], static_cast<float>(xi[1] - xj1));
+    const float az = fmaf(sq, ti[2], static_cast<float>(xi[2] - xj2));
+    const float tx = static_cast<float>(tj0), ty = static_cast<float>(tj1), tz = static_cast<float>(tj2);
+#pragma unroll
+    for (int h = 0; h < Q / 2; ++h) {
+      const float s = quad.s[h];
+      const float ux = s * tx, uy = s * ty, uz = s * tz;
+      float Kp[6], Km[6];
+      stokes_kernel(ax - ux, ay - uy, az - uz, pc.eps2, Kp);  // node h (s)
+      stokes_kernel(ax + ux, ay + uy, az + uz, pc.eps2, Km);  // node Q-1-h (-s)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        const float E = Kp[c] + Km[c];
+        const float O = Kp[c] - Km[c];
+#pragma unroll
+        for (int m = 0; m < NM; ++m) acc[m][c] = fmaf(quad.W[m][h], (m & 1) ? O : E, acc[m][c]);
+      }
+    }
+  } else {
+    const double R0[3] = {xi[0] - xj0 + sq_d * tid_[0], xi[1] - xj1 + sq_d * tid_[1], xi[2] - xj2 + sq_d * tid_[2]};
+    const double tjd[3] = {tj0, tj1, tj2};
+    const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
+    const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
+    double I1[N + 3], I3[N + 3], I5[N + 3];
+    moments<N>(bb, ll, I1, I3, I5);
+    analytic_modes<N, NQI, 0>(R0, tjd, pc.eps2_d, I1, I3, I5, acc);
+  }
+}
+
 template <int N, int NQI, bool ANALYTIC>
 __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t lda, const double* __restrict__ pos,
                                                                 const double* __restrict__ ori,
@@ -93,42 +139,7 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
       const double xj0 = pos[3 * j + 0], xj1 = pos[3 * j + 1], xj2 = pos[3 * j + 2];
       const double tj0 = ori[3 * j + 0], tj1 = ori[3 * j + 1], tj2 = ori[3 * j + 2];
       float acc[NM][6];
-#pragma unroll
-      for (int m = 0; m < NM; ++m)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) acc[m][c] = 0.0f;
-      if constexpr (!ANALYTIC) {
-        // R(q_i, q_j) = (x_i - x_j) + s_i t_i - s_j t_j; the centre difference
-        // is formed in FP64 and rounded once.
-        const float ax = fmaf(sq, ti[0], static_cast<float>(xi[0] - xj0));
-        const float ay = fmaf(sq, ti[1], static_cast<float>(xi[1] - xj1));
-        const float az = fmaf(sq, ti[2], static_cast<float>(xi[2] - xj2));
-        const float tx = static_cast<float>(tj0), ty = static_cast<float>(tj1), tz = static_cast<float>(tj2);
-#pragma unroll
-        for (int h = 0; h < Q / 2; ++h) {
-          const float s = quad.s[h];
-          const float ux = s * tx, uy = s * ty, uz = s * tz;
-          float Kp[6], Km[6];
-          stokes_kernel(ax - ux, ay - uy, az - uz, pc.eps2, Kp);  // node h (s)
-          stokes_kernel(ax + ux, ay + uy, az + uz, pc.eps2, Km);  // node Q-1-h (-s)
-#pragma unroll
-          for (int c = 0; c < 6; ++c) {
-            const float E = Kp[c] + Km[c];
-            const float O = Kp[c] - Km[c];
-#pragma unroll
-            for (int m = 0; m < NM; ++m) acc[m][c] = fmaf(quad.W[m][h], (m & 1) ? O : E, acc[m][c]);
-          }
-        }
-      } else {
-        const double R0[3] = {xi[0] - xj0 + sq_d * tid_[0], xi[1] - xj1 + sq_d * tid_[1],
-                              xi[2] - xj2 + sq_d * tid_[2]};
-        const double tjd[3] = {tj0, tj1, tj2};
-        const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
-        const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
-        double I1[N + 3], I3[N + 3], I5[N + 3];
-        moments<N>(bb, ll, I1, I3, I5);
-        analytic_modes<N, NQI, 0>(R0, tjd, pc.eps2_d, I1, I3, I5, acc);
-      }
+      inner_integrals<N, NQI, ANALYTIC>(xi, tid_, ti, sq_d, sq, xj0, xj1, xj2, tj0, tj1, tj2, pc, acc);
 #pragma unroll
       for (int m = 0; m < NM; ++m)
 #pragma unroll
@@ -233,6 +244,215 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
   }
 }
 
+// ---------------------------------------------------------------------------
+// Theta form of the system, for the iterative solvers.
+//
+// Every off-diagonal 3x3 block of A is fixed by t_i and the symmetric
+// projection Theta_{k_i,k_j} (fortran/assemble_matrix.incl:59-79,117-137):
+//   k_i = 0:  A_blk = D1 t_i Q0^T,  Q0 = Theta_{0,k_j} t_i            (3 numbers)
+//   k_i > 0:  A_blk = gamma_k (Theta - eigen_k t_i (Theta t_i)^T)     (6 numbers)
+// so 3 + 6(N-1) numbers per (i, j, k_j) carry the 9N entries of a block
+// column (27 instead of 45 for N = 5).  They are stored as component arrays
+//   T[(i * NC + c) * Lp + j * N + k_j],  NC = 3 + 6(N-1),  Lp >= M*N,
+// c = b for Q0_b, c = 3 + 6(k_i-1) + s for Theta_{k_i}[s] (s = xx yy zz xy xz yz),
+// zero for j == i (the identity diagonal blocks are implicit).
+//
+// Workgroup = one target fiber i x 8 consecutive source fibers per iteration
+// (lane = (source, q_i)); phase 2 jobs (column b, source, mode) are ordered so
+// that 40 consecutive lanes write 40 consecutive floats of each component.
+
+template <int N, int NQI, bool ANALYTIC>
+__global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int64_t Lp,
+                                                                      const double* __restrict__ pos,
+                                                                      const double* __restrict__ ori,
+                                                                      float* __restrict__ T,
+                                                                      double* __restrict__ bpart, int jgroup,
+                                                                      PhysConsts pc) {
+  constexpr Quad<N, NQI> quad{};
+  constexpr int Q = 3 * NQI;
+  constexpr int PAIRS = kPairThreads / Q;
+  constexpr int NM = N + 1;
+  constexpr int ROWS = 3 * N;
+  constexpr int NC = 3 + 6 * (N - 1);
+  static_assert(kPairThreads % Q == 0, "Q must divide the workgroup");
+
+  __shared__ float lds[PAIRS][NM][6][Q];
+  __shared__ double bred[PAIRS][ROWS];
+
+  const int tid = threadIdx.x;
+  const int p = tid / Q;
+  const int qi = tid - p * Q;
+  const int i = blockIdx.x;
+  const int jb = blockIdx.y * jgroup;
+  const int je = min(M, jb + jgroup);
+  const int64_t n = static_cast<int64_t>(ROWS) * M;
+
+  const double xi[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  const double tid_[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
+  const float ti[3] = {static_cast<float>(tid_[0]), static_cast<float>(tid_[1]), static_cast<float>(tid_[2])};
+  const double sq_d = node_fp64<NQI>(qi);
+  const float sq = static_cast<float>(sq_d);
+  float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
+
+  // RHS sums of each source slot, kept in LDS (only PAIRS lanes update them)
+  for (int r = tid; r < PAIRS * ROWS; r += kPairThreads) bred[r / ROWS][r % ROWS] = 0.0;
+  __syncthreads();
+
+  constexpr int kJobs = PAIRS + 3 * PAIRS * N;
+
+  for (int j0 = jb; j0 < je; j0 += PAIRS) {
+    const int j = j0 + p;
+    if (j < je && j != i) {
+      float acc[NM][6];
+      inner_integrals<N, NQI, ANALYTIC>(xi, tid_, ti, sq_d, sq, pos[3 * j], pos[3 * j + 1], pos[3 * j + 2],
+                                        ori[3 * j], ori[3 * j + 1], ori[3 * j + 2], pc, acc);
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lds[p][m][c][qi] = acc[m][c];
+    }
+    __syncthreads();
+    for (int job = tid; job < kJobs; job += kPairThreads) {
+      if (job < PAIRS) {  // RHS job of source slot p2
+        const int p2 = job, j2 = j0 + p2;
+        if (j2 >= je || j2 == i) continue;
+        float TF[N][3];
+#pragma unroll
+        for (int k = 0; k < N; ++k) TF[k][0] = TF[k][1] = TF[k][2] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const float g0 = lds[p2][0][0][q], g1 = lds[p2][0][1][q], g2 = lds[p2][0][2][q];
+          const float g3 = lds[p2][0][3][q], g4 = lds[p2][0][4][q], g5 = lds[p2][0][5][q];
+          const float f0 = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
+          const float f1 = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
+          const float f2 = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
+#pragma unroll
+          for (int k = 0; k < N; ++k) {
+            TF[k][0] = fmaf(quad.W[k + 1][q], f0, TF[k][0]);
+            TF[k][1] = fmaf(quad.W[k + 1][q], f1, TF[k][1]);
+            TF[k][2] = fmaf(quad.W[k + 1][q], f2, TF[k][2]);
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          const float QF = TF[k][0] * ti[0] + TF[k][1] * ti[1] + TF[k][2] * ti[2];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float v = (k == 0) ? pc.D1 * ti[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * ti[a] * QF);
+            bred[p2][3 * k + a] -= static_cast<double>(v);
+          }
+        }
+        continue;
+      }
+      const int jj = job - PAIRS;
+      const int bcol = jj / (PAIRS * N);
+      const int rem = jj - bcol * (PAIRS * N);
+      const int p2 = rem / N;
+      const int m = 1 + (rem - p2 * N);
+      const int j2 = j0 + p2;
+      if (j2 >= je) continue;
+      const int64_t e = static_cast<int64_t>(j2) * N + (m - 1);
+      // components written by column b: b=0 -> xx xy xz, b=1 -> yy yz, b=2 -> zz
+      if (j2 == i) {
+        Ti[bcol * Lp + e] = 0.0f;
+#pragma unroll
+        for (int k = 1; k < N; ++k) {
+          float* Tk = Ti + (3 + 6 * (k - 1)) * Lp + e;
+          if (bcol == 0) {
+            Tk[0 * Lp] = 0.0f;
+            Tk[3 * Lp] = 0.0f;
+            Tk[4 * Lp] = 0.0f;
+          } else if (bcol == 1) {
+            Tk[1 * Lp] = 0.0f;
+            Tk[5 * Lp] = 0.0f;
+          } else {
+            Tk[2 * Lp] = 0.0f;
+          }
+        }
+        continue;
+      }
+      const int c0 = sym_index(0, bcol), c1 = sym_index(1, bcol), c2 = sym_index(2, bcol);
+      float Th[N][3];
+#pragma unroll
+      for (int k = 0; k < N; ++k) Th[k][0] = Th[k][1] = Th[k][2] = 0.0f;
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const float g0 = lds[p2][m][c0][q], g1 = lds[p2][m][c1][q], g2 = lds[p2][m][c2][q];
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+          Th[k][0] = fmaf(quad.W[k + 1][q], g0, Th[k][0]);
+          Th[k][1] = fmaf(quad.W[k + 1][q], g1, Th[k][1]);
+          Th[k][2] = fmaf(quad.W[k + 1][q], g2, Th[k][2]);
+        }
+      }
+      // Q0_b = (Theta_0 t_i)_b
+      Ti[bcol * Lp + e] = Th[0][0] * ti[0] + Th[0][1] * ti[1] + Th[0][2] * ti[2];
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        float* Tk = Ti + (3 + 6 * (k - 1)) * Lp + e;
+        if (bcol == 0) {
+          Tk[0 * Lp] = Th[k][0];  // xx
+          Tk[3 * Lp] = Th[k][1];  // xy
+          Tk[4 * Lp] = Th[k][2];  // xz
+        } else if (bcol == 1) {
+          Tk[1 * Lp] = Th[k][1];  // yy
+          Tk[5 * Lp] = Th[k][2];  // yz
+        } else {
+          Tk[2 * Lp] = Th[k][2];  // zz
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (tid < ROWS) {  // the loop ended on a barrier: bred is complete
+    double s = 0.0;
+    for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
+    bpart[static_cast<int64_t>(blockIdx.y) * n + static_cast<int64_t>(ROWS) * i + tid] = s;
+  }
+}
+
+// Dense A (column-major, leading dimension n) from the theta form; identity
+// diagonal blocks.  One lane per (i, j, k_j).
+template <int N>
+__global__ void expand_theta_kernel(int M, int64_t Lp, const float* __restrict__ T, const double* __restrict__ ori,
+                                    PhysConsts pc, float* __restrict__ A) {
+  constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int ROWS = 3 * N;
+  const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t total = static_cast<int64_t>(M) * M * N;
+  if (id >= total) return;
+  const int kj = static_cast<int>(id % N);
+  const int64_t ij = id / N;
+  const int j = static_cast<int>(ij % M), i = static_cast<int>(ij / M);
+  const int64_t n = static_cast<int64_t>(ROWS) * M;
+  const int64_t e = static_cast<int64_t>(j) * N + kj;
+  const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
+  const float t[3] = {static_cast<float>(ori[3 * i]), static_cast<float>(ori[3 * i + 1]),
+                      static_cast<float>(ori[3 * i + 2])};
+  for (int b = 0; b < 3; ++b) {
+    float* col = A + (static_cast<int64_t>(ROWS) * j + 3 * kj + b) * n + static_cast<int64_t>(ROWS) * i;
+    if (i == j) {
+      for (int r = 0; r < ROWS; ++r) col[r] = (r == 3 * kj + b) ? 1.0f : 0.0f;
+      continue;
+    }
+    const float q0 = Ti[b * Lp + e];
+    for (int a = 0; a < 3; ++a) col[a] = pc.D1 * t[a] * q0;
+    for (int k = 1; k < N; ++k) {
+      const float* Tk = Ti + (3 + 6 * (k - 1)) * Lp + e;
+      float th[3], tht[3];
+      for (int a = 0; a < 3; ++a) th[a] = Tk[sym_index(a, b) * Lp];
+      // (Theta t)_b
+      float qb = 0.0f;
+      for (int a = 0; a < 3; ++a) {
+        tht[a] = Tk[sym_index(a, b) * Lp];
+        qb += tht[a] * t[a];
+      }
+      for (int a = 0; a < 3; ++a) col[3 * k + a] = pc.gamma[k] * (th[a] - pc.eigen[k] * t[a] * qb);
+    }
+  }
+}
+
 __global__ void sum_groups_kernel(int groups, int64_t len, const double* __restrict__ part, double* __restrict__ out) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= len) return;
@@ -253,14 +473,34 @@ PairGrid pair_grid(int M, int pairs_per_tile) {
   return g;
 }
 
+// Theta-form grid: blockIdx.x = target fiber, blockIdx.y = group of source
+// fibers (a multiple of the 8 / 16 / 4 sources one iteration covers).
+PairGrid theta_grid(int M, int sources_per_iteration) {
+  PairGrid g;
+  g.tiles = M;
+  int groups = (4096 + M - 1) / M;
+  const int max_groups = (M + sources_per_iteration - 1) / sources_per_iteration;
+  if (groups > max_groups) groups = max_groups;
+  if (groups < 1) groups = 1;
+  int jg = (M + groups - 1) / groups;
+  jg = (jg + sources_per_iteration - 1) / sources_per_iteration * sources_per_iteration;
+  g.jgroup = jg;
+  g.groups = (M + jg - 1) / jg;
+  return g;
+}
+
 int max_pair_groups(int M) {
   int best = 1;
   for (int pairs : {4, 8, 16}) {
     const PairGrid g = pair_grid(M, pairs);
     if (g.groups > best) best = g.groups;
+    const PairGrid t = theta_grid(M, pairs);
+    if (t.groups > best) best = t.groups;
   }
   return best;
 }
+
+int64_t theta_stride(int M, int N) { return (static_cast<int64_t>(M) * N + 63) / 64 * 64; }
 
 template <int N, int NQI, bool AN>
 static hipError_t launch_asm_t(int M, int64_t lda, const double* pos, const double* ori, float* A, double* bpart,
@@ -304,6 +544,73 @@ hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, 
   }
 }
 #undef RF_ASM_CASE
+
+template <int N, int NQI, bool AN>
+static hipError_t launch_theta_t(int M, int64_t Lp, const double* pos, const double* ori, float* T, double* bpart,
+                                 const PhysConsts& pc, hipStream_t s, int* groups) {
+  constexpr int PAIRS = kPairThreads / (3 * NQI);
+  const PairGrid g = theta_grid(M, PAIRS);
+  *groups = g.groups;
+  hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M, Lp, pos,
+                     ori, T, bpart, g.jgroup, pc);
+  return hipGetLastError();
+}
+
+template <int N, int NQI>
+static hipError_t launch_theta_n(bool an, int M, int64_t Lp, const double* pos, const double* ori, float* T,
+                                 double* bpart, const PhysConsts& pc, hipStream_t s, int* groups) {
+  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, pc, s, groups)
+            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, pc, s, groups);
+}
+
+#define RF_TH_CASE(NN)                                                                             \
+  case NN:                                                                                         \
+    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
+    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
+    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
+    return hipErrorInvalidValue;
+
+hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
+                                 const double* ori, float* T, double* bpart, const PhysConsts& pc, hipStream_t s,
+                                 int* groups) {
+  switch (N) {
+    RF_TH_CASE(1)
+    RF_TH_CASE(2)
+    RF_TH_CASE(3)
+    RF_TH_CASE(4)
+    RF_TH_CASE(5)
+    RF_TH_CASE(6)
+    RF_TH_CASE(7)
+    RF_TH_CASE(8)
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+#undef RF_TH_CASE
+
+hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
+                               float* A, hipStream_t s) {
+  const int64_t total = static_cast<int64_t>(M) * M * N;
+  const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
+  switch (N) {
+#define RF_EX(NN) \
+  case NN:        \
+    hipLaunchKernelGGL(expand_theta_kernel<NN>, dim3(blocks), dim3(256), 0, s, M, Lp, T, ori, pc, A); \
+    break;
+    RF_EX(1)
+    RF_EX(2)
+    RF_EX(3)
+    RF_EX(4)
+    RF_EX(5)
+    RF_EX(6)
+    RF_EX(7)
+    RF_EX(8)
+#undef RF_EX
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
 
 hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double* out, hipStream_t s) {
   const int threads = 256;

@@ -71,7 +71,11 @@ struct rf_simulation {
   bool have_next = false;
   bool have_solution = false;  // x holds a previous solution (initial guess)
 
-  float* A = nullptr;
+  float* A = nullptr;     // dense system (direct solvers / RF_MATRIX_DENSE)
+  bool theta = false;     // system held in theta form (T) instead of A
+  float* T = nullptr;
+  int64_t Lp = 0;         // theta component stride
+  int bgroups = 1;        // RHS partial slots written by the last assembly
   double* b = nullptr;
   double* x = nullptr;
   double* bpart = nullptr;
@@ -138,6 +142,10 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown solver");
   if (c->integration != RF_INTEGRATION_NUMERICAL && c->integration != RF_INTEGRATION_ANALYTICAL)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
+  if (c->matrix_format < RF_MATRIX_AUTO || c->matrix_format > RF_MATRIX_THETA)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown matrix format");
+  if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
   if (c->velocity_integration < RF_VELOCITY_FOLLOW || c->velocity_integration > RF_VELOCITY_ANALYTICAL)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity integration");
   if (c->solver == RF_SOLVER_GMRES) {
@@ -193,6 +201,8 @@ hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, 
     e1 = s->gev[2 * s->ngemv + 1];
     ++s->ngemv;
   }
+  if (s->theta)
+    return rf::gemv_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], x, y, b, mode, s->pc, s->stream, e0, e1);
   return rf::gemv(s->plan, s->A, x, part, y, b, mode, s->stream, e0, e1);
 }
 
@@ -505,7 +515,16 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     st = s->alloc(&s->tv[k], fiber3);
     if (st == RF_OK) st = s->alloc(&s->rv[k], fiber3);
   }
-  if (st == RF_OK) st = s->alloc(&s->A, static_cast<size_t>(s->lda) * static_cast<size_t>(n));
+  const bool iterative = cfg->solver == RF_SOLVER_GMRES || cfg->solver == RF_SOLVER_BICGSTAB;
+  s->theta = (cfg->matrix_format == RF_MATRIX_THETA) || (cfg->matrix_format == RF_MATRIX_AUTO && iterative);
+  if (s->theta && !iterative) return cleanup(fail(RF_ERR_UNSUPPORTED, "the direct solvers need the dense matrix"));
+  if (s->theta) {
+    const int NC = 3 + 6 * (s->N - 1);
+    s->Lp = rf::theta_stride(M, s->N);
+    if (st == RF_OK) st = s->alloc(&s->T, static_cast<size_t>(M) * NC * static_cast<size_t>(s->Lp));
+  } else if (st == RF_OK) {
+    st = s->alloc(&s->A, static_cast<size_t>(s->lda) * static_cast<size_t>(n));
+  }
   if (st == RF_OK) st = s->alloc(&s->b, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->x, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->bpart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
@@ -560,10 +579,15 @@ void rf_destroy(rf_simulation* s) {
 
 rf_status rf_assemble_system(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
-  RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
-                             s->bpart, s->pc, s->stream));
-  const rf::PairGrid g = rf::pair_grid(s->M, 192 / (3 * s->NQI));
-  RF_HIP(rf::launch_sum_groups(g.groups, s->n, s->bpart, s->b, s->stream));
+  if (s->theta) {
+    RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
+                                     s->bpart, s->pc, s->stream, &s->bgroups));
+  } else {
+    RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
+                               s->bpart, s->pc, s->stream));
+    s->bgroups = rf::pair_grid(s->M, 192 / (3 * s->NQI)).groups;
+  }
+  RF_HIP(rf::launch_sum_groups(s->bgroups, s->n, s->bpart, s->b, s->stream));
   return RF_OK;
 }
 
@@ -698,9 +722,24 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
 rf_status rf_get_system(rf_simulation* s, float* a_matrix, double* b_vector, double* x_vector) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
   const int64_t n = s->n;
-  if (a_matrix)
+  if (a_matrix && s->theta) {
+    // rebuild the dense blocks in a temporary device buffer
+    float* tmp = nullptr;
+    hipError_t e = hipMalloc(&tmp, sizeof(float) * static_cast<size_t>(n) * static_cast<size_t>(n));
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RF_ERR_OUT_OF_MEMORY, "no device memory to expand the theta-form matrix");
+    }
+    e = rf::launch_expand_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], s->pc, tmp, s->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(a_matrix, tmp, sizeof(float) * static_cast<size_t>(n) * n, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(RF_ERR_DEVICE, std::string("expanding the matrix: ") + hipGetErrorString(e));
+  } else if (a_matrix) {
     RF_HIP(hipMemcpy2DAsync(a_matrix, n * sizeof(float), s->A, s->lda * sizeof(float), n * sizeof(float), n,
                             hipMemcpyDeviceToHost, s->stream));
+  }
   if (b_vector) RF_HIP(hipMemcpyAsync(b_vector, s->b, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   if (x_vector) RF_HIP(hipMemcpyAsync(x_vector, s->x, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   RF_HIP(hipStreamSynchronize(s->stream));

@@ -130,45 +130,56 @@ __device__ __forceinline__ void stokes_kernel(float rx, float ry, float rz, floa
 // recurrence started from zeros at n = 27 keeping n < N + 3.
 template <int N>
 __device__ __forceinline__ void moments(double b, double l, double I1[N + 3], double I3[N + 3], double I5[N + 3]) {
+  // Divisions by the recurrence's integer constants are multiplications by
+  // reciprocals folded at compile time; the remaining divisions are 1/m, 1/uu,
+  // 1/ul, 1/l and the ratio inside the single logarithm.
   const double m = l - 0.25 * b * b;
   const double uu = sqrt(1.0 + b + l);
   const double ul = sqrt(1.0 - b + l);
+  const double iuu = 1.0 / uu, iul = 1.0 / ul;
   const double pu = 2.0 + b, pl = -2.0 + b;
-  I1[0] = log(fabs(pu + 2.0 * uu)) - log(fabs(pl + 2.0 * ul));
+  // log|pu + 2uu| - log|pl + 2ul|
+  I1[0] = log(fabs((pu + 2.0 * uu) / (pl + 2.0 * ul)));
   I1[1] = uu - ul - 0.5 * b * I1[0];
   if (m < 1e-7) {
-    I3[0] = (-2.0 / (pu * pu)) - (-2.0 / (pl * pl));
-    I5[0] = (-4.0 / (pu * pu * pu * pu)) - (-4.0 / (pl * pl * pl * pl));
-    I5[1] = (-8.0 / (3.0 * pu * pu * pu)) - (-8.0 / (3.0 * pl * pl * pl)) - 0.5 * b * I5[0];
+    const double ipu = 1.0 / pu, ipl = 1.0 / pl;
+    const double ipu2 = ipu * ipu, ipl2 = ipl * ipl;
+    I3[0] = -2.0 * ipu2 + 2.0 * ipl2;
+    I5[0] = -4.0 * ipu2 * ipu2 + 4.0 * ipl2 * ipl2;
+    I5[1] = (-8.0 / 3.0) * ipu2 * ipu + (8.0 / 3.0) * ipl2 * ipl - 0.5 * b * I5[0];
   } else {
     const double im = 1.0 / m;
-    const double iu3 = 1.0 / (uu * uu * uu), il3 = 1.0 / (ul * ul * ul);
-    I3[0] = 0.5 * im * (pu / uu - pl / ul);
-    I5[0] = (im / 6.0) * (pu * iu3 - pl * il3) + (2.0 / 3.0) * im * I3[0];
-    I5[1] = (im / 6.0) * (-(b + 2.0 * l) * iu3 + (-b + 2.0 * l) * il3) - (b / 3.0) * im * I3[0];
+    const double iu3 = iuu * iuu * iuu, il3 = iul * iul * iul;
+    I3[0] = 0.5 * im * (pu * iuu - pl * iul);
+    I5[0] = (im * (1.0 / 6.0)) * (pu * iu3 - pl * il3) + (2.0 / 3.0) * im * I3[0];
+    I5[1] = (im * (1.0 / 6.0)) * (-(b + 2.0 * l) * iu3 + (-b + 2.0 * l) * il3) - (1.0 / 3.0) * b * im * I3[0];
   }
-  I3[1] = (-1.0 / uu) + (1.0 / ul) - 0.5 * b * I3[0];
+  I3[1] = -iuu + iul - 0.5 * b * I3[0];
   if (l < 10.0) {
 #pragma unroll
     for (int n = 2; n <= N + 2; ++n) {
-      const double rn = n;
       // s^(n-1) at s = +1 and s = -1
-      const double su = 1.0, sl = (n & 1) ? 1.0 : -1.0;
-      I1[n] = (su * uu - sl * ul) / rn + ((1.0 - 2.0 * rn) * b) / (2.0 * rn) * I1[n - 1] -
-              ((rn - 1.0) * l) / rn * I1[n - 2];
+      const double sl = (n & 1) ? 1.0 : -1.0;
+      const double inv_n = 1.0 / n;                                  // folded
+      const double c1 = (1.0 - 2.0 * n) / (2.0 * n);                 // folded
+      const double c2 = (n - 1.0) / n;                               // folded
+      I1[n] = (uu - sl * ul) * inv_n + c1 * b * I1[n - 1] - c2 * l * I1[n - 2];
       I3[n] = I1[n - 2] - b * I3[n - 1] - l * I3[n - 2];
       I5[n] = I3[n - 2] - b * I5[n - 1] - l * I5[n - 2];
     }
   } else {
     const double il = 1.0 / l;
+    const double dm = uu - ul, dp = uu + ul;  // uu - s^(n+1) ul for even / odd n+1
     double i1n1 = 0.0, i1n2 = 0.0, i3n1 = 0.0, i3n2 = 0.0, i5n1 = 0.0, i5n2 = 0.0;
 #pragma unroll
     for (int n = 27; n >= 2; --n) {
       const double rn = n;
-      // s^(n+1) at s = +1 and s = -1
-      const double sl = ((n + 1) & 1) ? -1.0 : 1.0;
-      const double i1n0 = (rn + 2.0) / (rn + 1.0) * il *
-                          ((uu - sl * ul) / (rn + 2.0) + ((1.0 - 2.0 * (rn + 2.0)) / (2.0 * (rn + 2.0))) * b * i1n1 - i1n2);
+      // s^(n+1) at s = -1 is +1 for odd n
+      const double d = (n & 1) ? dm : dp;
+      const double cA = 1.0 / (rn + 1.0);                                                   // folded
+      const double cB = (rn + 2.0) / (rn + 1.0) * ((1.0 - 2.0 * (rn + 2.0)) / (2.0 * (rn + 2.0)));  // folded
+      const double cC = (rn + 2.0) / (rn + 1.0);                                             // folded
+      const double i1n0 = il * (cA * d + cB * b * i1n1 - cC * i1n2);
       const double i3n0 = il * (i1n0 - b * i3n1 - i3n2);
       const double i5n0 = il * (i3n0 - b * i5n1 - i5n2);
       if (n < N + 3) {

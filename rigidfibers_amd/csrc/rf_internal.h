@@ -25,6 +25,21 @@ int max_pair_groups(int M);
 hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, const double* pos,
                            const double* ori, float* A, double* bpart, const PhysConsts& pc, hipStream_t s);
 hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double* out, hipStream_t s);
+
+// Theta form (rf_assemble.hip): T holds NC = 3 + 6(N-1) component arrays of
+// stride Lp = theta_stride(M, N) per target fiber; *groups receives the number
+// of RHS partial slots written.
+PairGrid theta_grid(int M, int sources_per_iteration);
+int64_t theta_stride(int M, int N);
+hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
+                                 const double* ori, float* T, double* bpart, const PhysConsts& pc, hipStream_t s,
+                                 int* groups);
+hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
+                               float* A, hipStream_t s);
+// y = A x (mode 0) or y = b - A x (mode 1) with A in theta form.
+hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
+                      const double* b, int mode, const PhysConsts& pc, hipStream_t s, hipEvent_t ev_begin = nullptr,
+                      hipEvent_t ev_end = nullptr);
 hipError_t launch_velocities(int N, int NQI, bool analytical, int M, const double* pos, const double* ori,
                              const double* x, float* wf, double* fmono, double* vpart, double* tvel, double* rvel,
                              const PhysConsts& pc, hipStream_t s);

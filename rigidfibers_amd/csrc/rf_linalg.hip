@@ -118,6 +118,107 @@ hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part
 }
 
 // ---------------------------------------------------------------------------
+// y = A x with A in theta form (see rf_assemble.hip): for target fiber i
+//   s0    = sum_{j,k_j} Q0_{ij,k_j} . x_{j,k_j}
+//   U_k   = sum_{j,k_j} Theta_{ij,k,k_j} x_{j,k_j}                  (k >= 1)
+//   y_i,0 = x_i,0 + D1 t_i s0
+//   y_i,k = x_i,k + gamma_k (U_k - eigen_k t_i (t_i . U_k))
+// using (Theta t_i)^T x = t_i . (Theta x) for the symmetric Theta.  One
+// workgroup per target fiber streams its NC component arrays (each lane one
+// (j, k_j) entry per sweep: every load instruction reads 256 contiguous
+// bytes); FP64 accumulation, workgroup reduction, 15 outputs.
+
+template <int N>
+__global__ __launch_bounds__(256) void gemv_theta_kernel(int M, int64_t Lp, const float* __restrict__ T,
+                                                         const double* __restrict__ ori,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         const double* __restrict__ b, int mode, PhysConsts pc) {
+  constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int NACC = 1 + 3 * (N - 1);
+  __shared__ double red[256 / 64][NACC];
+  const int i = blockIdx.x;
+  const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
+  const int L = M * N;
+  double acc[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+  for (int e = threadIdx.x; e < L; e += 256) {
+    const double x0 = x[3 * static_cast<int64_t>(e)], x1 = x[3 * static_cast<int64_t>(e) + 1],
+                 x2 = x[3 * static_cast<int64_t>(e) + 2];
+    const float* p = Ti + e;
+    const float q0 = __builtin_nontemporal_load(p), q1 = __builtin_nontemporal_load(p + Lp),
+                q2 = __builtin_nontemporal_load(p + 2 * Lp);
+    acc[0] = fma(static_cast<double>(q0), x0, fma(static_cast<double>(q1), x1, fma(static_cast<double>(q2), x2, acc[0])));
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      const float* pk = p + (3 + 6 * (k - 1)) * Lp;
+      const double txx = __builtin_nontemporal_load(pk), tyy = __builtin_nontemporal_load(pk + Lp),
+                   tzz = __builtin_nontemporal_load(pk + 2 * Lp), txy = __builtin_nontemporal_load(pk + 3 * Lp),
+                   txz = __builtin_nontemporal_load(pk + 4 * Lp), tyz = __builtin_nontemporal_load(pk + 5 * Lp);
+      double* u = acc + 1 + 3 * (k - 1);
+      u[0] = fma(txx, x0, fma(txy, x1, fma(txz, x2, u[0])));
+      u[1] = fma(txy, x0, fma(tyy, x1, fma(tyz, x2, u[1])));
+      u[2] = fma(txz, x0, fma(tyz, x1, fma(tzz, x2, u[2])));
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) {
+    double v = acc[a];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * N) {
+    const int r = threadIdx.x, k = r / 3, a = r - 3 * k;
+    const double t[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
+    double out;
+    if (k == 0) {
+      const double s0 = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+      out = static_cast<double>(pc.D1) * t[a] * s0;
+    } else {
+      double U[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int idx = 1 + 3 * (k - 1) + c;
+        U[c] = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
+      }
+      const double tU = t[0] * U[0] + t[1] * U[1] + t[2] * U[2];
+      out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * t[a] * tU);
+    }
+    const int64_t row = static_cast<int64_t>(3 * N) * i + r;
+    out += x[row];  // identity diagonal block
+    y[row] = (mode == 1) ? b[row] - out : out;
+  }
+}
+
+hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
+                      const double* b, int mode, const PhysConsts& pc, hipStream_t s, hipEvent_t ev_begin,
+                      hipEvent_t ev_end) {
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
+  switch (N) {
+#define RF_GT(NN)                                                                                                 \
+  case NN:                                                                                                        \
+    hipLaunchKernelGGL(gemv_theta_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T, ori, x, y, b, mode, pc); \
+    break;
+    RF_GT(1)
+    RF_GT(2)
+    RF_GT(3)
+    RF_GT(4)
+    RF_GT(5)
+    RF_GT(6)
+    RF_GT(7)
+    RF_GT(8)
+#undef RF_GT
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (ev_end) (void)hipEventRecord(ev_end, s);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // Multi-dot: out[k] = <V_k, w>, k < K.  Grid (blocks, K); block (b, k) reduces
 // rows [b*rpb, (b+1)*rpb) of vector k; a second kernel sums the blocks.
 

@@ -22,7 +22,7 @@ def _dptr(a):
 
 def make_config(params: Parameters, solver="gmres", integration="numerical", velocity_integration="follow",
                 external_force=(0.0, 0.0, -1.0), refinement_tolerance=1e-10, refinement_max_iterations=50,
-                initial_guess="previous"):
+                initial_guess="previous", matrix_format="auto"):
     if solver not in _lib.SOLVERS:
         raise ValueError(f"unknown solver {solver!r}")
     if integration not in _lib.INTEGRATIONS:
@@ -48,6 +48,7 @@ def make_config(params: Parameters, solver="gmres", integration="numerical", vel
     cfg.refinement_tolerance = refinement_tolerance
     cfg.refinement_max_iterations = refinement_max_iterations
     cfg.initial_guess = _lib.INITIAL_GUESSES[initial_guess]
+    cfg.matrix_format = _lib.MATRIX_FORMATS[matrix_format]
     return cfg
 
 

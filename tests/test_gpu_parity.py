@@ -221,3 +221,31 @@ def test_ten_steps_against_oracle(oracle64):
     ep, eo = rel_err(p, p_ref), rel_err(o, o_ref)
     print(f"10 steps: pos {ep:.1e} ori {eo:.1e}")
     assert ep < 1e-5 and eo < 1e-5
+
+
+@pytest.mark.parametrize("analytical", [False, True])
+def test_theta_form_matches_dense(analytical):
+    """The theta-form system (the iterative solvers' storage) expands to the
+    dense assembly and gives the same GMRES solution and state."""
+    M = 50
+    pos, ori = random_scene(M, side=4.5, seed=21, min_dist=0.4)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-10
+    out = {}
+    for fmt in ("dense", "theta"):
+        with _sim(params, pos, ori, integration="analytical" if analytical else "numerical",
+                  matrix_format=fmt) as sim:
+            sim.assemble_system()
+            A, b, _ = sim.system(with_matrix=True)
+            sim.solve_system()
+            _, _, x = sim.system()
+            sim.update_velocities()
+            sim.update_fibers(True)
+            p, o = sim.next_state()
+        out[fmt] = (A, b, x, p, o)
+    Ad, bd, xd, pd, od = out["dense"]
+    At, bt, xt, pt, ot = out["theta"]
+    print("A", rel_err(At, Ad), "b", rel_err(bt, bd), "x", rel_err(xt, xd), "pos", rel_err(pt, pd))
+    assert rel_err(At, Ad) < 1e-6 and rel_err(bt, bd) < 1e-12
+    assert rel_err(xt, xd) < 1e-6 and rel_err(pt, pd) < 1e-8 and rel_err(ot, od) < 1e-7
