@@ -54,7 +54,7 @@ def run_gpu(args, pos, ori):
                         NUMBER_OF_QUADRATURE_INTERVALS=8, GMRES_RESTART=args.restart,
                         GMRES_MAX_ITERATIONS=1000, GMRES_TOLERANCE=args.tol, BICGSTAB_TOLERANCE=args.tol)
     sim = Simulation(params, pos, ori, solver=args.solver, integration=args.integration,
-                     matrix_format=args.matrix_format)
+                     matrix_format=args.matrix_format, velocity_operator=args.velocity_operator)
     try:
         for t in range(args.warmup):
             sim.step(t)
@@ -137,6 +137,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--restart", type=int, default=10)
     ap.add_argument("--matrix-format", default="auto", choices=["auto", "dense", "theta"])
+    ap.add_argument("--velocity-operator", default="stored", choices=["stored", "pairwise"])
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

@@ -81,6 +81,17 @@ typedef enum rf_initial_guess { RF_GUESS_ZERO = 0, RF_GUESS_PREVIOUS = 1 } rf_in
  * (rf_get_system) always return the dense matrix. */
 typedef enum rf_matrix_format { RF_MATRIX_AUTO = 0, RF_MATRIX_DENSE = 1, RF_MATRIX_THETA = 2 } rf_matrix_format;
 
+/* How the velocity update forms its sums over source fibers
+ * (fortran/update_velocities.incl:29-86).  Both are projections of the inner
+ * integrals the assembly already computes, so with STORED (the default) a
+ * theta-form assembly whose integration the velocities share also keeps 12
+ * FP32 numbers per (target, source, source mode), and rf_update_velocities
+ * becomes one streaming pass over them (valid until the state changes).
+ * PAIRWISE recomputes the kernel integrals for every fiber pair.  STORED
+ * falls back to PAIRWISE for the dense format, when the velocity integration
+ * differs from the system's, or after rf_set_state without a new assembly. */
+typedef enum rf_velocity_operator { RF_VELOCITY_STORED = 0, RF_VELOCITY_PAIRWISE = 1 } rf_velocity_operator;
+
 /* The keys of the reference's configuration file (tests/config.ini:18-37,
  * read by runner/cuda_runner.py:18-47), as run-time values. */
 typedef struct rf_config {
@@ -103,7 +114,8 @@ typedef struct rf_config {
   int32_t refinement_max_iterations;
   int32_t initial_guess;                              /* rf_initial_guess */
   int32_t matrix_format;                              /* rf_matrix_format */
-  int32_t reserved[6];
+  int32_t velocity_operator;                          /* rf_velocity_operator */
+  int32_t reserved[5];
 } rf_config;
 
 typedef struct rf_solver_stats {

@@ -25,6 +25,7 @@ INTEGRATIONS = {"numerical": 0, "analytical": 1}
 VELOCITY_INTEGRATIONS = {"follow": 0, "numerical": 1, "analytical": 2}
 INITIAL_GUESSES = {"zero": 0, "previous": 1}
 MATRIX_FORMATS = {"auto": 0, "dense": 1, "theta": 2}
+VELOCITY_OPERATORS = {"stored": 0, "pairwise": 1}
 
 # Every symbol include/rigidfibers.h declares.
 EXPORTS = (
@@ -56,7 +57,8 @@ class RfConfig(C.Structure):
         ("refinement_max_iterations", C.c_int32),
         ("initial_guess", C.c_int32),
         ("matrix_format", C.c_int32),
-        ("reserved", C.c_int32 * 6),
+        ("velocity_operator", C.c_int32),
+        ("reserved", C.c_int32 * 5),
     ]
 
 

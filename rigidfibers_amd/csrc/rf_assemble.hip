@@ -261,23 +261,55 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
 // (lane = (source, q_i)); phase 2 jobs (column b, source, mode) are ordered so
 // that 40 consecutive lanes write 40 consecutive floats of each component.
 
+// Velocity operator (when the velocities integrate like the system): the
+// reference's velocity sums (fortran/update_velocities.incl:29-86) are
+// projections of the same inner integrals G_m(q_i) the assembly forms,
+//   sum_j TFA0_ij = sum_j [Phi0_{ij,0} F/2 + sum_k Phi0_{ij,k+1} x_{j,k}]   (weights w)
+//   sum_j TFA1_ij = sum_j [Theta_{ij,0,P0} F/2 + sum_k Theta_{ij,0,k} x_{j,k}] (weights w P_1)
+// so with Tv != nullptr the kernel also stores, per (i, j, k_j), the
+// symmetric Theta_{0,k_j} and Phi0_{k_j+1} = sum_{q_i} w G_{k_j+1}(q_i)
+//   Tv[(i * 12 + c) * Lp + j * N + k_j],  c = s (Theta_0), 6 + s (Phi0),
+// and in cpart[group][i][6] the constant sums of Phi0_{ij,0} F/2 and
+// Theta_{ij,0,P0} F/2 over its sources.
+
+// Column b of a symmetric 3x3 projection (v_a = S[(a, b)]) into its component
+// arrays s = xx yy zz xy xz yz at stride Lp: b = 0 writes xx xy xz, b = 1 yy yz,
+// b = 2 zz, so the three column jobs together write each component once.
+__device__ __forceinline__ void store_sym_column(float* base, int64_t Lp, int bcol, float v0, float v1, float v2) {
+  if (bcol == 0) {
+    base[0] = v0;       // xx
+    base[3 * Lp] = v1;  // xy
+    base[4 * Lp] = v2;  // xz
+  } else if (bcol == 1) {
+    base[Lp] = v1;      // yy
+    base[5 * Lp] = v2;  // yz
+  } else {
+    base[2 * Lp] = v2;  // zz
+  }
+}
+
 template <int N, int NQI, bool ANALYTIC>
-__global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int64_t Lp,
+// (3 waves per SIMD: the FP64 moments of the analytical form otherwise take
+// the register count just past the 168 that allows it)
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_kernel(
+    int M, int64_t Lp,
                                                                       const double* __restrict__ pos,
                                                                       const double* __restrict__ ori,
                                                                       float* __restrict__ T,
                                                                       double* __restrict__ bpart, int jgroup,
-                                                                      PhysConsts pc) {
+                                                                      PhysConsts pc, float* __restrict__ Tv,
+                                                                      double* __restrict__ cpart) {
   constexpr Quad<N, NQI> quad{};
   constexpr int Q = 3 * NQI;
   constexpr int PAIRS = kPairThreads / Q;
   constexpr int NM = N + 1;
   constexpr int ROWS = 3 * N;
   constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int NR = ROWS + 6;  // RHS rows + the two velocity constants
   static_assert(kPairThreads % Q == 0, "Q must divide the workgroup");
 
   __shared__ float lds[PAIRS][NM][6][Q];
-  __shared__ double bred[PAIRS][ROWS];
+  __shared__ double bred[PAIRS][NR];
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -294,8 +326,10 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
   const float sq = static_cast<float>(sq_d);
   float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
 
-  // RHS sums of each source slot, kept in LDS (only PAIRS lanes update them)
-  for (int r = tid; r < PAIRS * ROWS; r += kPairThreads) bred[r / ROWS][r % ROWS] = 0.0;
+  // RHS (and velocity-constant) sums of each source slot, kept in LDS (only
+  // PAIRS lanes update them)
+  for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
+  float* Vi = Tv ? Tv + static_cast<int64_t>(i) * 12 * Lp : nullptr;
   __syncthreads();
 
   constexpr int kJobs = PAIRS + 3 * PAIRS * N;
@@ -316,7 +350,7 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
       if (job < PAIRS) {  // RHS job of source slot p2
         const int p2 = job, j2 = j0 + p2;
         if (j2 >= je || j2 == i) continue;
-        float TF[N][3];
+        float TF[N][3], TV[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int k = 0; k < N; ++k) TF[k][0] = TF[k][1] = TF[k][2] = 0.0f;
 #pragma unroll
@@ -332,6 +366,9 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
             TF[k][1] = fmaf(quad.W[k + 1][q], f1, TF[k][1]);
             TF[k][2] = fmaf(quad.W[k + 1][q], f2, TF[k][2]);
           }
+          TV[0] = fmaf(quad.W[0][q], f0, TV[0]);
+          TV[1] = fmaf(quad.W[0][q], f1, TV[1]);
+          TV[2] = fmaf(quad.W[0][q], f2, TV[2]);
         }
 #pragma unroll
         for (int k = 0; k < N; ++k) {
@@ -341,6 +378,12 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
             const float v = (k == 0) ? pc.D1 * ti[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * ti[a] * QF);
             bred[p2][3 * k + a] -= static_cast<double>(v);
           }
+        }
+        // velocity constants: sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          bred[p2][ROWS + a] += static_cast<double>(TV[a]);
+          bred[p2][ROWS + 3 + a] += static_cast<double>(TF[0][a]);
         }
         continue;
       }
@@ -352,27 +395,18 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
       const int j2 = j0 + p2;
       if (j2 >= je) continue;
       const int64_t e = static_cast<int64_t>(j2) * N + (m - 1);
-      // components written by column b: b=0 -> xx xy xz, b=1 -> yy yz, b=2 -> zz
       if (j2 == i) {
         Ti[bcol * Lp + e] = 0.0f;
 #pragma unroll
-        for (int k = 1; k < N; ++k) {
-          float* Tk = Ti + (3 + 6 * (k - 1)) * Lp + e;
-          if (bcol == 0) {
-            Tk[0 * Lp] = 0.0f;
-            Tk[3 * Lp] = 0.0f;
-            Tk[4 * Lp] = 0.0f;
-          } else if (bcol == 1) {
-            Tk[1 * Lp] = 0.0f;
-            Tk[5 * Lp] = 0.0f;
-          } else {
-            Tk[2 * Lp] = 0.0f;
-          }
+        for (int k = 1; k < N; ++k) store_sym_column(Ti + (3 + 6 * (k - 1)) * Lp + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
+        if (Vi) {
+          store_sym_column(Vi + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
+          store_sym_column(Vi + 6 * Lp + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
         }
         continue;
       }
       const int c0 = sym_index(0, bcol), c1 = sym_index(1, bcol), c2 = sym_index(2, bcol);
-      float Th[N][3];
+      float Th[N][3], Ph[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int k = 0; k < N; ++k) Th[k][0] = Th[k][1] = Th[k][2] = 0.0f;
 #pragma unroll
@@ -384,22 +418,20 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
           Th[k][1] = fmaf(quad.W[k + 1][q], g1, Th[k][1]);
           Th[k][2] = fmaf(quad.W[k + 1][q], g2, Th[k][2]);
         }
+        if (Vi) {
+          Ph[0] = fmaf(quad.W[0][q], g0, Ph[0]);
+          Ph[1] = fmaf(quad.W[0][q], g1, Ph[1]);
+          Ph[2] = fmaf(quad.W[0][q], g2, Ph[2]);
+        }
       }
       // Q0_b = (Theta_0 t_i)_b
       Ti[bcol * Lp + e] = Th[0][0] * ti[0] + Th[0][1] * ti[1] + Th[0][2] * ti[2];
 #pragma unroll
-      for (int k = 1; k < N; ++k) {
-        float* Tk = Ti + (3 + 6 * (k - 1)) * Lp + e;
-        if (bcol == 0) {
-          Tk[0 * Lp] = Th[k][0];  // xx
-          Tk[3 * Lp] = Th[k][1];  // xy
-          Tk[4 * Lp] = Th[k][2];  // xz
-        } else if (bcol == 1) {
-          Tk[1 * Lp] = Th[k][1];  // yy
-          Tk[5 * Lp] = Th[k][2];  // yz
-        } else {
-          Tk[2 * Lp] = Th[k][2];  // zz
-        }
+      for (int k = 1; k < N; ++k)
+        store_sym_column(Ti + (3 + 6 * (k - 1)) * Lp + e, Lp, bcol, Th[k][0], Th[k][1], Th[k][2]);
+      if (Vi) {
+        store_sym_column(Vi + e, Lp, bcol, Th[0][0], Th[0][1], Th[0][2]);
+        store_sym_column(Vi + 6 * Lp + e, Lp, bcol, Ph[0], Ph[1], Ph[2]);
       }
     }
     __syncthreads();
@@ -409,6 +441,10 @@ __global__ __launch_bounds__(kPairThreads) void assemble_theta_kernel(int M, int
     double s = 0.0;
     for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
     bpart[static_cast<int64_t>(blockIdx.y) * n + static_cast<int64_t>(ROWS) * i + tid] = s;
+  } else if (cpart && tid < NR) {
+    double s = 0.0;
+    for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
+    cpart[(static_cast<int64_t>(blockIdx.y) * M + i) * 6 + (tid - ROWS)] = s;
   }
 }
 
@@ -547,32 +583,33 @@ hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, 
 
 template <int N, int NQI, bool AN>
 static hipError_t launch_theta_t(int M, int64_t Lp, const double* pos, const double* ori, float* T, double* bpart,
-                                 const PhysConsts& pc, hipStream_t s, int* groups) {
+                                 float* Tv, double* cpart, const PhysConsts& pc, hipStream_t s, int* groups) {
   constexpr int PAIRS = kPairThreads / (3 * NQI);
   const PairGrid g = theta_grid(M, PAIRS);
   *groups = g.groups;
   hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M, Lp, pos,
-                     ori, T, bpart, g.jgroup, pc);
+                     ori, T, bpart, g.jgroup, pc, Tv, cpart);
   return hipGetLastError();
 }
 
 template <int N, int NQI>
 static hipError_t launch_theta_n(bool an, int M, int64_t Lp, const double* pos, const double* ori, float* T,
-                                 double* bpart, const PhysConsts& pc, hipStream_t s, int* groups) {
-  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, pc, s, groups)
-            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, pc, s, groups);
+                                 double* bpart, float* Tv, double* cpart, const PhysConsts& pc, hipStream_t s,
+                                 int* groups) {
+  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups)
+            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups);
 }
 
 #define RF_TH_CASE(NN)                                                                             \
   case NN:                                                                                         \
-    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
-    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
-    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, pc, s, groups); \
+    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
+    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
+    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
     return hipErrorInvalidValue;
 
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
-                                 const double* ori, float* T, double* bpart, const PhysConsts& pc, hipStream_t s,
-                                 int* groups) {
+                                 const double* ori, float* T, double* bpart, float* Tv, double* cpart,
+                                 const PhysConsts& pc, hipStream_t s, int* groups) {
   switch (N) {
     RF_TH_CASE(1)
     RF_TH_CASE(2)

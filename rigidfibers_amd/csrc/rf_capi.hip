@@ -76,6 +76,10 @@ struct rf_simulation {
   float* T = nullptr;
   int64_t Lp = 0;         // theta component stride
   int bgroups = 1;        // RHS partial slots written by the last assembly
+  bool vstored = false;   // the theta assembly also stores the velocity operator
+  bool vvalid = false;    // ... and it belongs to the current state
+  float* Tv = nullptr;    // 12 component arrays of stride Lp per target fiber
+  double* cpart = nullptr;
   double* b = nullptr;
   double* x = nullptr;
   double* bpart = nullptr;
@@ -148,6 +152,8 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
   if (c->velocity_integration < RF_VELOCITY_FOLLOW || c->velocity_integration > RF_VELOCITY_ANALYTICAL)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity integration");
+  if (c->velocity_operator != RF_VELOCITY_STORED && c->velocity_operator != RF_VELOCITY_PAIRWISE)
+    return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity operator");
   if (c->solver == RF_SOLVER_GMRES) {
     if (c->gmres_restart < 1 || c->gmres_restart > 1024)
       return fail(RF_ERR_INVALID_ARGUMENT, "GMRES RESTART must be in [1..1024]");
@@ -522,6 +528,10 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     const int NC = 3 + 6 * (s->N - 1);
     s->Lp = rf::theta_stride(M, s->N);
     if (st == RF_OK) st = s->alloc(&s->T, static_cast<size_t>(M) * NC * static_cast<size_t>(s->Lp));
+    s->vstored = cfg->velocity_operator == RF_VELOCITY_STORED && s->vel_analytical == s->analytical;
+    if (s->vstored && st == RF_OK) st = s->alloc(&s->Tv, static_cast<size_t>(M) * 12 * static_cast<size_t>(s->Lp));
+    if (s->vstored && st == RF_OK)
+      st = s->alloc(&s->cpart, static_cast<size_t>(rf::max_pair_groups(M)) * M * 6);
   } else if (st == RF_OK) {
     st = s->alloc(&s->A, static_cast<size_t>(s->lda) * static_cast<size_t>(n));
   }
@@ -581,7 +591,8 @@ rf_status rf_assemble_system(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
   if (s->theta) {
     RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
-                                     s->bpart, s->pc, s->stream, &s->bgroups));
+                                     s->bpart, s->Tv, s->cpart, s->pc, s->stream, &s->bgroups));
+    s->vvalid = s->vstored;
   } else {
     RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
                                s->bpart, s->pc, s->stream));
@@ -611,8 +622,12 @@ rf_status rf_solve_system(rf_simulation* s) {
 
 rf_status rf_update_velocities(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
-  RF_HIP(rf::launch_velocities(s->N, s->NQI, s->vel_analytical, s->M, s->pos[s->cur], s->ori[s->cur], s->x, s->wf,
-                               s->fm, s->vpart, s->tv[s->vcur], s->rv[s->vcur], s->pc, s->stream));
+  if (s->vstored && s->vvalid)
+    RF_HIP(rf::launch_velocities_stored(s->N, s->M, s->Lp, s->Tv, s->cpart, s->bgroups, s->ori[s->cur], s->x, s->pc,
+                                        s->tv[s->vcur], s->rv[s->vcur], s->stream));
+  else
+    RF_HIP(rf::launch_velocities(s->N, s->NQI, s->vel_analytical, s->M, s->pos[s->cur], s->ori[s->cur], s->x,
+                                 s->wf, s->fm, s->vpart, s->tv[s->vcur], s->rv[s->vcur], s->pc, s->stream));
   s->vlast = s->vcur;
   return RF_OK;
 }
@@ -638,6 +653,7 @@ rf_status rf_advance(rf_simulation* s) {
   std::swap(s->vcur, s->vprev);
   s->have_history = true;
   s->have_next = false;
+  s->vvalid = false;  // the stored velocity operator belongs to the old state
   return RF_OK;
 }
 
@@ -716,6 +732,7 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
   RF_HIP(hipStreamSynchronize(s->stream));
   s->have_history = history != 0;
   s->have_next = false;
+  s->vvalid = false;
   return RF_OK;
 }
 

@@ -28,12 +28,20 @@ hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double
 
 // Theta form (rf_assemble.hip): T holds NC = 3 + 6(N-1) component arrays of
 // stride Lp = theta_stride(M, N) per target fiber; *groups receives the number
-// of RHS partial slots written.
+// of RHS partial slots written.  With Tv != nullptr it also stores the
+// velocity operator: 12 component arrays of the same stride per target fiber
+// (Theta_0 and Phi0 of every source mode) and the velocity constants
+// cpart[group][i][6] (cpart then holds max_pair_groups(M) * M * 6 doubles).
 PairGrid theta_grid(int M, int sources_per_iteration);
 int64_t theta_stride(int M, int N);
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
-                                 const double* ori, float* T, double* bpart, const PhysConsts& pc, hipStream_t s,
-                                 int* groups);
+                                 const double* ori, float* T, double* bpart, float* Tv, double* cpart,
+                                 const PhysConsts& pc, hipStream_t s, int* groups);
+// Velocities from the stored operator of the last theta assembly (one pass
+// over Tv): tvel, rvel [3*M] FP64.
+hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, const double* cpart, int groups,
+                                    const double* ori, const double* x, const PhysConsts& pc, double* tvel,
+                                    double* rvel, hipStream_t s);
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
                                float* A, hipStream_t s);
 // y = A x (mode 0) or y = b - A x (mode 1) with A in theta form.

@@ -190,6 +190,80 @@ __global__ void velocity_finalize_kernel(int M, int groups, const double* __rest
   }
 }
 
+// Velocities from the operator the theta assembly stored: per target i,
+//   S_v = c_v + sum_{j,k} Phi0_{ij,k+1} x_{j,k},   S_w = c_w + sum_{j,k} Theta_{ij,0,k} x_{j,k}
+// (Tv components s = xx yy zz xy xz yz of Theta_0, then of Phi0), which are the
+// sums velocity_kernel forms pair by pair.  One workgroup per target fiber
+// streams its 12 component arrays once; FP64 accumulation, fixed-order
+// reduction.
+constexpr int kVelStoredThreads = 256;
+
+__global__ __launch_bounds__(kVelStoredThreads) void velocity_stored_kernel(
+    int M, int N, int64_t Lp, const float* __restrict__ Tv, const double* __restrict__ cpart, int groups,
+    const double* __restrict__ ori, const double* __restrict__ x, PhysConsts pc, double* __restrict__ tvel,
+    double* __restrict__ rvel) {
+  __shared__ double red[kVelStoredThreads / 64][6];
+  const int i = blockIdx.x;
+  const float* Vi = Tv + static_cast<int64_t>(i) * 12 * Lp;
+  const int L = M * N;
+  double sw[3] = {0.0, 0.0, 0.0}, sv[3] = {0.0, 0.0, 0.0};
+  for (int e = threadIdx.x; e < L; e += kVelStoredThreads) {
+    const double x0 = x[3 * static_cast<int64_t>(e)], x1 = x[3 * static_cast<int64_t>(e) + 1],
+                 x2 = x[3 * static_cast<int64_t>(e) + 2];
+    const float* p = Vi + e;
+    const double wxx = __builtin_nontemporal_load(p), wyy = __builtin_nontemporal_load(p + Lp),
+                 wzz = __builtin_nontemporal_load(p + 2 * Lp), wxy = __builtin_nontemporal_load(p + 3 * Lp),
+                 wxz = __builtin_nontemporal_load(p + 4 * Lp), wyz = __builtin_nontemporal_load(p + 5 * Lp);
+    const double vxx = __builtin_nontemporal_load(p + 6 * Lp), vyy = __builtin_nontemporal_load(p + 7 * Lp),
+                 vzz = __builtin_nontemporal_load(p + 8 * Lp), vxy = __builtin_nontemporal_load(p + 9 * Lp),
+                 vxz = __builtin_nontemporal_load(p + 10 * Lp), vyz = __builtin_nontemporal_load(p + 11 * Lp);
+    sw[0] = fma(wxx, x0, fma(wxy, x1, fma(wxz, x2, sw[0])));
+    sw[1] = fma(wxy, x0, fma(wyy, x1, fma(wyz, x2, sw[1])));
+    sw[2] = fma(wxz, x0, fma(wyz, x1, fma(wzz, x2, sw[2])));
+    sv[0] = fma(vxx, x0, fma(vxy, x1, fma(vxz, x2, sv[0])));
+    sv[1] = fma(vxy, x0, fma(vyy, x1, fma(vyz, x2, sv[1])));
+    sv[2] = fma(vxz, x0, fma(vyz, x1, fma(vzz, x2, sv[2])));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double acc[6] = {sv[0], sv[1], sv[2], sw[0], sw[1], sw[2]};
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double v = acc[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double S[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += cpart[(static_cast<int64_t>(g) * M + i) * 6 + c];
+#pragma unroll
+    for (int w = 0; w < kVelStoredThreads / 64; ++w) s += red[w][c];
+    S[c] = s;
+  }
+  const double t[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
+  const double F[3] = {2.0 * pc.Fh_d[0], 2.0 * pc.Fh_d[1], 2.0 * pc.Fh_d[2]};
+  const double d = pc.d;
+  const double tF = t[0] * F[0] + t[1] * F[1] + t[2] * F[2];
+  const double tS = t[0] * S[3] + t[1] * S[4] + t[2] * S[5];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    tvel[3 * i + a] = (0.5 / d) * ((d + 2.0) * F[a] + (d - 2.0) * tF * t[a] + S[a]);
+    rvel[3 * i + a] = (1.5 / d) * (S[3 + a] - t[a] * tS);
+  }
+}
+
+hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, const double* cpart, int groups,
+                                    const double* ori, const double* x, const PhysConsts& pc, double* tvel,
+                                    double* rvel, hipStream_t s) {
+  hipLaunchKernelGGL(velocity_stored_kernel, dim3(M), dim3(kVelStoredThreads), 0, s, M, N, Lp, Tv, cpart, groups, ori,
+                     x, pc, tvel, rvel);
+  return hipGetLastError();
+}
+
 __global__ void update_kernel(int first, int M, double dt, const double* __restrict__ pos_prev,
                               const double* __restrict__ pos, const double* __restrict__ ori_prev,
                               const double* __restrict__ ori, const double* __restrict__ tv_prev,

@@ -113,24 +113,62 @@ def test_assembly_matches_oracle(oracle64, oracle32, N, nqi, analytical):
         assert ea < 2e-6 and eb < 2e-6
 
 
-@pytest.mark.parametrize("analytical", [False, True])
-def test_velocities_match_oracle(oracle64, analytical):
+@pytest.mark.parametrize("operator", ["stored", "pairwise"])
+@pytest.mark.parametrize("N,nqi,analytical", [(5, 8, False), (5, 8, True), (3, 4, True), (8, 16, False),
+                                              (1, 8, True)])
+def test_velocities_match_oracle(oracle64, N, nqi, analytical, operator):
     """Velocities from the GPU's own solution against the oracle's velocity
-    routine fed the same solution (isolates the velocity kernel)."""
+    routine fed the same solution (isolates the velocity update), for the
+    operator stored by the theta assembly and for the pairwise kernel."""
     from oracle.pyoracle import make_params
 
     M = 40
     pos, ori = random_scene(M, side=5.0, seed=11, min_dist=0.4)
     params = ref_config()
     params.NUMBER_OF_FIBERS = M
+    params.NUMBER_OF_TERMS_IN_FORCE_EXPANSION = N
+    params.NUMBER_OF_QUADRATURE_INTERVALS = nqi
     params.GMRES_TOLERANCE = 1e-10
-    with _sim(params, pos, ori, solver="gmres", integration="analytical" if analytical else "numerical") as sim:
+    with _sim(params, pos, ori, solver="gmres", integration="analytical" if analytical else "numerical",
+              velocity_operator=operator) as sim:
         sim.assemble_system()
         sim.solve_system()
         _, _, x = sim.system()
         sim.update_velocities()
         v, w = sim.velocities()
-    v_ref, w_ref = oracle64.velocities(make_params(M), pos, ori, x, analytical)
+    v_ref, w_ref = oracle64.velocities(make_params(M, N=N, nqi=nqi), pos, ori, x, analytical)
+    ev, ew = rel_err(v, v_ref), rel_err(w, w_ref)
+    print(f"N={N} nqi={nqi} analytical={analytical} {operator}: v {ev:.1e} w {ew:.1e}")
+    assert ev < 1e-6 and ew < 1e-6
+
+
+def test_stored_velocities_follow_state_changes(oracle64):
+    """After rf_set_state the stored velocity operator belongs to the old state:
+    rf_update_velocities must not use it (pairwise fallback), and a new
+    assembly makes it current again."""
+    from oracle.pyoracle import make_params
+
+    M = 30
+    pos, ori = random_scene(M, side=4.5, seed=5, min_dist=0.4)
+    pos2, ori2 = random_scene(M, side=4.5, seed=6, min_dist=0.4)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-10
+    with _sim(params, pos, ori) as sim:
+        sim.assemble_system()
+        sim.solve_system()
+        _, _, x = sim.system()
+        sim.set_state(pos2, ori2)
+        sim.update_velocities()  # operator of (pos, ori) is stale: pairwise for (pos2, ori2)
+        v, w = sim.velocities()
+        v_ref, w_ref = oracle64.velocities(make_params(M), pos2, ori2, x, False)
+        assert rel_err(v, v_ref) < 1e-6 and rel_err(w, w_ref) < 1e-6
+        sim.assemble_system()
+        sim.solve_system()
+        _, _, x2 = sim.system()
+        sim.update_velocities()
+        v, w = sim.velocities()
+    v_ref, w_ref = oracle64.velocities(make_params(M), pos2, ori2, x2, False)
     assert rel_err(v, v_ref) < 1e-6 and rel_err(w, w_ref) < 1e-6
 
 
