@@ -6,7 +6,7 @@ of length 2 with centres uniform in a ball of radius 35 (minimum centre
 distance 0.4, seed 2026), orientations uniform on the sphere, slenderness 0.01,
 N = 5 force terms, 8 x 3 quadrature, unit gravity, dt = 0.1, analytical inner
 integrals (FP64 recurrences), GMRES(10) to a relative residual of 1e-6 starting
-from the previous solution.  n = 75 000 unknowns, A = 22.5 GB FP32 resident in HBM.
+from the previous solution.  n = 75 000 unknowns.
 
 One "step" is one call of the library's rf_step (assemble -> solve -> update
 velocities -> update fibers) on device-resident state.  W warm-up steps are
@@ -14,11 +14,17 @@ run untimed, then exactly K steps are timed between two device
 synchronisations.  value = M * K / elapsed.
 
 Also reported:
-  roofline      the matvec of the solver (HBM-bound): its algorithmic bytes
-                per launch (theta form: M * M * N * (3 + 6(N-1)) * 4, the FP32
-                numbers that define A; dense: n^2 * 4) / mean launch duration
-                (HIP events on the library's stream), against 8 TB/s;
-  kernels       per-phase times and the assembly kernel's rate;
+  roofline      iterative solvers: the solver's matvec (HBM-bound), algorithmic
+                bytes per launch (theta form: M * M * N * (3 + 6(N-1)) * 4, the
+                FP32 numbers that define A; dense: n^2 * 4) / mean launch
+                duration (HIP events on the library's stream), against 8 TB/s.
+                Direct solvers: the LU's trailing-update GEMMs (MFMA-bound),
+                their 2 m n K flops / summed duration, against the dense MFMA
+                peak of the mode (FP32 157.3, BF16/FP16 2500 TFLOP/s);
+  precision     plain FP32 LU against the BF16- and FP16-MFMA LUs with FP64
+                refinement (to 1e-10) on the same scene: fiber-steps/s, the
+                speed-up, and the relative position error against the FP32-LU
+                run after the same steps;
   cpu_baseline  the FP32 OpenMP port of the reference Fortran (oracle build
                 librf_oracle32.so) on a bounded sample: the rows of S target
                 fibers assembled against all 5000 sources, their velocities,
@@ -36,7 +42,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FP32_VALU_PEAK_TFLOPS = 157.3
+MFMA_PEAK_TFLOPS = {"direct": 157.3, "mixed-bf16": 2500.0, "mixed-fp16": 2500.0}
+GEMM_KERNEL = {"direct": "gemm_packed_sub_kernel<float>", "mixed-bf16": "gemm_packed_sub_kernel<__bf16>",
+               "mixed-fp16": "gemm_packed_sub_kernel<_Float16>"}
+ITERATIVE = ("gmres", "bicgstab")
 
 
 def build_scene(M, radius, seed):
@@ -45,7 +54,7 @@ def build_scene(M, radius, seed):
     return ball(M, radius, min_dist=0.4, seed=seed)
 
 
-def run_gpu(args, pos, ori):
+def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
     from rigidfibers_amd.config import Parameters
     from rigidfibers_amd.simulation import Simulation
 
@@ -53,22 +62,25 @@ def run_gpu(args, pos, ori):
     params = Parameters(NUMBER_OF_FIBERS=M, TIMESTEP=0.1, SLENDERNESS=0.01, NUMBER_OF_TERMS_IN_FORCE_EXPANSION=5,
                         NUMBER_OF_QUADRATURE_INTERVALS=8, GMRES_RESTART=args.restart,
                         GMRES_MAX_ITERATIONS=1000, GMRES_TOLERANCE=args.tol, BICGSTAB_TOLERANCE=args.tol)
-    sim = Simulation(params, pos, ori, solver=args.solver, integration=args.integration,
-                     matrix_format=args.matrix_format, velocity_operator=args.velocity_operator)
+    matrix_format = args.matrix_format if solver in ITERATIVE else "auto"
+    sim = Simulation(params, pos, ori, solver=solver, integration=args.integration, matrix_format=matrix_format,
+                     velocity_operator=args.velocity_operator, refinement_tolerance=1e-10)
     try:
-        for t in range(args.warmup):
+        for t in range(warmup):
             sim.step(t)
-            print(f"# warmup step {t}: {sim.timings()['total']:.1f} ms, solver {sim.solver_stats()}", flush=True)
+            print(f"# {tag}{solver} warmup step {t}: {sim.timings()['total']:.1f} ms, solver {sim.solver_stats()}",
+                  flush=True)
         phases = {"assemble_system": 0.0, "solve_system": 0.0, "update_velocities": 0.0, "update_fibers": 0.0}
-        gemv_ms, gemv_n, iters = 0.0, 0, []
+        acc = {"gemv": 0.0, "gemv_launches": 0, "gemm": 0.0, "gemm_flop": 0.0, "gemm_launches": 0}
+        iters = []
         t0 = time.perf_counter()
-        for t in range(args.warmup, args.warmup + args.steps):
+        for t in range(warmup, warmup + steps):
             sim.step(t)
             tm = sim.timings()
             for k in phases:
                 phases[k] += tm[k]
-            gemv_ms += tm["gemv"]
-            gemv_n += tm["gemv_launches"]
+            for k in acc:
+                acc[k] += tm[k]
             iters.append(sim.solver_stats()["iterations"])
         elapsed = time.perf_counter() - t0
         stats = sim.solver_stats()
@@ -77,8 +89,88 @@ def run_gpu(args, pos, ori):
         n = sim.n
     finally:
         sim.close()
-    return {"elapsed": elapsed, "phases": phases, "gemv_ms": gemv_ms, "gemv_n": gemv_n, "iters": iters,
-            "stats": stats, "n": n, "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end}
+    return {"elapsed": elapsed, "phases": phases, "acc": acc, "iters": iters, "stats": stats, "n": n,
+            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps}
+
+
+def roofline(args, solver, g):
+    M, N = args.fibers, 5
+    acc = g["acc"]
+    if solver in ITERATIVE:
+        theta = args.matrix_format == "theta" or args.matrix_format == "auto"
+        if theta:  # 3 + 6(N-1) FP32 numbers per (target, source, source mode)
+            per_launch, kernel = float(M) * (3 + 6 * (N - 1)) * M * N * 4.0, "gemv_theta_kernel"
+        else:      # n^2 FP32 entries
+            per_launch, kernel = float(g["n"]) ** 2 * 4.0, "gemv_f32_kernel"
+        avg_ms = acc["gemv"] / max(acc["gemv_launches"], 1)
+        achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": per_launch,
+                "avg_launch_ms": round(avg_ms, 4)}
+    peak = MFMA_PEAK_TFLOPS[solver]
+    achieved = acc["gemm_flop"] / (acc["gemm"] * 1e-3) / 1e12 if acc["gemm"] > 0 else 0.0
+    return {"bound": "mfma", "kernel": GEMM_KERNEL[solver], "achieved": round(achieved, 1), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+            "flop_per_step": acc["gemm_flop"] / g["steps"], "launches_per_step": acc["gemm_launches"] / g["steps"],
+            "avg_launch_ms": round(acc["gemm"] / max(acc["gemm_launches"], 1), 4)}
+
+
+def rel_err(a, b):
+    import numpy as np
+
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
+
+
+def precision_study(args, pos, ori):
+    """Plain FP32 LU vs the BF16/FP16-MFMA LUs with FP64 refinement, same scene and steps."""
+    out, ref = {}, None
+    for solver in ("direct", "mixed-bf16", "mixed-fp16"):
+        g = run_gpu(args, pos, ori, solver, args.lu_steps, 1, tag="precision ")
+        rate = args.fibers * g["steps"] / g["elapsed"]
+        entry = {"fiber_steps_per_s": round(rate, 2), "ms_per_step": round(1e3 * g["elapsed"] / g["steps"], 2),
+                 "solve_ms_per_step": round(g["phases"]["solve_system"] / g["steps"], 2),
+                 "refinement_sweeps": g["iters"], "relative_residual": g["stats"]["relative_residual"],
+                 "roofline": roofline(args, solver, g)}
+        if ref is None:
+            ref = (g["pos"], g["ori"], rate)
+        else:
+            entry["speedup_vs_fp32_lu"] = round(rate / ref[2], 3)
+            entry["pos_rel_err_vs_fp32_lu"] = rel_err(g["pos"], ref[0])
+            entry["ori_rel_err_vs_fp32_lu"] = rel_err(g["ori"], ref[1])
+        out["plain_fp32_lu" if solver == "direct" else solver.replace("-", "_")] = entry
+    out["steps"] = args.lu_steps
+    out["warmup"] = 1
+    return out
+
+
+def serving(args):
+    """The serving stream: suspensions with M ~ U[16, 400] from the cube
+    generator at number density 0.1 (own seed each), 20 steps each, numerical
+    integrals, GMRES tol 1e-6, Poisson arrivals; served with continuous
+    batching, then the same stream one request at a time."""
+    from rigidfibers_amd.config import Parameters
+    from rigidfibers_amd.serve import Server, request_stream, serial_stream, serve_stream, summarize
+
+    params = Parameters(NUMBER_OF_FIBERS=1, TIMESTEP=0.1, SLENDERNESS=0.01, NUMBER_OF_TERMS_IN_FORCE_EXPANSION=5,
+                        NUMBER_OF_QUADRATURE_INTERVALS=8, GMRES_RESTART=10, GMRES_MAX_ITERATIONS=1000,
+                        GMRES_TOLERANCE=1e-6)
+    reqs = request_stream(count=args.serve_requests, m_range=(16, 400), density=0.1, rate=args.serve_rate,
+                          steps=20, seed=args.seed)
+    with Server(params, integration="numerical", memory_budget=0, max_active=args.serve_slots) as srv:
+        res, wall = serve_stream(srv, reqs)
+        st = srv.stats()
+    out = {"workload": f"{len(reqs)} suspensions, M ~ U[16,400], cube at density 0.1, 20 steps, numerical, "
+                       f"GMRES tol 1e-6, Poisson {args.serve_rate:g}/s",
+           "batched": summarize(res, wall),
+           "server": {"ticks": st["ticks"], "busy_s": round(st["busy_seconds"], 3), "slots": args.serve_slots,
+                      "peak_bytes": st["peak_bytes"], "memory_budget": st["memory_budget"],
+                      "failed": sum(1 for r in res if r["status"] not in (0, 100))}}
+    if args.serve_serial > 0:
+        sub = reqs[:args.serve_serial]
+        sres, swall = serial_stream(params, sub, integration="numerical")
+        out["one_at_a_time"] = summarize(sres, swall)
+        out["one_at_a_time"]["requests_of_stream"] = len(sub)
+    return out
 
 
 def cpu_baseline(args, pos, ori, iters_per_step):
@@ -138,6 +230,13 @@ def main():
     ap.add_argument("--restart", type=int, default=10)
     ap.add_argument("--matrix-format", default="auto", choices=["auto", "dense", "theta"])
     ap.add_argument("--velocity-operator", default="stored", choices=["stored", "pairwise"])
+    ap.add_argument("--lu-steps", type=int, default=2, help="timed steps per solver of the precision study")
+    ap.add_argument("--no-precision-study", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-slots", type=int, default=64, help="suspensions stepped together")
+    ap.add_argument("--serve-serial", type=int, default=512, help="requests of the one-at-a-time comparison")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -147,24 +246,10 @@ def main():
         sys.exit(2)
 
     pos, ori = build_scene(args.fibers, args.radius, args.seed)
-    g = run_gpu(args, pos, ori)
-    M, K, n = args.fibers, args.steps, g["n"]
+    g = run_gpu(args, pos, ori, args.solver, args.steps, args.warmup)
+    M, K = args.fibers, args.steps
     value = M * K / g["elapsed"]
-    gemv_avg_ms = g["gemv_ms"] / max(g["gemv_n"], 1)
-    iterative = args.solver in ("gmres", "bicgstab")
-    theta = args.matrix_format == "theta" or (args.matrix_format == "auto" and iterative)
-    N = 5
-    if theta:  # 3 + 6(N-1) FP32 numbers per (target, source, source mode)
-        gemv_bytes = float(M) * (3 + 6 * (N - 1)) * M * N * 4.0
-        gemv_kernel = "gemv_theta_kernel"
-    else:      # n^2 FP32 entries
-        gemv_bytes = float(n) * n * 4.0
-        gemv_kernel = "gemv_f32_kernel"
-    achieved = gemv_bytes / (gemv_avg_ms * 1e-3) / 1e9
-    asm_ms = g["phases"]["assemble_system"] / K
-    # algorithmic count of the analytical assembly: per ordered pair and target
-    # node, FP64 moments (~2 log + 2 sqrt + 26 x 11 recurrence ops) and the
-    # G of 6 modes (~6 x 110) plus the projection (~(3N+1) x 3N x Q / Q per node)
+    fmt = ("theta" if args.matrix_format in ("auto", "theta") else "dense") if args.solver in ITERATIVE else "dense"
     line = {
         "metric": "fiber-steps/s at M=5000 (plain FP32 vs mixed-prec, pos err<=1e-6); p99 serve latency",
         "value": round(value, 2),
@@ -181,19 +266,20 @@ def main():
         "config": {"workload": f"rigid-fiber timestep, M={M} fibers in a ball of radius {args.radius:g}, "
                                f"min centre distance 0.4, N=5, 8x3 quadrature, {args.integration} integrals, "
                                f"{args.solver} tol {args.tol:g} (restart {args.restart}), dt=0.1",
-                   "fibers": M, "unknowns": n, "solver": args.solver, "integration": args.integration,
-                   "matrix_format": "theta" if theta else "dense"},
-        "roofline": {"bound": "hbm", "kernel": gemv_kernel, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": gemv_bytes, "avg_launch_ms": round(gemv_avg_ms, 4)},
+                   "fibers": M, "unknowns": g["n"], "solver": args.solver, "integration": args.integration,
+                   "matrix_format": fmt},
+        "roofline": roofline(args, args.solver, g),
         "kernels": {k: round(v / K, 3) for k, v in g["phases"].items()},
         "solver": {"iterations_per_step": g["iters"], "last": g["stats"]},
         "device_bytes": g["device_bytes"],
     }
-    line["kernels"]["gemv_ms_per_step"] = round(g["gemv_ms"] / K, 3)
-    line["kernels"]["assemble_ms_per_step"] = round(asm_ms, 3)
+    line["kernels"]["gemv_ms_per_step"] = round(g["acc"]["gemv"] / K, 3)
+    if not args.no_precision_study:
+        line["precision"] = precision_study(args, pos, ori)
+    if not args.no_serving:
+        line["serving"] = serving(args)
     if not args.no_cpu_baseline:
-        iters = sum(g["iters"]) / len(g["iters"]) + 1.0
+        iters = sum(g["iters"]) / len(g["iters"]) + 1.0 if args.solver in ITERATIVE else 1.0
         line["cpu_baseline"] = cpu_baseline(args, pos, ori, iters)
     print(json.dumps(line), flush=True)
 

@@ -49,7 +49,9 @@ typedef enum rf_solver {
   RF_SOLVER_BICGSTAB = 1,
   RF_SOLVER_DIRECT = 2,            /* FP32 LU with partial pivoting (magma_sgesv_gpu / sgesv) */
   RF_SOLVER_DIRECT_MIXED_BF16 = 3, /* BF16-MFMA LU + FP64 iterative refinement */
-  RF_SOLVER_DIRECT_MIXED_FP16 = 4  /* FP16-MFMA LU + FP64 iterative refinement */
+  RF_SOLVER_DIRECT_MIXED_FP16 = 4, /* FP16-MFMA LU + FP64 iterative refinement */
+  RF_SOLVER_GMRES_IR = 5           /* GMRES on the FP16-stored theta system inside an iterative refinement
+                                      whose residual uses the FP32 theta system in FP64 */
 } rf_solver;
 
 /* --numerical / --analytical (fibers_runner.py:105-108). */
@@ -115,7 +117,8 @@ typedef struct rf_config {
   int32_t initial_guess;                              /* rf_initial_guess */
   int32_t matrix_format;                              /* rf_matrix_format */
   int32_t velocity_operator;                          /* rf_velocity_operator */
-  int32_t reserved[5];
+  double inner_tolerance;                             /* GMRES-IR: relative residual of each inner solve (0: 1e-4) */
+  int32_t reserved[4];
 } rf_config;
 
 typedef struct rf_solver_stats {
@@ -134,6 +137,11 @@ typedef struct rf_timings {
   double total;
   double gemv;             /* summed duration of the matrix-vector kernels of the solve */
   int32_t gemv_launches;   /* how many there were */
+  int32_t gemm_launches;   /* direct solvers: trailing-update GEMMs of the LU factorisation */
+  double gemm;             /* their summed duration */
+  double gemm_flop;        /* their algorithmic flops, 2 m n K each */
+  double gemv_half;        /* GMRES-IR: summed duration of the FP16 matrix-vector kernels */
+  int32_t gemv_half_launches;
   int32_t reserved;
 } rf_timings;
 
@@ -194,6 +202,66 @@ RF_API int64_t rf_device_bytes(const rf_simulation* sim);
 
 RF_API const char* rf_last_error(void);
 RF_API const char* rf_version(void);
+
+/* ------------------------------------------------------------------------
+ * Serving: many independent suspensions stepped together on one GPU.
+ *
+ * No counterpart in the reference (one suspension per process,
+ * src/main.cu:83-99); each suspension is the reference's Simulation with its
+ * own fiber count, state and number of steps.  A worker thread inside the
+ * server admits queued requests when a slot and device memory are free
+ * (footprint = the suspension's theta-form system, velocity operator, Krylov
+ * basis and state), and advances every admitted suspension one timestep per
+ * tick: one launch covers the assembly of all of them, one the matrix-vector
+ * products of a lockstep GMRES iteration, one the velocities, one the update.
+ * Every per-suspension reduction has a fixed order, so a suspension's result
+ * is bitwise the same alone and in any mix.  The shared settings come from
+ * rf_server_config.base (GMRES is the solver; its tolerance, restart and
+ * iteration limit apply to each suspension separately). */
+typedef struct rf_server rf_server;
+
+typedef struct rf_server_config {
+  rf_config base;            /* physics and GMRES settings shared by all suspensions */
+  int64_t memory_budget;     /* device bytes for admitted suspensions; 0 = 80% of the free memory */
+  int32_t max_active;        /* suspensions stepped together, >= 1 */
+  int32_t reserved[5];
+} rf_server_config;
+
+typedef struct rf_request_info {
+  int64_t id;
+  int32_t fibers, steps;
+  int32_t status;            /* RF_OK, RF_WARN_NOT_CONVERGED (some step hit the iteration limit) or an error */
+  int32_t gmres_iterations;  /* summed over the steps */
+  double submitted, admitted, finished;  /* seconds on the server's clock */
+  int64_t footprint;         /* device bytes while admitted */
+} rf_request_info;
+
+typedef struct rf_server_stats {
+  int64_t ticks, fiber_steps, completed, queued, active;
+  double busy_seconds;       /* wall time spent in ticks */
+  int64_t peak_bytes;        /* device bytes held by admitted suspensions, high-water mark */
+  int64_t memory_budget;
+} rf_server_stats;
+
+/* Starts the worker thread on the current device. */
+RF_API rf_status rf_server_create(const rf_server_config* cfg, rf_server** out);
+/* Stops the worker after the current tick and frees everything. */
+RF_API void rf_server_destroy(rf_server* srv);
+/* Queues a suspension (positions/orientations copied); *id identifies it.
+ * RF_ERR_OUT_OF_MEMORY if its footprint alone exceeds the budget. */
+RF_API rf_status rf_server_submit(rf_server* srv, int32_t fibers, const double* positions,
+                                  const double* orientations, int32_t steps, int64_t* id);
+/* Blocks until the request has finished (timeout_s < 0: no limit); *done = 1
+ * if it has. */
+RF_API rf_status rf_server_wait(rf_server* srv, int64_t id, double timeout_s, int32_t* done);
+/* Final state (3*fibers each, may be NULL), per-step latencies in ms (steps
+ * values, may be NULL: the wall time of the tick that advanced the step) and
+ * the request's record; then forgets the request. */
+RF_API rf_status rf_server_collect(rf_server* srv, int64_t id, double* positions, double* orientations,
+                                   double* step_ms, rf_request_info* info);
+RF_API rf_status rf_server_get_stats(rf_server* srv, rf_server_stats* stats);
+/* Seconds on the server's clock (the times in rf_request_info). */
+RF_API double rf_server_clock(rf_server* srv);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,7 @@ RF_ERR_NONFINITE = 5
 RF_ERR_SINGULAR = 6
 RF_WARN_NOT_CONVERGED = 100
 
-SOLVERS = {"gmres": 0, "bicgstab": 1, "direct": 2, "magma": 2, "mixed-bf16": 3, "mixed-fp16": 4}
+SOLVERS = {"gmres": 0, "bicgstab": 1, "direct": 2, "magma": 2, "mixed-bf16": 3, "mixed-fp16": 4, "gmres-ir": 5}
 INTEGRATIONS = {"numerical": 0, "analytical": 1}
 VELOCITY_INTEGRATIONS = {"follow": 0, "numerical": 1, "analytical": 2}
 INITIAL_GUESSES = {"zero": 0, "previous": 1}
@@ -33,6 +33,8 @@ EXPORTS = (
     "rf_update_velocities", "rf_update_fibers", "rf_advance", "rf_get_state", "rf_get_next_state",
     "rf_get_velocities", "rf_set_state", "rf_get_system", "rf_system_size", "rf_get_solver_stats",
     "rf_get_timings", "rf_device_bytes", "rf_last_error", "rf_version",
+    "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_collect",
+    "rf_server_get_stats", "rf_server_clock",
 )
 
 
@@ -58,7 +60,8 @@ class RfConfig(C.Structure):
         ("initial_guess", C.c_int32),
         ("matrix_format", C.c_int32),
         ("velocity_operator", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("inner_tolerance", C.c_double),
+        ("reserved", C.c_int32 * 4),
     ]
 
 
@@ -75,7 +78,48 @@ class RfTimings(C.Structure):
         ("total", C.c_double),
         ("gemv", C.c_double),
         ("gemv_launches", C.c_int32),
+        ("gemm_launches", C.c_int32),
+        ("gemm", C.c_double),
+        ("gemm_flop", C.c_double),
+        ("gemv_half", C.c_double),
+        ("gemv_half_launches", C.c_int32),
         ("reserved", C.c_int32),
+    ]
+
+
+class RfServerConfig(C.Structure):
+    _fields_ = [
+        ("base", RfConfig),
+        ("memory_budget", C.c_int64),
+        ("max_active", C.c_int32),
+        ("reserved", C.c_int32 * 5),
+    ]
+
+
+class RfRequestInfo(C.Structure):
+    _fields_ = [
+        ("id", C.c_int64),
+        ("fibers", C.c_int32),
+        ("steps", C.c_int32),
+        ("status", C.c_int32),
+        ("gmres_iterations", C.c_int32),
+        ("submitted", C.c_double),
+        ("admitted", C.c_double),
+        ("finished", C.c_double),
+        ("footprint", C.c_int64),
+    ]
+
+
+class RfServerStats(C.Structure):
+    _fields_ = [
+        ("ticks", C.c_int64),
+        ("fiber_steps", C.c_int64),
+        ("completed", C.c_int64),
+        ("queued", C.c_int64),
+        ("active", C.c_int64),
+        ("busy_seconds", C.c_double),
+        ("peak_bytes", C.c_int64),
+        ("memory_budget", C.c_int64),
     ]
 
 
@@ -114,6 +158,13 @@ def load(path: str = LIB_PATH):
         "rf_device_bytes": ([sim], C.c_int64),
         "rf_last_error": ([], C.c_char_p),
         "rf_version": ([], C.c_char_p),
+        "rf_server_create": ([C.POINTER(RfServerConfig), C.POINTER(C.c_void_p)], C.c_int),
+        "rf_server_destroy": ([C.c_void_p], None),
+        "rf_server_submit": ([C.c_void_p, C.c_int32, dp, dp, C.c_int32, C.POINTER(C.c_int64)], C.c_int),
+        "rf_server_wait": ([C.c_void_p, C.c_int64, C.c_double, C.POINTER(C.c_int32)], C.c_int),
+        "rf_server_collect": ([C.c_void_p, C.c_int64, dp, dp, dp, C.POINTER(RfRequestInfo)], C.c_int),
+        "rf_server_get_stats": ([C.c_void_p, C.POINTER(RfServerStats)], C.c_int),
+        "rf_server_clock": ([C.c_void_p], C.c_double),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)

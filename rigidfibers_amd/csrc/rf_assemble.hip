@@ -288,17 +288,12 @@ __device__ __forceinline__ void store_sym_column(float* base, int64_t Lp, int bc
   }
 }
 
+// The work of one workgroup: target fiber i, source group gy.
 template <int N, int NQI, bool ANALYTIC>
-// (3 waves per SIMD: the FP64 moments of the analytical form otherwise take
-// the register count just past the 168 that allows it)
-__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_kernel(
-    int M, int64_t Lp,
-                                                                      const double* __restrict__ pos,
-                                                                      const double* __restrict__ ori,
-                                                                      float* __restrict__ T,
-                                                                      double* __restrict__ bpart, int jgroup,
-                                                                      PhysConsts pc, float* __restrict__ Tv,
-                                                                      double* __restrict__ cpart) {
+__device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64_t Lp, const double* __restrict__ pos,
+                                                     const double* __restrict__ ori, float* __restrict__ T,
+                                                     double* __restrict__ bpart, int jgroup, const PhysConsts& pc,
+                                                     float* __restrict__ Tv, double* __restrict__ cpart) {
   constexpr Quad<N, NQI> quad{};
   constexpr int Q = 3 * NQI;
   constexpr int PAIRS = kPairThreads / Q;
@@ -314,8 +309,7 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3,
   const int tid = threadIdx.x;
   const int p = tid / Q;
   const int qi = tid - p * Q;
-  const int i = blockIdx.x;
-  const int jb = blockIdx.y * jgroup;
+  const int jb = gy * jgroup;
   const int je = min(M, jb + jgroup);
   const int64_t n = static_cast<int64_t>(ROWS) * M;
 
@@ -440,12 +434,31 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3,
   if (tid < ROWS) {  // the loop ended on a barrier: bred is complete
     double s = 0.0;
     for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
-    bpart[static_cast<int64_t>(blockIdx.y) * n + static_cast<int64_t>(ROWS) * i + tid] = s;
+    bpart[static_cast<int64_t>(gy) * n + static_cast<int64_t>(ROWS) * i + tid] = s;
   } else if (cpart && tid < NR) {
     double s = 0.0;
     for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
-    cpart[(static_cast<int64_t>(blockIdx.y) * M + i) * 6 + (tid - ROWS)] = s;
+    cpart[(static_cast<int64_t>(gy) * M + i) * 6 + (tid - ROWS)] = s;
   }
+}
+
+// (3 waves per SIMD: the FP64 moments of the analytical form otherwise take
+// the register count just past the 168 that allows it)
+template <int N, int NQI, bool ANALYTIC>
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_kernel(
+    int M, int64_t Lp, const double* __restrict__ pos, const double* __restrict__ ori, float* __restrict__ T,
+    double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart) {
+  assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart);
+}
+
+// Serving: workgroup = (suspension, target fiber), all sources in one group,
+// so b and the velocity constants are written directly.
+template <int N, int NQI, bool ANALYTIC>
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_batch_kernel(
+    const SusDev* __restrict__ sus, const int2* __restrict__ targets, PhysConsts pc) {
+  const int2 e = targets[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  assemble_theta_block<N, NQI, ANALYTIC>(e.y, 0, d.M, d.Lp, d.pos, d.ori, d.T, d.b, d.M, pc, d.Tv, d.cpart);
 }
 
 // Dense A (column-major, leading dimension n) from the theta form; identity
@@ -647,6 +660,42 @@ hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const d
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <int N, int NQI>
+static hipError_t launch_theta_batch_n(bool an, int ntargets, const SusDev* sus, const int2* targets,
+                                       const PhysConsts& pc, hipStream_t s) {
+  if (an)
+    hipLaunchKernelGGL((assemble_theta_batch_kernel<N, NQI, true>), dim3(ntargets), dim3(kPairThreads), 0, s, sus,
+                       targets, pc);
+  else
+    hipLaunchKernelGGL((assemble_theta_batch_kernel<N, NQI, false>), dim3(ntargets), dim3(kPairThreads), 0, s, sus,
+                       targets, pc);
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble_theta_batch(int N, int NQI, bool analytical, int ntargets, const SusDev* sus,
+                                       const int2* targets, const PhysConsts& pc, hipStream_t s) {
+  if (ntargets <= 0) return hipSuccess;
+  switch (N) {
+#define RF_TB(NN)                                                                                 \
+  case NN:                                                                                        \
+    if (NQI == 4) return launch_theta_batch_n<NN, 4>(analytical, ntargets, sus, targets, pc, s);   \
+    if (NQI == 8) return launch_theta_batch_n<NN, 8>(analytical, ntargets, sus, targets, pc, s);   \
+    if (NQI == 16) return launch_theta_batch_n<NN, 16>(analytical, ntargets, sus, targets, pc, s); \
+    return hipErrorInvalidValue;
+    RF_TB(1)
+    RF_TB(2)
+    RF_TB(3)
+    RF_TB(4)
+    RF_TB(5)
+    RF_TB(6)
+    RF_TB(7)
+    RF_TB(8)
+#undef RF_TB
+    default:
+      return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double* out, hipStream_t s) {

@@ -14,6 +14,7 @@
 // lib/viennacl/linalg/bicgstab.hpp:92-160.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -22,6 +23,7 @@
 #include <vector>
 
 #include "../../include/rigidfibers.h"
+#include "rf_host.h"
 #include "rf_internal.h"
 #include "rf_lu.h"
 
@@ -86,6 +88,8 @@ struct rf_simulation {
   float* wf = nullptr;
   double* fm = nullptr;
   double* vpart = nullptr;
+  _Float16* T16 = nullptr;  // GMRES-IR: FP16 copy of T
+  float* S16 = nullptr;     // and its per-block scales
 
   // Krylov / refinement workspace
   int kdim = 0;           // basis vectors allocated
@@ -97,6 +101,7 @@ struct rf_simulation {
   int* flag = nullptr;
   rf::GemvPlan plan{};
   rf::LuWorkspace lu{};
+  rf::LuResult lu_last{};  // factorisation statistics of the last direct solve
 
   rf_solver_stats stats{};
   rf_timings timings{};
@@ -104,6 +109,8 @@ struct rf_simulation {
   static constexpr int kGemvEvents = 256;
   hipEvent_t gev[2 * kGemvEvents] = {};
   int ngemv = 0;
+  hipEvent_t hev[2 * kGemvEvents] = {};  // the same for the FP16 products of GMRES-IR
+  int nhalf = 0;
   int64_t device_bytes = 0;
   std::vector<void*> allocations;
 
@@ -142,7 +149,7 @@ rf_status validate_config(const rf_config* c) {
   if (!std::isfinite(c->timestep)) return fail(RF_ERR_INVALID_ARGUMENT, "TIMESTEP must be finite");
   for (int a = 0; a < 3; ++a)
     if (!std::isfinite(c->external_force[a])) return fail(RF_ERR_INVALID_ARGUMENT, "external force must be finite");
-  if (c->solver < RF_SOLVER_GMRES || c->solver > RF_SOLVER_DIRECT_MIXED_FP16)
+  if (c->solver < RF_SOLVER_GMRES || c->solver > RF_SOLVER_GMRES_IR)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown solver");
   if (c->integration != RF_INTEGRATION_NUMERICAL && c->integration != RF_INTEGRATION_ANALYTICAL)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
@@ -154,7 +161,9 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity integration");
   if (c->velocity_operator != RF_VELOCITY_STORED && c->velocity_operator != RF_VELOCITY_PAIRWISE)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity operator");
-  if (c->solver == RF_SOLVER_GMRES) {
+  if (c->solver == RF_SOLVER_GMRES_IR && !(c->inner_tolerance >= 0.0 && c->inner_tolerance < 1.0))
+    return fail(RF_ERR_INVALID_ARGUMENT, "GMRES-IR inner tolerance must be in [0, 1)");
+  if (c->solver == RF_SOLVER_GMRES || c->solver == RF_SOLVER_GMRES_IR) {
     if (c->gmres_restart < 1 || c->gmres_restart > 1024)
       return fail(RF_ERR_INVALID_ARGUMENT, "GMRES RESTART must be in [1..1024]");
     if (c->gmres_max_iterations < 1) return fail(RF_ERR_INVALID_ARGUMENT, "GMRES MAX_ITERATIONS must be >= 1");
@@ -196,12 +205,24 @@ void make_consts(const rf_config& c, rf::PhysConsts* pc) {
   }
 }
 
-int solver_kdim(const rf_config& c) { return (c.solver == RF_SOLVER_GMRES) ? c.gmres_restart + 1 : 0; }
+int solver_kdim(const rf_config& c) {
+  return (c.solver == RF_SOLVER_GMRES || c.solver == RF_SOLVER_GMRES_IR) ? c.gmres_restart + 1 : 0;
+}
 
 // y = A x or y = b - A x on the simulation's stream, timed with an event pair
 // while the pool lasts.
-hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, const double* b, int mode) {
+hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, const double* b, int mode,
+                    bool half = false) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (half) {
+    if (s->nhalf < rf_simulation::kGemvEvents) {
+      e0 = s->hev[2 * s->nhalf];
+      e1 = s->hev[2 * s->nhalf + 1];
+      ++s->nhalf;
+    }
+    return rf::gemv_theta16(s->N, s->M, s->Lp, s->T16, s->S16, s->ori[s->cur], x, y, b, mode, s->pc, s->stream, e0,
+                            e1);
+  }
   if (s->ngemv < rf_simulation::kGemvEvents) {
     e0 = s->gev[2 * s->ngemv];
     e1 = s->gev[2 * s->ngemv + 1];
@@ -240,23 +261,24 @@ rf_status device_dot(rf_simulation* s, const double* u, const double* v, double*
 }
 
 // Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
-rf_status solve_gmres(rf_simulation* s) {
+// GMRES(m) for A x = b: A the FP32 system, or its FP16 copy (half).  Starts
+// from x when warm, else from 0; stops at ||b - A x|| <= tol ||b|| or after
+// maxit iterations.  *out receives the iterations, convergence and residual.
+rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, double tol, int maxit, bool half,
+                     rf_solver_stats* out) {
   const int64_t n = s->n;
   const int m = s->cfg.gmres_restart;
-  const int maxit = s->cfg.gmres_max_iterations;
-  const double tol = s->cfg.gmres_tolerance;
   double* V = s->V;
   auto Vk = [&](int k) { return V + static_cast<int64_t>(k) * n; };
 
   double bnorm = 0.0;
-  RF_TRY(device_norm(s, s->b, &bnorm));
-  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
-  if (!warm) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
-  s->stats = rf_solver_stats{0, 0, 0.0};
+  RF_TRY(device_norm(s, b, &bnorm));
+  if (!warm) RF_HIP(rf::fill(n, 0.0, x, s->stream));
+  *out = rf_solver_stats{0, 0, 0.0};
   if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
   if (bnorm == 0.0) {
-    RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
-    s->stats.converged = 1;
+    RF_HIP(rf::fill(n, 0.0, x, s->stream));
+    out->converged = 1;
     return RF_OK;
   }
   std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
@@ -267,12 +289,12 @@ rf_status solve_gmres(rf_simulation* s) {
   while (true) {
     double beta;
     if (first && !warm) {
-      RF_HIP(rf::axpby(n, 1.0, s->b, 0.0, Vk(0), s->stream));
+      RF_HIP(rf::axpby(n, 1.0, b, 0.0, Vk(0), s->stream));
       beta = bnorm;
       first = false;
     } else {
       first = false;
-      RF_HIP(sim_gemv(s, s->x, s->gpart, Vk(0), s->b, 1));
+      RF_HIP(sim_gemv(s, x, s->gpart, Vk(0), b, 1, half));
       RF_TRY(device_norm(s, Vk(0), &beta));
     }
     res = beta;
@@ -289,7 +311,7 @@ rf_status solve_gmres(rf_simulation* s) {
     for (int k = 0; k < m && it < maxit; ++k) {
       ++it;
       double* w = Vk(k + 1);
-      RF_HIP(sim_gemv(s, Vk(k), s->gpart, w, nullptr, 0));
+      RF_HIP(sim_gemv(s, Vk(k), s->gpart, w, nullptr, 0, half));
       // h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||^2
       double* h1 = s->hdev;
       double* h2 = s->hdev + (m + 1);
@@ -330,12 +352,60 @@ rf_status solve_gmres(rf_simulation* s) {
       y[i] = acc / H[i + static_cast<size_t>(i) * (m + 1)];
     }
     RF_HIP(hipMemcpyAsync(s->hdev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s->stream));
-    RF_HIP(rf::axpy_multi(n, k_used, V, n, s->hdev, 1.0, s->x, s->stream));
+    RF_HIP(rf::axpy_multi(n, k_used, V, n, s->hdev, 1.0, x, s->stream));
     // the loop head recomputes the true residual b - A x
   }
+  out->iterations = it;
+  out->converged = converged ? 1 : 0;
+  out->relative_residual = res / bnorm;
+  return RF_OK;
+}
+
+rf_status solve_gmres(rf_simulation* s) {
+  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  return gmres_core(s, s->b, s->x, warm, s->cfg.gmres_tolerance, s->cfg.gmres_max_iterations, false, &s->stats);
+}
+
+// GMRES-IR: x <- x + d with d from GMRES on the FP16 copy, A16 d = r, each
+// outer residual r = b - A x from the FP32 system in FP64, until
+// ||r|| <= tol ||b||.  The inner solves stop at inner_tolerance relative to
+// ||r|| (no tighter than the outer target needs).
+rf_status solve_gmres_ir(rf_simulation* s) {
+  const int64_t n = s->n;
+  const double tol = s->cfg.gmres_tolerance;
+  const double itol = s->cfg.inner_tolerance > 0.0 ? s->cfg.inner_tolerance : 1e-4;
+  const int maxit = s->cfg.gmres_max_iterations;
+  double* r = s->work;
+  double* d = s->work + n;
+  double bnorm = 0.0;
+  RF_TRY(device_norm(s, s->b, &bnorm));
+  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  if (!warm || bnorm == 0.0) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
+  s->stats = rf_solver_stats{0, 0, 0.0};
+  if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
+  if (bnorm == 0.0) {
+    s->stats.converged = 1;
+    return RF_OK;
+  }
+  int it = 0;
+  while (true) {
+    RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1, false));
+    double res = 0.0;
+    RF_TRY(device_norm(s, r, &res));
+    if (!std::isfinite(res)) return fail(RF_ERR_NONFINITE, "non-finite residual in GMRES-IR");
+    s->stats.relative_residual = res / bnorm;
+    if (res / bnorm <= tol) {
+      s->stats.converged = 1;
+      break;
+    }
+    if (it >= maxit) break;
+    rf_solver_stats inner{};
+    const double target = std::max(itol, 0.5 * tol * bnorm / res);
+    RF_TRY(gmres_core(s, r, d, false, target, maxit - it, true, &inner));
+    it += std::max(inner.iterations, 1);
+    RF_HIP(rf::axpby(n, 1.0, d, 1.0, s->x, s->stream));
+  }
   s->stats.iterations = it;
-  s->stats.converged = converged ? 1 : 0;
-  s->stats.relative_residual = res / bnorm;
   return RF_OK;
 }
 
@@ -433,6 +503,7 @@ rf_status solve_direct(rf_simulation* s) {
                               s->A, s->lda, s->n, s->b, s->x, s->gpart, s->plan, s->work, s->dpart, s->hdev,
                               s->cfg.refinement_tolerance, s->cfg.refinement_max_iterations, s->stream, &r);
   if (e != hipSuccess) return fail(RF_ERR_DEVICE, std::string("LU: ") + hipGetErrorString(e));
+  s->lu_last = r;
   if (r.info != 0) return fail(RF_ERR_SINGULAR, "zero pivot at column " + std::to_string(r.info));
   s->stats.iterations = r.refinement_iterations;
   s->stats.relative_residual = r.relative_residual;
@@ -448,6 +519,12 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 }
 
 }  // namespace
+
+namespace rf {
+rf_status set_error(rf_status code, const std::string& msg) { return fail(code, msg); }
+rf_status check_config(const rf_config* c) { return validate_config(c); }
+void physical_constants(const rf_config& c, PhysConsts* pc) { make_consts(c, pc); }
+}  // namespace rf
 
 extern "C" {
 
@@ -509,6 +586,8 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
   for (auto& e : s->gev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
+  for (auto& e : s->hev)
+    if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
 
   const int64_t n = s->n;
   const size_t fiber3 = 3 * static_cast<size_t>(M);
@@ -521,7 +600,10 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     st = s->alloc(&s->tv[k], fiber3);
     if (st == RF_OK) st = s->alloc(&s->rv[k], fiber3);
   }
-  const bool iterative = cfg->solver == RF_SOLVER_GMRES || cfg->solver == RF_SOLVER_BICGSTAB;
+  const bool iterative =
+      cfg->solver == RF_SOLVER_GMRES || cfg->solver == RF_SOLVER_BICGSTAB || cfg->solver == RF_SOLVER_GMRES_IR;
+  if (cfg->solver == RF_SOLVER_GMRES_IR && cfg->matrix_format == RF_MATRIX_DENSE)
+    return cleanup(fail(RF_ERR_UNSUPPORTED, "GMRES-IR needs the theta form"));
   s->theta = (cfg->matrix_format == RF_MATRIX_THETA) || (cfg->matrix_format == RF_MATRIX_AUTO && iterative);
   if (s->theta && !iterative) return cleanup(fail(RF_ERR_UNSUPPORTED, "the direct solvers need the dense matrix"));
   if (s->theta) {
@@ -532,6 +614,10 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     if (s->vstored && st == RF_OK) st = s->alloc(&s->Tv, static_cast<size_t>(M) * 12 * static_cast<size_t>(s->Lp));
     if (s->vstored && st == RF_OK)
       st = s->alloc(&s->cpart, static_cast<size_t>(rf::max_pair_groups(M)) * M * 6);
+    if (cfg->solver == RF_SOLVER_GMRES_IR) {
+      if (st == RF_OK) st = s->alloc(&s->T16, static_cast<size_t>(M) * NC * static_cast<size_t>(s->Lp));
+      if (st == RF_OK) st = s->alloc(&s->S16, static_cast<size_t>(M) * static_cast<size_t>(M));
+    }
   } else if (st == RF_OK) {
     st = s->alloc(&s->A, static_cast<size_t>(s->lda) * static_cast<size_t>(n));
   }
@@ -583,6 +669,8 @@ void rf_destroy(rf_simulation* s) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : s->gev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : s->hev)
+    if (e) (void)hipEventDestroy(e);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -593,6 +681,7 @@ rf_status rf_assemble_system(rf_simulation* s) {
     RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
                                      s->bpart, s->Tv, s->cpart, s->pc, s->stream, &s->bgroups));
     s->vvalid = s->vstored;
+    if (s->T16) RF_HIP(rf::theta_to_fp16(s->N, s->M, s->Lp, s->T, s->T16, s->S16, s->stream));
   } else {
     RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
                                s->bpart, s->pc, s->stream));
@@ -605,9 +694,14 @@ rf_status rf_assemble_system(rf_simulation* s) {
 rf_status rf_solve_system(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
   s->ngemv = 0;
+  s->nhalf = 0;
+  s->lu_last = rf::LuResult{};
   switch (s->cfg.solver) {
     case RF_SOLVER_GMRES:
       RF_TRY(solve_gmres(s));
+      break;
+    case RF_SOLVER_GMRES_IR:
+      RF_TRY(solve_gmres_ir(s));
       break;
     case RF_SOLVER_BICGSTAB:
       RF_TRY(solve_bicgstab(s));
@@ -678,6 +772,12 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.gemv = 0.0;
   for (int k = 0; k < s->ngemv; ++k) s->timings.gemv += elapsed(s->gev[2 * k], s->gev[2 * k + 1]);
   s->timings.gemv_launches = s->ngemv;
+  s->timings.gemv_half = 0.0;
+  for (int k = 0; k < s->nhalf; ++k) s->timings.gemv_half += elapsed(s->hev[2 * k], s->hev[2 * k + 1]);
+  s->timings.gemv_half_launches = s->nhalf;
+  s->timings.gemm_launches = s->lu_last.gemm_launches;
+  s->timings.gemm = s->lu_last.gemm_ms;
+  s->timings.gemm_flop = s->lu_last.gemm_flop;
   RF_TRY(rf_advance(s));
   if (!s->stats.converged) {
     g_last_error = "solver did not reach its tolerance (relative residual " +

@@ -44,6 +44,12 @@ hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, c
                                     double* rvel, hipStream_t s);
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
                                float* A, hipStream_t s);
+// FP16 copy of T with one FP32 scale per (target, source) block, S[i * M + j]
+// (rf_linalg.hip), and the theta GEMV on it.
+hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s);
+hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
+                        const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,
+                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // y = A x (mode 0) or y = b - A x (mode 1) with A in theta form.
 hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
                       const double* b, int mode, const PhysConsts& pc, hipStream_t s, hipEvent_t ev_begin = nullptr,
@@ -56,6 +62,68 @@ hipError_t launch_update(bool first, int M, double dt, const double* pos_prev, c
                          const double* rv_prev, const double* rv, double* pos_next, double* ori_next,
                          hipStream_t s);
 bool assemble_supported(int N, int NQI);
+
+// ---------------------------------------------------------------------------
+// Batched launches of the serving engine (rf_serve.hip): one launch covers
+// every admitted suspension.  Workgroups find their suspension through tables
+// of int2 entries (suspension, fiber) or (suspension, 256-element block).
+
+// Device pointers of one suspension for the current tick.
+struct SusDev {
+  int M, n;
+  int64_t Lp;
+  int vb_first, vb_count;          // its range in the vector-block table
+  const double* pos;               // current state
+  const double* ori;
+  float* T;                        // theta-form system (groups of one: b written directly)
+  float* Tv;                       // stored velocity operator
+  double* cpart;                   // velocity constants [M][6]
+  double* b;
+  double* x;
+  double* V;                       // Krylov basis, kdim x n
+  double* tv;                      // velocities of this step
+  double* rv;
+  const double* pos_prev;
+  const double* ori_prev;
+  const double* tv_prev;
+  const double* rv_prev;
+  double* pos_next;
+  double* ori_next;
+  int first;                       // Euler step
+};
+
+// One lockstep GMRES iteration of one suspension.
+struct SusIter {
+  int zero_x;                      // x = 0 first
+  int ycount;                      // then x += V[0..ycount) ycoef
+  const double* ycoef;
+  double* svec;                    // then svec *= sval (nullptr: none)
+  double sval;
+  const double* gx;                // then gy = A gx (gmode 0) or b - A gx (1); nullptr: none
+  double* gy;
+  int gmode;
+  double* w;                       // orthogonalised against V[0..K), then ||w||^2
+  int K;
+  double* hout;                    // 3 (kdim + 1): <V_j, w> twice, ||w||^2
+};
+
+hipError_t launch_assemble_theta_batch(int N, int NQI, bool analytical, int ntargets, const SusDev* sus,
+                                       const int2* targets, const PhysConsts& pc, hipStream_t s);
+hipError_t launch_gemv_theta_batch(int N, int ntargets, const SusDev* sus, const SusIter* it, const int2* targets,
+                                   const PhysConsts& pc, hipStream_t s);
+hipError_t launch_velocities_stored_batch(int N, int ntargets, const SusDev* sus, const int2* targets,
+                                          const PhysConsts& pc, hipStream_t s);
+hipError_t launch_update_batch(int nfblocks, const SusDev* sus, const int2* fblocks, double dt, hipStream_t s);
+// x = 0 / x += V y / svec *= sval, per suspension (before the iteration's GEMV)
+hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it, const int2* vblocks,
+                                  hipStream_t s);
+// phase 0/1: hout[phase (kdim+1) + j] = <V_j, w>, j < K; phase 2: hout[2 (kdim+1)] = <w, w>.
+// part: nvb * kdim doubles.
+hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, const SusDev* sus,
+                             const SusIter* it, const int2* vblocks, double* part, hipStream_t s);
+// w -= V[0..K) hout[phase (kdim+1) ..]
+hipError_t launch_orth_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
+                             const int2* vblocks, hipStream_t s);
 
 // Dense linear algebra (rf_linalg.hip).
 struct GemvPlan {

@@ -129,14 +129,13 @@ hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part
 // bytes); FP64 accumulation, workgroup reduction, 15 outputs.
 
 template <int N>
-__global__ __launch_bounds__(256) void gemv_theta_kernel(int M, int64_t Lp, const float* __restrict__ T,
-                                                         const double* __restrict__ ori,
-                                                         const double* __restrict__ x, double* __restrict__ y,
-                                                         const double* __restrict__ b, int mode, PhysConsts pc) {
+__device__ __forceinline__ void gemv_theta_block(int i, int M, int64_t Lp, const float* __restrict__ T,
+                                                 const double* __restrict__ ori, const double* __restrict__ x,
+                                                 double* __restrict__ y, const double* __restrict__ b, int mode,
+                                                 const PhysConsts& pc) {
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int NACC = 1 + 3 * (N - 1);
   __shared__ double red[256 / 64][NACC];
-  const int i = blockIdx.x;
   const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
   const int L = M * N;
   double acc[NACC];
@@ -191,6 +190,50 @@ __global__ __launch_bounds__(256) void gemv_theta_kernel(int M, int64_t Lp, cons
     out += x[row];  // identity diagonal block
     y[row] = (mode == 1) ? b[row] - out : out;
   }
+}
+
+template <int N>
+__global__ __launch_bounds__(256) void gemv_theta_kernel(int M, int64_t Lp, const float* __restrict__ T,
+                                                         const double* __restrict__ ori,
+                                                         const double* __restrict__ x, double* __restrict__ y,
+                                                         const double* __restrict__ b, int mode, PhysConsts pc) {
+  gemv_theta_block<N>(blockIdx.x, M, Lp, T, ori, x, y, b, mode, pc);
+}
+
+// Serving: workgroup = (suspension, target fiber) of the suspensions with a
+// product in this iteration.
+template <int N>
+__global__ __launch_bounds__(256) void gemv_theta_batch_kernel(const SusDev* __restrict__ sus,
+                                                               const SusIter* __restrict__ it,
+                                                               const int2* __restrict__ targets, PhysConsts pc) {
+  const int2 e = targets[blockIdx.x];
+  const SusIter& c = it[e.x];
+  if (c.gx == nullptr) return;
+  const SusDev& d = sus[e.x];
+  gemv_theta_block<N>(e.y, d.M, d.Lp, d.T, d.ori, c.gx, c.gy, d.b, c.gmode, pc);
+}
+
+hipError_t launch_gemv_theta_batch(int N, int ntargets, const SusDev* sus, const SusIter* it, const int2* targets,
+                                   const PhysConsts& pc, hipStream_t s) {
+  if (ntargets <= 0) return hipSuccess;
+  switch (N) {
+#define RF_GB(NN)                                                                                            \
+  case NN:                                                                                                   \
+    hipLaunchKernelGGL(gemv_theta_batch_kernel<NN>, dim3(ntargets), dim3(256), 0, s, sus, it, targets, pc); \
+    break;
+    RF_GB(1)
+    RF_GB(2)
+    RF_GB(3)
+    RF_GB(4)
+    RF_GB(5)
+    RF_GB(6)
+    RF_GB(7)
+    RF_GB(8)
+#undef RF_GB
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
@@ -326,6 +369,286 @@ __global__ void nonfinite_kernel(int64_t n, const double* __restrict__ x, int* _
 hipError_t nonfinite_count(int64_t n, const double* x, int* flag_dev, hipStream_t s) {
   hipLaunchKernelGGL(nonfinite_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, x,
                      flag_dev);
+  return hipGetLastError();
+}
+
+// Vector work of one lockstep GMRES iteration over 256-element blocks of
+// every suspension's vectors (entry (suspension, block) of the table).
+constexpr int kVecBlock = 256;
+
+__global__ __launch_bounds__(kVecBlock) void iter_prep_batch_kernel(const SusDev* __restrict__ sus,
+                                                                    const SusIter* __restrict__ it,
+                                                                    const int2* __restrict__ vblocks) {
+  const int2 e = vblocks[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  const SusIter& c = it[e.x];
+  const int r = e.y * kVecBlock + threadIdx.x;
+  if (r >= d.n) return;
+  if (c.zero_x) d.x[r] = 0.0;
+  if (c.ycount > 0) {
+    double s = 0.0;
+    for (int k = 0; k < c.ycount; ++k) s = fma(d.V[static_cast<int64_t>(k) * d.n + r], c.ycoef[k], s);
+    d.x[r] += s;
+  }
+  if (c.svec) c.svec[r] *= c.sval;
+}
+
+// Block partial sums: phase 0/1 <V_j, w> for j < K (blockIdx.y = j), phase 2 <w, w>.
+__global__ __launch_bounds__(kVecBlock) void dots_batch_kernel(int phase, int nvb, const SusDev* __restrict__ sus,
+                                                               const SusIter* __restrict__ it,
+                                                               const int2* __restrict__ vblocks,
+                                                               double* __restrict__ part) {
+  __shared__ double sh[kVecBlock / 64];
+  const int2 e = vblocks[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  const SusIter& c = it[e.x];
+  const int j = blockIdx.y;
+  if (c.w == nullptr || (phase < 2 && j >= c.K) || (phase == 2 && j > 0)) return;
+  const int r = e.y * kVecBlock + threadIdx.x;
+  double v = 0.0;
+  if (r < d.n) {
+    const double wr = c.w[r];
+    v = (phase < 2) ? d.V[static_cast<int64_t>(j) * d.n + r] * wr : wr * wr;
+  }
+  const double t = block_sum(v, sh);
+  if (threadIdx.x == 0) part[static_cast<int64_t>(j) * nvb + blockIdx.x] = t;
+}
+
+// Sum of a suspension's block partials in table order -> hout.
+__global__ __launch_bounds__(64) void dots_final_batch_kernel(int phase, int nvb, int kdim,
+                                                              const SusDev* __restrict__ sus,
+                                                              const SusIter* __restrict__ it,
+                                                              const double* __restrict__ part) {
+  const int s = blockIdx.x, j = blockIdx.y;
+  const SusDev& d = sus[s];
+  const SusIter& c = it[s];
+  if (c.w == nullptr || (phase < 2 && j >= c.K) || (phase == 2 && j > 0)) return;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < d.vb_count; b += 64) v += part[static_cast<int64_t>(j) * nvb + d.vb_first + b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (threadIdx.x == 0) c.hout[phase * (kdim + 1) + j] = v;
+}
+
+// w -= V[0..K) h, h = hout[phase (kdim + 1) ..]
+__global__ __launch_bounds__(kVecBlock) void orth_batch_kernel(int phase, int kdim, const SusDev* __restrict__ sus,
+                                                               const SusIter* __restrict__ it,
+                                                               const int2* __restrict__ vblocks) {
+  const int2 e = vblocks[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  const SusIter& c = it[e.x];
+  if (c.w == nullptr || c.K == 0) return;
+  const int r = e.y * kVecBlock + threadIdx.x;
+  if (r >= d.n) return;
+  const double* h = c.hout + phase * (kdim + 1);
+  double s = 0.0;
+  for (int k = 0; k < c.K; ++k) s = fma(d.V[static_cast<int64_t>(k) * d.n + r], h[k], s);
+  c.w[r] -= s;
+}
+
+hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it, const int2* vblocks,
+                                  hipStream_t s) {
+  if (nvb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(iter_prep_batch_kernel, dim3(nvb), dim3(kVecBlock), 0, s, sus, it, vblocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, const SusDev* sus,
+                             const SusIter* it, const int2* vblocks, double* part, hipStream_t s) {
+  if (nvb <= 0 || nsus <= 0) return hipSuccess;
+  const int jy = (phase == 2) ? 1 : kmax;
+  if (jy <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dots_batch_kernel, dim3(nvb, jy), dim3(kVecBlock), 0, s, phase, nvb, sus, it, vblocks, part);
+  hipLaunchKernelGGL(dots_final_batch_kernel, dim3(nsus, jy), dim3(64), 0, s, phase, nvb, kdim, sus, it, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_orth_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
+                             const int2* vblocks, hipStream_t s) {
+  if (nvb <= 0) return hipSuccess;
+  hipLaunchKernelGGL(orth_batch_kernel, dim3(nvb), dim3(kVecBlock), 0, s, phase, kdim, sus, it, vblocks);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------
+// FP16 storage of the theta form (GMRES-IR inner solves).  Every (target i,
+// source j) block -- the 27 x N numbers of a 3N x 3N block of A -- shares one
+// FP32 scale S[i][j] = max|T| / 32768, so the stored values use the FP16 range
+// whatever the block's distance (far blocks would otherwise sink into FP16's
+// subnormals).  Same component layout as T.
+
+template <int N>
+__global__ __launch_bounds__(256) void theta_to_fp16_kernel(int M, int64_t Lp, const float* __restrict__ T,
+                                                            _Float16* __restrict__ T16, float* __restrict__ S) {
+  constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int JS = 256 / N;  // sources per sweep (whole blocks only)
+  __shared__ float mx[256];
+  __shared__ float sc[JS];
+  const int i = blockIdx.x;
+  const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
+  _Float16* Hi = T16 + static_cast<int64_t>(i) * NC * Lp;
+  const int el = threadIdx.x;
+  for (int j0 = 0; j0 < M; j0 += JS) {
+    const int j = j0 + el / N;
+    const bool active = el < JS * N && j < M;
+    const int64_t e = static_cast<int64_t>(j) * N + el % N;
+    float v[NC];
+    float m = 0.0f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      v[c] = active ? Ti[c * Lp + e] : 0.0f;
+      m = fmaxf(m, fabsf(v[c]));
+    }
+    mx[el] = m;
+    __syncthreads();
+    if (el < JS && j0 + el < M) {
+      float bm = 0.0f;
+#pragma unroll
+      for (int k = 0; k < N; ++k) bm = fmaxf(bm, mx[el * N + k]);
+      const float sv = bm * (1.0f / 32768.0f);
+      sc[el] = sv;
+      S[static_cast<int64_t>(i) * M + j0 + el] = sv;
+    }
+    __syncthreads();
+    if (active) {
+      const float sv = sc[el / N];
+      const float inv = (sv > 0.0f) ? 1.0f / sv : 0.0f;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) Hi[c * Lp + e] = static_cast<_Float16>(v[c] * inv);
+    }
+    __syncthreads();
+  }
+}
+
+hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s) {
+  switch (N) {
+#define RF_TH16(NN)                                                                                 \
+  case NN:                                                                                          \
+    hipLaunchKernelGGL(theta_to_fp16_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T, T16, S);     \
+    break;
+    RF_TH16(1)
+    RF_TH16(2)
+    RF_TH16(3)
+    RF_TH16(4)
+    RF_TH16(5)
+    RF_TH16(6)
+    RF_TH16(7)
+    RF_TH16(8)
+#undef RF_TH16
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// y = A16 x (mode 0) or b - A16 x (mode 1): the theta GEMV on the FP16 copy.
+// Each lane takes two consecutive entries (one 4-byte load per component, a
+// wave reads 256 contiguous bytes) and folds the block scale into x.
+template <int N>
+__global__ __launch_bounds__(256) void gemv_theta16_kernel(int M, int64_t Lp, const _Float16* __restrict__ T16,
+                                                           const float* __restrict__ S,
+                                                           const double* __restrict__ ori,
+                                                           const double* __restrict__ x, double* __restrict__ y,
+                                                           const double* __restrict__ b, int mode, PhysConsts pc) {
+  typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int NACC = 1 + 3 * (N - 1);
+  __shared__ double red[256 / 64][NACC];
+  const int i = blockIdx.x;
+  const _Float16* Hi = T16 + static_cast<int64_t>(i) * NC * Lp;
+  const float* Si = S + static_cast<int64_t>(i) * M;
+  const int L = M * N;
+  double acc[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+  auto entry = [&](int e, double hv[NC]) {
+    const double sj = static_cast<double>(Si[e / N]);
+    const double x0 = sj * x[3 * static_cast<int64_t>(e)], x1 = sj * x[3 * static_cast<int64_t>(e) + 1],
+                 x2 = sj * x[3 * static_cast<int64_t>(e) + 2];
+    acc[0] = fma(hv[0], x0, fma(hv[1], x1, fma(hv[2], x2, acc[0])));
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+      const double* t = hv + 3 + 6 * (k - 1);  // xx yy zz xy xz yz
+      double* u = acc + 1 + 3 * (k - 1);
+      u[0] = fma(t[0], x0, fma(t[3], x1, fma(t[4], x2, u[0])));
+      u[1] = fma(t[3], x0, fma(t[1], x1, fma(t[5], x2, u[1])));
+      u[2] = fma(t[4], x0, fma(t[5], x1, fma(t[2], x2, u[2])));
+    }
+  };
+  const int Lpair = L & ~1;
+  for (int e = 2 * threadIdx.x; e < Lpair; e += 512) {
+    double h0[NC], h1[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const h2 v = __builtin_nontemporal_load(reinterpret_cast<const h2*>(Hi + c * Lp + e));
+      h0[c] = static_cast<double>(static_cast<float>(v.x));
+      h1[c] = static_cast<double>(static_cast<float>(v.y));
+    }
+    entry(e, h0);
+    entry(e + 1, h1);
+  }
+  if ((L & 1) && threadIdx.x == 0) {
+    double h0[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) h0[c] = static_cast<double>(static_cast<float>(Hi[c * Lp + L - 1]));
+    entry(L - 1, h0);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) {
+    double v = acc[a];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wave][a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3 * N) {
+    const int r = threadIdx.x, k = r / 3, a = r - 3 * k;
+    const double t[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
+    double out;
+    if (k == 0) {
+      const double s0 = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+      out = static_cast<double>(pc.D1) * t[a] * s0;
+    } else {
+      double U[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int idx = 1 + 3 * (k - 1) + c;
+        U[c] = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
+      }
+      const double tU = t[0] * U[0] + t[1] * U[1] + t[2] * U[2];
+      out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * t[a] * tU);
+    }
+    const int64_t row = static_cast<int64_t>(3 * N) * i + r;
+    out += x[row];  // identity diagonal block
+    y[row] = (mode == 1) ? b[row] - out : out;
+  }
+}
+
+hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
+                        const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,
+                        hipEvent_t ev_begin, hipEvent_t ev_end) {
+  if (ev_begin) (void)hipEventRecord(ev_begin, s);
+  switch (N) {
+#define RF_G16(NN)                                                                                            \
+  case NN:                                                                                                    \
+    hipLaunchKernelGGL(gemv_theta16_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T16, S, ori, x, y, b, mode, \
+                       pc);                                                                                   \
+    break;
+    RF_G16(1)
+    RF_G16(2)
+    RF_G16(3)
+    RF_G16(4)
+    RF_G16(5)
+    RF_G16(6)
+    RF_G16(7)
+    RF_G16(8)
+#undef RF_G16
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (ev_end) (void)hipEventRecord(ev_end, s);
   return hipGetLastError();
 }
 

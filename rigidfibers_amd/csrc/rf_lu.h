@@ -12,14 +12,21 @@ namespace rf {
 
 enum class LuMode { kFP32, kBF16, kFP16 };
 
+constexpr int kLuNb = 64;       // diagonal block: pivoting happens inside it
+constexpr int kLuPanel = 1024;  // outer panel: the K of the trailing MFMA update
+
 struct LuWorkspace {
   LuMode mode = LuMode::kFP32;
   int64_t n = 0, lda = 0;
-  int nb = 0;
   float* F = nullptr;      // factors (FP32, column-major, lda); the assembled A is kept intact
-  int* ipiv = nullptr;     // device pivots (global row index chosen at each column)
+  int* perm = nullptr;     // perm[k0 + c] = original row of factored row k0 + c (block-local)
   int* info = nullptr;     // device: first zero pivot (1-based) or 0
   double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
+  void* At = nullptr;      // packed L panel: n rows x kLuPanel, K-contiguous, mode's element type
+  void* Bt = nullptr;      // packed U panel: n columns x kLuPanel
+  double* Dinv = nullptr;  // per diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 FP64 each, [j][c]
+  static constexpr int kEvents = 256;  // event pairs around the trailing updates
+  hipEvent_t ev[2 * kEvents] = {};
   int64_t bytes = 0;
 };
 
@@ -28,11 +35,18 @@ struct LuResult {
   int refinement_iterations = 0;  // FP64-residual refinement sweeps
   double relative_residual = 0.0; // ||b - A x|| / ||b|| with the FP32 A, in FP64
   bool converged = false;
+  double factor_ms = 0.0;         // factorisation (device time)
+  double gemm_ms = 0.0;           // summed duration of the trailing-update GEMM launches
+  double gemm_flop = 0.0;         // their algorithmic flops (2 m n K each)
+  int gemm_launches = 0;
 };
 
-// C -= A B, column-major FP32 blocks, products on MFMA in the given mode.
-hipError_t gemm_sub(LuMode mode, int64_t m, int64_t n, int64_t k, const float* A, int64_t lda, const float* B,
-                    int64_t ldb, float* C, int64_t ldc, hipStream_t s);
+// C -= A B with K-contiguous packs At (m rows, row stride ldak elements) and Bt
+// (n rows, stride ldbk) of the mode's element type; K a multiple of
+// gemm_stage_k(mode), pack rows 16-byte aligned.  C FP32 column-major.
+int gemm_stage_k(LuMode mode);
+hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
+                           const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s);
 
 hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws);
 int64_t lu_workspace_bytes(const LuWorkspace& ws);

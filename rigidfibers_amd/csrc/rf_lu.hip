@@ -4,20 +4,33 @@
 // Reference: magma_sgesv_gpu (src/simulation.cu:403-418) and LAPACK sgesv
 // (fortran/solve_system.incl:3): LU with row pivoting, then L y = P b, U x = y.
 //
-// Factorisation of a copy F of A (A itself is kept for the residuals):
-//   for each panel of nb columns at k0:
-//     1. diag: one workgroup factors the nb x nb diagonal block in LDS with
-//        partial pivoting among its rows (the pivot search is restricted to
-//        the diagonal block, which keeps the panel off the critical path of a
-//        75 000-row reduction per column);
-//     2. swap: the block's row interchanges are applied to every other column;
-//     3. trsm: L21 = A21 U11^-1 (one lane per row) and U12 = L11^-1 A12 (one
-//        lane per column);
-//     4. gemm: A22 -= L21 U12 (the O(n^3) part).
+// Factorisation of a copy F of A (A itself is kept for the residuals), in
+// outer panels of kLuPanel columns made of diagonal blocks of kLuNb:
+//   for each block k0 of the panel:
+//     diag    one wavefront factors the 64 x 64 diagonal block in registers
+//             with partial pivoting among the block's rows, keeping the row
+//             order as a permutation (perm[k0 + c] = source row of row c);
+//     l21     L21 = A21 U11^-1 for every row below, written to F and packed
+//             K-contiguous into At (the GEMM's A operand, mode's type);
+//     u12     the block's permutation is gathered and U12 = L11^-1 A12 solved
+//             for the panel's remaining columns, written to F and packed into Bt;
+//     gemm    A22 -= L21 U12 over the panel's remaining columns (K = 64).
+//   then the panel's rows of U for the trailing columns (u12 + gemm per block)
+//   and one trailing update A22 -= L21 U12 with K = kLuPanel on MFMA.
+// The trailing updates are the O(n^3) part; their K = 1024 keeps the FP32 C
+// read-modify-write traffic at 8 n^3 / (3 K) bytes.
+//
+// Pivoting is restricted to the diagonal block (the system's identity-
+// dominated diagonal blocks make that pivot choice safe; the refinement below
+// verifies every solve).  The columns left of a block are not permuted: the
+// forward solve subtracts their contribution in the block's original row
+// order and then applies the block's permutation, which is the same L y = P b.
+//
 // Every solve is followed by FP64 iterative refinement with residuals of the
 // FP32 A (r = b - A x accumulated in FP64), so the returned x carries a
-// verified ||b - A x|| / ||b||; a matrix that would need pivoting outside the
-// diagonal block shows up as a refinement that does not converge.
+// verified ||b - A x|| / ||b||.  The trailing updates round L21 and U12 to the
+// mode's type (unit scale: the factors of this operator are O(1), far inside
+// the FP16 range; an overflow would surface as a non-finite residual).
 #include <cmath>
 #include <vector>
 
@@ -25,270 +38,406 @@
 
 namespace rf {
 
-constexpr int kLuNb = 64;      // inner panel (diagonal block in LDS)
-constexpr int kLuOuter = 256;  // outer panel: the K of the trailing MFMA update
+// 64 FP32 values -> 16-byte stores of the pack type.
+template <typename T>
+__device__ __forceinline__ void store_packed(const float* x, T* __restrict__ dst) {
+  constexpr int E = 16 / sizeof(T);
+#pragma unroll
+  for (int c = 0; c < kLuNb; c += E) {
+    union {
+      T v[E];
+      uint4 u;
+    } pk;
+#pragma unroll
+    for (int e = 0; e < E; ++e) pk.v[e] = static_cast<T>(x[c + e]);
+    *reinterpret_cast<uint4*>(dst + c) = pk.u;
+  }
+}
 
 // ---------------------------------------------------------------------------
-// 1. diagonal block: GEPP inside one workgroup.
-template <int NB>
-__global__ __launch_bounds__(256) void lu_diag_kernel(int64_t n, int64_t lda, float* __restrict__ F, int64_t k0,
-                                                      int kb, int* __restrict__ ipiv, int* __restrict__ info) {
-  __shared__ float blk[NB][NB + 1];  // [col][row]
-  __shared__ int piv_row;
-  __shared__ float piv_abs[256];
-  __shared__ int piv_idx[256];
-  const int tid = threadIdx.x;
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int c = e / kb, r = e - c * kb;
-    blk[c][r] = F[(k0 + c) * lda + k0 + r];
-  }
-  __syncthreads();
-  for (int c = 0; c < kb; ++c) {
-    // pivot: argmax |blk[c][r]|, r >= c
-    float best = -1.0f;
-    int bi = c;
-    for (int r = c + tid; r < kb; r += 256) {
-      const float v = fabsf(blk[c][r]);
-      if (v > best) {
-        best = v;
-        bi = r;
+// diag: GEPP of the kb x kb block at (k0, k0) by one wavefront, lane = row.
+struct DiagLane {
+  float row[kLuNb];  // the lane's row of the block
+  bool used;         // chosen as a pivot row already
+  int pos;           // its row in the factored block
+  int zero_col;      // first column with a zero pivot (1-based), 0 if none
+};
+
+// Elimination step for column C (compile-time, so the row stays in registers).
+template <int C>
+__device__ __forceinline__ void diag_step(DiagLane& d, int kb, int lane) {
+  if constexpr (C < kLuNb) {
+    if (C >= kb) return;
+    // pivot: largest |row[C]| among the rows not yet chosen, ties to the lower
+    // row; NaN ranks highest so the order is total and every row gets a place
+    const float a = d.row[C];
+    float v = d.used ? -1.0f : (a != a ? INFINITY : fabsf(a));
+    int idx = lane;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const float ov = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(idx, off, 64);
+      if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
       }
     }
-    piv_abs[tid] = best;
-    piv_idx[tid] = bi;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (tid < off) {
-        const float o = piv_abs[tid + off];
-        const int oi = piv_idx[tid + off];
-        if (o > piv_abs[tid] || (o == piv_abs[tid] && oi < piv_idx[tid])) {
-          piv_abs[tid] = o;
-          piv_idx[tid] = oi;
-        }
-      }
-      __syncthreads();
+    // every lane holds the same idx; the pivot row is read lane-to-scalar
+    const int pr = __builtin_amdgcn_readfirstlane(idx);
+    if (lane == pr) {
+      d.used = true;
+      d.pos = C;
     }
-    if (tid == 0) {
-      piv_row = piv_idx[0];
-      ipiv[k0 + c] = static_cast<int>(k0) + piv_idx[0];
-      if (piv_abs[0] == 0.0f && info[0] == 0) info[0] = static_cast<int>(k0) + c + 1;
+    const float pv = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, d.row[C]), pr));
+    if (pv == 0.0f && d.zero_col == 0) d.zero_col = C + 1;
+    const float inv = (pv != 0.0f) ? 1.0f / pv : 0.0f;
+    const float l = d.row[C] * inv;
+    if (!d.used) d.row[C] = l;
+#pragma unroll
+    for (int cc = C + 1; cc < kLuNb; ++cc) {
+      const float p = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, d.row[cc]), pr));
+      if (!d.used) d.row[cc] = fmaf(-l, p, d.row[cc]);
     }
-    __syncthreads();
-    const int pr = piv_row;
-    if (pr != c)
-      for (int cc = tid; cc < kb; cc += 256) {
-        const float t = blk[cc][c];
-        blk[cc][c] = blk[cc][pr];
-        blk[cc][pr] = t;
-      }
-    __syncthreads();
-    const float d = blk[c][c];
-    const float inv = (d != 0.0f) ? 1.0f / d : 0.0f;
-    for (int r = c + 1 + tid; r < kb; r += 256) blk[c][r] *= inv;
-    __syncthreads();
-    const int rem = kb - c - 1;
-    for (int e = tid; e < rem * rem; e += 256) {
-      const int cc = c + 1 + e / rem, r = c + 1 + (e - (e / rem) * rem);
-      blk[cc][r] = fmaf(-blk[c][r], blk[cc][c], blk[cc][r]);
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < kb * kb; e += 256) {
-    const int c = e / kb, r = e - c * kb;
-    F[(k0 + c) * lda + k0 + r] = blk[c][r];
+    diag_step<C + 1>(d, kb, lane);
   }
 }
 
-// 2. row interchanges of block k0 applied to the columns outside [k0, k0+kb).
-__global__ void lu_swap_kernel(int64_t n, int64_t lda, float* __restrict__ F, int64_t k0, int kb,
-                               const int* __restrict__ ipiv) {
-  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= n || (c >= k0 && c < k0 + kb)) return;
-  float* col = F + c * lda;
-  for (int k = 0; k < kb; ++k) {
-    const int64_t p = ipiv[k0 + k];
-    if (p != k0 + k) {
-      const float t = col[k0 + k];
-      col[k0 + k] = col[p];
-      col[p] = t;
-    }
+__global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restrict__ F, int64_t k0, int kb,
+                                                     int* __restrict__ perm, int* __restrict__ info) {
+  const int lane = threadIdx.x;
+  DiagLane d;
+#pragma unroll
+  for (int c = 0; c < kLuNb; ++c) d.row[c] = (c < kb && lane < kb) ? F[(k0 + c) * lda + k0 + lane] : 0.0f;
+  d.used = lane >= kb;  // rows past the block never pivot
+  d.pos = lane;
+  d.zero_col = 0;
+  diag_step<0>(d, kb, lane);
+  const int pos = d.pos, zero_col = d.zero_col;
+  const float* row = d.row;
+  if (lane < kb) {
+#pragma unroll
+    for (int c = 0; c < kLuNb; ++c)
+      if (c < kb) F[(k0 + c) * lda + k0 + pos] = row[c];
+    perm[k0 + pos] = static_cast<int>(k0) + lane;
   }
+  if (lane == 0 && zero_col != 0 && info[0] == 0) info[0] = static_cast<int>(k0) + zero_col;
 }
 
-// 3a. L21 = A21 U11^-1: lane per row r in [r0, r1), forward over the block's columns.
-template <int NB>
-__global__ __launch_bounds__(256) void lu_trsm_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
-                                                          int64_t k0, int kb) {
-  __shared__ float U[NB][NB + 1];  // [col][row]
-  for (int e = threadIdx.x; e < kb * kb; e += 256) {
-    const int c = e / kb, r = e - c * kb;
-    U[c][r] = F[(k0 + c) * lda + k0 + r];
+// l21: rows [r0, r1), lane = row: x U11 = a, written to F and packed into
+// At[(r - K0) * kLuPanel + (k0 - K0) + c].
+template <typename T>
+__global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
+                                                     int64_t k0, int64_t K0, T* __restrict__ At) {
+  __shared__ float Ut[kLuNb][kLuNb];  // Ut[c][j] = U[j][c]
+  __shared__ float rd[kLuNb];
+  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += 256) {
+    const int c = e >> 6, j = e & 63;
+    Ut[c][j] = F[(k0 + c) * lda + k0 + j];
+  }
+  if (threadIdx.x < kLuNb) {
+    const float d = F[(k0 + threadIdx.x) * lda + k0 + threadIdx.x];
+    rd[threadIdx.x] = (d != 0.0f) ? 1.0f / d : 0.0f;
   }
   __syncthreads();
   const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (r >= r1) return;
-  float x[NB];
+  float x[kLuNb];
 #pragma unroll
-  for (int c = 0; c < NB; ++c) x[c] = (c < kb) ? F[(k0 + c) * lda + r] : 0.0f;
+  for (int c = 0; c < kLuNb; ++c) x[c] = F[(k0 + c) * lda + r];
 #pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    if (c < kb) {
-      float v = x[c];
+  for (int c = 0; c < kLuNb; ++c) {
+    float v = x[c];
 #pragma unroll
-      for (int j = 0; j < NB; ++j)
-        if (j < c) v = fmaf(-x[j], U[c][j], v);
-      x[c] = v / U[c][c];
-    }
+    for (int j = 0; j < c; ++j) v = fmaf(-x[j], Ut[c][j], v);
+    x[c] = v * rd[c];
   }
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
-    if (c < kb) F[(k0 + c) * lda + r] = x[c];
+  for (int c = 0; c < kLuNb; ++c) F[(k0 + c) * lda + r] = x[c];
+  store_packed<T>(x, At + (r - K0) * kLuPanel + (k0 - K0));
 }
 
-// 3b. U12 = L11^-1 A12: lane per column c in [c0, c1) (unit lower L11).
-template <int NB>
-__global__ __launch_bounds__(256) void lu_trsm_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
-                                                          int64_t k0, int kb) {
-  __shared__ float L[NB][NB + 1];  // [col][row]
-  for (int e = threadIdx.x; e < kb * kb; e += 256) {
-    const int c = e / kb, r = e - c * kb;
-    L[c][r] = F[(k0 + c) * lda + k0 + r];
+// u12: columns [c0, c1), 64 per workgroup (lane = column for the solve): rows
+// k0.. of the block gathered by its permutation, L11 y = a (unit lower),
+// written to F and packed into Bt[(c - K0) * kLuPanel + (k0 - K0) + r].
+template <typename T>
+__global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
+                                                    int64_t k0, int64_t K0, const int* __restrict__ perm,
+                                                    T* __restrict__ Bt) {
+  __shared__ float S[kLuNb][kLuNb + 1];   // [column][row]
+  __shared__ float Lr[kLuNb][kLuNb + 1];  // Lr[r][j] = L[r][j]
+  const int lane = threadIdx.x;
+  const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * kLuNb;
+#pragma unroll 8
+  for (int j = 0; j < kLuNb; ++j) Lr[lane][j] = F[(k0 + j) * lda + k0 + lane];
+  const int src = min(max(perm[k0 + lane] - static_cast<int>(k0), 0), kLuNb - 1);
+#pragma unroll 8
+  for (int cc = 0; cc < kLuNb; ++cc) {
+    const int64_t col = cb + cc;
+    S[cc][lane] = (col < c1) ? F[col * lda + k0 + src] : 0.0f;
   }
   __syncthreads();
-  const int64_t c = c0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (c >= c1) return;
-  float* col = F + c * lda + k0;
-  float x[NB];
+  float x[kLuNb];
 #pragma unroll
-  for (int r = 0; r < NB; ++r) x[r] = (r < kb) ? col[r] : 0.0f;
+  for (int r = 0; r < kLuNb; ++r) x[r] = S[lane][r];
 #pragma unroll
-  for (int r = 0; r < NB; ++r) {
-    if (r < kb) {
-      float v = x[r];
+  for (int r = 1; r < kLuNb; ++r) {
+    float v = x[r];
 #pragma unroll
-      for (int j = 0; j < NB; ++j)
-        if (j < r) v = fmaf(-L[j][r], x[j], v);
-      x[r] = v;
-    }
+    for (int j = 0; j < r; ++j) v = fmaf(-Lr[r][j], x[j], v);
+    x[r] = v;
   }
+  const int64_t mycol = cb + lane;
+  if (mycol < c1) store_packed<T>(x, Bt + (mycol - K0) * kLuPanel + (k0 - K0));
 #pragma unroll
-  for (int r = 0; r < NB; ++r)
-    if (r < kb) col[r] = x[r];
+  for (int r = 0; r < kLuNb; ++r) S[lane][r] = x[r];
+  __syncthreads();
+#pragma unroll 8
+  for (int cc = 0; cc < kLuNb; ++cc) {
+    const int64_t col = cb + cc;
+    if (col < c1) F[col * lda + k0 + lane] = S[cc][lane];
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Solves with the factors (FP64 vectors, FP32 factors).
+// Solves with the factors (FP64 vectors, FP32 factors), by panels of
+// kLuPanel rows: one workgroup solves the panel's diagonal part block by
+// block (its values in LDS), a row-parallel GEMV applies the panel's columns
+// to the rows outside it.
 
-__global__ void lu_apply_pivots_kernel(int64_t n, const int* __restrict__ ipiv, double* __restrict__ y) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int64_t k = 0; k < n; ++k) {
-    const int64_t p = ipiv[k];
-    if (p != k) {
-      const double t = y[k];
-      y[k] = y[p];
-      y[p] = t;
-    }
+// Inverses of the diagonal blocks, once per factorisation (every solve and
+// refinement sweep reuses them): block b at k0 = 64 b, lane = column c of the
+// inverse, FP64 substitution on the FP32 factors.  Stored transposed,
+// Dinv[b][0][c][r] = (L_bb^-1)[r][c] and Dinv[b][1][c][r] = (U_bb^-1)[r][c],
+// so that lane r of a solve reads row r of the inverse coalesced.
+__global__ __launch_bounds__(64) void lu_block_inverse_kernel(int64_t n, int64_t lda, const float* __restrict__ F,
+                                                              double* __restrict__ Dinv) {
+  __shared__ float blk[kLuNb][kLuNb + 1];  // [col][row]
+  const int lane = threadIdx.x;
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kLuNb;
+  const int kb = static_cast<int>(min(static_cast<int64_t>(kLuNb), n - k0));
+#pragma unroll 8
+  for (int c = 0; c < kLuNb; ++c)
+    blk[c][lane] = (c < kb && lane < kb) ? F[(k0 + c) * lda + k0 + lane] : (c == lane ? 1.0f : 0.0f);
+  __syncthreads();
+  double* Li = Dinv + static_cast<int64_t>(blockIdx.x) * 2 * kLuNb * kLuNb;
+  double* Ui = Li + kLuNb * kLuNb;
+  __shared__ double X[kLuNb][kLuNb];  // X[r][c]: lane c's column (one-off kernel: LDS, not registers)
+  // L X = I (unit lower): x_r = delta_rc - sum_{j<r} L[r][j] x_j
+  for (int r = 0; r < kLuNb; ++r) {
+    double v = (r == lane) ? 1.0 : 0.0;
+    for (int j = 0; j < r; ++j) v = fma(-static_cast<double>(blk[j][r]), X[j][lane], v);
+    X[r][lane] = v;
   }
+  for (int r = 0; r < kLuNb; ++r) Li[lane * kLuNb + r] = X[r][lane];
+  // U X = I: x_r = (delta_rc - sum_{j>r} U[r][j] x_j) / U[r][r]
+  for (int r = kLuNb - 1; r >= 0; --r) {
+    double v = (r == lane) ? 1.0 : 0.0;
+    for (int j = r + 1; j < kLuNb; ++j) v = fma(-static_cast<double>(blk[j][r]), X[j][lane], v);
+    X[r][lane] = v / static_cast<double>(blk[r][r]);
+  }
+  for (int r = 0; r < kLuNb; ++r) Ui[lane * kLuNb + r] = X[r][lane];
 }
 
-// y[k0:k0+kb] <- T^-1 y[k0:k0+kb], T the unit-lower (lower = 1) or upper diagonal block.
-__global__ __launch_bounds__(64) void lu_trsv_block_kernel(int64_t lda, const float* __restrict__ F, int64_t k0,
-                                                           int kb, int lower, double* __restrict__ y) {
-  __shared__ double ys[kLuNb];
-  const int t = threadIdx.x;
-  if (t < kb) ys[t] = y[k0 + t];
-  __syncthreads();
-  if (lower) {
-    for (int c = 0; c < kb; ++c) {
-      if (t > c && t < kb) ys[t] -= static_cast<double>(F[(k0 + c) * lda + k0 + t]) * ys[c];
-      __syncthreads();
+// sum_c F[(k0 + c) lda + row] ys[c] over one block's columns (p = F + k0 lda +
+// row); a full block issues its 64 loads together.
+__device__ __forceinline__ double row_block_dot(const float* __restrict__ p, int64_t lda, const double* ys, int kb) {
+  double s0 = 0.0, s1 = 0.0;
+  if (kb == kLuNb) {
+    float v[kLuNb];
+#pragma unroll
+    for (int c = 0; c < kLuNb; ++c) v[c] = p[c * lda];
+#pragma unroll
+    for (int c = 0; c < kLuNb; c += 2) {
+      s0 = fma(static_cast<double>(v[c]), ys[c], s0);
+      s1 = fma(static_cast<double>(v[c + 1]), ys[c + 1], s1);
     }
   } else {
-    for (int c = kb - 1; c >= 0; --c) {
-      if (t == c) ys[c] /= static_cast<double>(F[(k0 + c) * lda + k0 + c]);
-      __syncthreads();
-      if (t < c) ys[t] -= static_cast<double>(F[(k0 + c) * lda + k0 + t]) * ys[c];
-      __syncthreads();
-    }
+    for (int c = 0; c < kb; ++c) s0 = fma(static_cast<double>(p[c * lda]), ys[c], s0);
   }
-  if (t < kb) y[k0 + t] = ys[t];
+  return s0 + s1;
 }
 
-// y[rlo:rhi] -= F[rlo:rhi, k0:k0+kb] y[k0:k0+kb]
-__global__ void lu_panel_gemv_kernel(int64_t lda, const float* __restrict__ F, int64_t k0, int kb, int64_t rlo,
-                                     int64_t rhi, double* __restrict__ y) {
-  const int64_t r = rlo + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= rhi) return;
-  double s = 0.0;
-  for (int c = 0; c < kb; ++c) s = fma(static_cast<double>(F[(k0 + c) * lda + r]), y[k0 + c], s);
-  y[r] -= s;
+// out_r = sum_c Minv[c][r] t[c] for the lane's row r (Minv transposed, 64 x 64).
+__device__ __forceinline__ double block_inverse_row(const double* __restrict__ Minv, const double* t, int r, int kb) {
+  double s0 = 0.0, s1 = 0.0;
+  if (kb == kLuNb) {
+    double v[kLuNb];
+#pragma unroll
+    for (int c = 0; c < kLuNb; ++c) v[c] = Minv[c * kLuNb + r];
+#pragma unroll
+    for (int c = 0; c < kLuNb; c += 2) {
+      s0 = fma(v[c], t[c], s0);
+      s1 = fma(v[c + 1], t[c + 1], s1);
+    }
+  } else {
+    for (int c = 0; c < kb; ++c) s0 = fma(Minv[c * kLuNb + r], t[c], s0);
+  }
+  return s0 + s1;
 }
 
-// Two-level right-looking factorisation: inner panels of kLuNb columns are
-// factored inside an outer panel of kLuOuter columns (their updates touch only
-// the outer panel's columns); the outer panel's rows of U are then solved for
-// and the trailing matrix receives one rank-kLuOuter update on MFMA.
-static hipError_t lu_factor(LuWorkspace& ws, LuMode mode, hipStream_t s) {
-  const int64_t n = ws.n, lda = ws.lda;
+constexpr int kSolveThreads = 1024;  // >= the rows of a panel outside one block
+
+// Forward: y[K0:KE] <- L_panel^-1 P y[K0:KE] (block permutations applied
+// after the contributions of the columns left of each block).
+__global__ __launch_bounds__(kSolveThreads) void lu_fwd_panel_kernel(int64_t lda, const float* __restrict__ F,
+                                                                     const int* __restrict__ perm,
+                                                                     const double* __restrict__ Dinv, int64_t K0,
+                                                                     int64_t KE, double* __restrict__ y) {
+  __shared__ double ys[kLuPanel];
+  __shared__ double tb[kLuNb];
+  const int tid = threadIdx.x;
+  const int len = static_cast<int>(KE - K0);
+  for (int r = tid; r < len; r += kSolveThreads) ys[r] = y[K0 + r];
+  __syncthreads();
+  for (int b0 = 0; b0 < len; b0 += kLuNb) {
+    const int kb = min(kLuNb, len - b0);
+    const int64_t k0 = K0 + b0;
+    if (tid < kb) {  // the block's values in factored order
+      const int src = min(max(perm[k0 + tid] - static_cast<int>(k0), 0), kb - 1);
+      tb[tid] = ys[b0 + src];
+    }
+    __syncthreads();
+    if (tid < kb) ys[b0 + tid] = block_inverse_row(Dinv + (k0 / kLuNb) * 2 * kLuNb * kLuNb, tb, tid, kb);
+    __syncthreads();
+    const int r = b0 + kb + tid;
+    if (r < len) ys[r] -= row_block_dot(F + k0 * lda + K0 + r, lda, ys + b0, kb);
+    __syncthreads();
+  }
+  for (int r = tid; r < len; r += kSolveThreads) y[K0 + r] = ys[r];
+}
+
+// Backward: y[K0:KE] <- U_panel^-1 y[K0:KE].
+__global__ __launch_bounds__(kSolveThreads) void lu_bwd_panel_kernel(int64_t lda, const float* __restrict__ F,
+                                                                     const double* __restrict__ Dinv, int64_t K0,
+                                                                     int64_t KE, double* __restrict__ y) {
+  __shared__ double ys[kLuPanel];
+  __shared__ double tb[kLuNb];
+  const int tid = threadIdx.x;
+  const int len = static_cast<int>(KE - K0);
+  for (int r = tid; r < len; r += kSolveThreads) ys[r] = y[K0 + r];
+  __syncthreads();
+  for (int b0 = ((len - 1) / kLuNb) * kLuNb; b0 >= 0; b0 -= kLuNb) {
+    const int kb = min(kLuNb, len - b0);
+    const int64_t k0 = K0 + b0;
+    if (tid < kb) tb[tid] = ys[b0 + tid];
+    __syncthreads();
+    if (tid < kb)
+      ys[b0 + tid] = block_inverse_row(Dinv + (k0 / kLuNb) * 2 * kLuNb * kLuNb + kLuNb * kLuNb, tb, tid, kb);
+    __syncthreads();
+    if (tid < b0) ys[tid] -= row_block_dot(F + k0 * lda + K0 + tid, lda, ys + b0, kb);
+    __syncthreads();
+  }
+  for (int r = tid; r < len; r += kSolveThreads) y[K0 + r] = ys[r];
+}
+
+// y[r0:r1] -= F[r0:r1, c0:c0+w] y[c0:c0+w]  (ranges disjoint), lane = row.
+__global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const float* __restrict__ F, int64_t c0,
+                                                            int w, int64_t r0, int64_t r1, double* __restrict__ y) {
+  __shared__ double xs[kLuPanel];
+  for (int c = threadIdx.x; c < w; c += 256) xs[c] = y[c0 + c];
+  __syncthreads();
+  const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (r >= r1) return;
+  const float* p = F + c0 * lda + r;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int c = 0;
+  for (; c + 16 <= w; c += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = p[(c + u) * lda];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) s[u & 3] = fma(static_cast<double>(v[u]), xs[c + u], s[u & 3]);
+  }
+  for (; c < w; ++c) s[0] = fma(static_cast<double>(p[c * lda]), xs[c], s[0]);
+  y[r] -= (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
+
+template <typename T>
+static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
+  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
   float* F = ws.F;
+  T* At = static_cast<T*>(ws.At);
+  T* Bt = static_cast<T*>(ws.Bt);
   auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
-  auto blocks = [](int64_t len) { return static_cast<unsigned>((len + 255) / 256); };
   hipError_t e = hipSuccess;
-  for (int64_t K0 = 0; K0 < n; K0 += kLuOuter) {
-    const int64_t KE = std::min<int64_t>(n, K0 + kLuOuter);
+  int nev = 0;
+  for (int64_t K0 = 0; K0 < n; K0 += W) {
+    const int64_t KE = std::min<int64_t>(n, K0 + W);
     for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
       const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
       const int64_t k1 = k0 + kb;
-      hipLaunchKernelGGL((lu_diag_kernel<kLuNb>), dim3(1), dim3(256), 0, s, n, lda, F, k0, kb, ws.ipiv, ws.info);
-      hipLaunchKernelGGL(lu_swap_kernel, dim3(blocks(n)), dim3(256), 0, s, n, lda, F, k0, kb, ws.ipiv);
-      if (k1 < n)
-        hipLaunchKernelGGL((lu_trsm_l21_kernel<kLuNb>), dim3(blocks(n - k1)), dim3(256), 0, s, k1, n, lda, F, k0, kb);
+      hipLaunchKernelGGL(lu_diag_kernel, dim3(1), dim3(64), 0, s, lda, F, k0, kb, ws.perm, ws.info);
+      if (k1 < n) {  // a full block (only the last block of the matrix is short)
+        hipLaunchKernelGGL((lu_l21_kernel<T>), dim3(grid_of(n - k1, 256)), dim3(256), 0, s, k1, n, lda, F, k0, K0,
+                           At);
+      }
       if (k1 < KE) {
-        hipLaunchKernelGGL((lu_trsm_u12_kernel<kLuNb>), dim3(blocks(KE - k1)), dim3(256), 0, s, k1, KE, lda, F, k0,
-                           kb);
-        if ((e = gemm_sub(mode, n - k1, KE - k1, kb, at(k1, k0), lda, at(k0, k1), lda, at(k1, k1), lda, s)) !=
-            hipSuccess)
+        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, s, k1, KE, lda, F, k0,
+                           K0, ws.perm, Bt);
+        if ((e = gemm_packed_sub(mode, n - k1, KE - k1, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
+                                 Bt + (k1 - K0) * W + (k0 - K0), W, at(k1, k1), lda, s)) != hipSuccess)
           return e;
       }
     }
     if (KE < n) {
-      // rows K0..KE of U for the trailing columns: U12 = L11^-1 A12, blockwise
+      // the panel's rows of U for the trailing columns
       for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
-        const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
-        const int64_t k1 = k0 + kb;
-        hipLaunchKernelGGL((lu_trsm_u12_kernel<kLuNb>), dim3(blocks(n - KE)), dim3(256), 0, s, KE, n, lda, F, k0, kb);
-        if (k1 < KE &&
-            (e = gemm_sub(mode, KE - k1, n - KE, kb, at(k1, k0), lda, at(k0, KE), lda, at(k1, KE), lda, s)) !=
-                hipSuccess)
+        const int64_t k1 = k0 + kLuNb;
+        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(n - KE, kLuNb)), dim3(64), 0, s, KE, n, lda, F, k0, K0,
+                           ws.perm, Bt);
+        if (k1 < KE && (e = gemm_packed_sub(mode, KE - k1, n - KE, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
+                                            Bt + (KE - K0) * W + (k0 - K0), W, at(k1, KE), lda, s)) != hipSuccess)
           return e;
       }
-      if ((e = gemm_sub(mode, n - KE, n - KE, KE - K0, at(KE, K0), lda, at(K0, KE), lda, at(KE, KE), lda, s)) !=
-          hipSuccess)
+      // trailing update, K = W
+      const bool timed = nev < LuWorkspace::kEvents;
+      if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
+      if ((e = gemm_packed_sub(mode, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
+                               at(KE, KE), lda, s)) != hipSuccess)
         return e;
+      if (timed) {
+        (void)hipEventRecord(ws.ev[2 * nev + 1], s);
+        ++nev;
+        res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(n - KE) * static_cast<double>(KE - K0);
+      }
     }
   }
+  res->gemm_launches = nev;
+  hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(n, kLuNb)), dim3(64), 0, s, n, lda, F, ws.Dinv);
   return hipGetLastError();
+}
+
+static hipError_t lu_factor(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
+  switch (mode) {
+    case LuMode::kFP32:
+      return lu_factor_t<float>(ws, mode, s, res);
+    case LuMode::kBF16:
+      return lu_factor_t<__bf16>(ws, mode, s, res);
+    case LuMode::kFP16:
+      return lu_factor_t<_Float16>(ws, mode, s, res);
+  }
+  return hipErrorInvalidValue;
 }
 
 // y <- (LU)^-1 P y
 static hipError_t lu_apply_inverse(LuWorkspace& ws, double* y, hipStream_t s) {
-  const int64_t n = ws.n, lda = ws.lda;
-  hipLaunchKernelGGL(lu_apply_pivots_kernel, dim3(1), dim3(1), 0, s, n, ws.ipiv, y);
-  for (int64_t k0 = 0; k0 < n; k0 += kLuNb) {
-    const int kb = static_cast<int>(std::min<int64_t>(kLuNb, n - k0));
-    hipLaunchKernelGGL(lu_trsv_block_kernel, dim3(1), dim3(64), 0, s, lda, ws.F, k0, kb, 1, y);
-    const int64_t rlo = k0 + kb;
-    if (rlo < n)
-      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(static_cast<unsigned>((n - rlo + 255) / 256)), dim3(256), 0, s,
-                         lda, ws.F, k0, kb, rlo, n, y);
+  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
+  for (int64_t K0 = 0; K0 < n; K0 += W) {
+    const int64_t KE = std::min<int64_t>(n, K0 + W);
+    hipLaunchKernelGGL(lu_fwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.perm, ws.Dinv, K0, KE,
+                       y);
+    if (KE < n)
+      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(grid_of(n - KE, 256)), dim3(256), 0, s, lda, ws.F, K0,
+                         static_cast<int>(KE - K0), KE, n, y);
   }
-  const int64_t last = ((n - 1) / kLuNb) * kLuNb;
-  for (int64_t k0 = last; k0 >= 0; k0 -= kLuNb) {
-    const int kb = static_cast<int>(std::min<int64_t>(kLuNb, n - k0));
-    hipLaunchKernelGGL(lu_trsv_block_kernel, dim3(1), dim3(64), 0, s, lda, ws.F, k0, kb, 0, y);
-    if (k0 > 0)
-      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(static_cast<unsigned>((k0 + 255) / 256)), dim3(256), 0, s, lda,
-                         ws.F, k0, kb, int64_t(0), k0, y);
+  for (int64_t K0 = ((n - 1) / W) * W; K0 >= 0; K0 -= W) {
+    const int64_t KE = std::min<int64_t>(n, K0 + W);
+    hipLaunchKernelGGL(lu_bwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.Dinv, K0, KE, y);
+    if (K0 > 0)
+      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(grid_of(K0, 256)), dim3(256), 0, s, lda, ws.F, K0,
+                         static_cast<int>(KE - K0), int64_t(0), K0, y);
   }
   return hipGetLastError();
 }
@@ -297,14 +446,21 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   ws->mode = mode;
   ws->n = n;
   ws->lda = lda;
-  ws->nb = kLuNb;
+  const size_t esz = mode == LuMode::kFP32 ? 4 : 2;
+  const size_t pack = esz * static_cast<size_t>(n) * kLuPanel;
   hipError_t e;
   if ((e = hipMalloc(&ws->F, sizeof(float) * static_cast<size_t>(lda) * n)) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->ipiv, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->perm, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->info, sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->At, pack)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Bt, pack)) != hipSuccess) return e;
+  const size_t dinv = sizeof(double) * 2 * kLuNb * kLuNb * static_cast<size_t>((n + kLuNb - 1) / kLuNb);
+  if ((e = hipMalloc(&ws->Dinv, dinv)) != hipSuccess) return e;
+  for (auto& ev : ws->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n));
+                                   sizeof(double) * 3 * static_cast<size_t>(n) + 2 * pack + dinv);
   return hipSuccess;
 }
 
@@ -312,9 +468,14 @@ int64_t lu_workspace_bytes(const LuWorkspace& ws) { return ws.bytes; }
 
 void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->F) (void)hipFree(ws->F);
-  if (ws->ipiv) (void)hipFree(ws->ipiv);
+  if (ws->perm) (void)hipFree(ws->perm);
   if (ws->info) (void)hipFree(ws->info);
   if (ws->vec) (void)hipFree(ws->vec);
+  if (ws->At) (void)hipFree(ws->At);
+  if (ws->Bt) (void)hipFree(ws->Bt);
+  if (ws->Dinv) (void)hipFree(ws->Dinv);
+  for (auto& ev : ws->ev)
+    if (ev) (void)hipEventDestroy(ev);
   *ws = LuWorkspace{};
 }
 
@@ -323,14 +484,19 @@ hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const float* A, int64_t lda, i
                     double tol, int maxit, hipStream_t s, LuResult* result) {
   hipError_t e;
   *result = LuResult{};
+  if (mode != ws.mode || n != ws.n || lda != ws.lda) return hipErrorInvalidValue;
   if ((e = hipMemcpy2DAsync(ws.F, lda * sizeof(float), A, lda * sizeof(float), n * sizeof(float), n,
                             hipMemcpyDeviceToDevice, s)) != hipSuccess)
     return e;
   if ((e = hipMemsetAsync(ws.info, 0, sizeof(int), s)) != hipSuccess) return e;
-  if ((e = lu_factor(ws, mode, s)) != hipSuccess) return e;
+  if ((e = lu_factor(ws, mode, s, result)) != hipSuccess) return e;
   int info = 0;
   if ((e = hipMemcpyAsync(&info, ws.info, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  for (int k = 0; k < result->gemm_launches; ++k) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ws.ev[2 * k], ws.ev[2 * k + 1]) == hipSuccess) result->gemm_ms += ms;
+  }
   result->info = info;
   if (info != 0) return hipSuccess;
   double* r = work;

@@ -1,0 +1,778 @@
+// rf_serve.hip -- serving engine of include/rigidfibers.h: many independent
+// suspensions (each the reference's Simulation, src/simulation.h:38-98, with
+// its own fiber count, state and step count) advanced together on one GPU.
+//
+// A worker thread owns the device work.  Requests wait in a FIFO queue and
+// are admitted while a slot is free and their device footprint fits the
+// memory budget; every admitted suspension keeps its state, theta-form
+// system, stored velocity operator, Krylov basis and previous solution
+// resident until it finishes.  One tick advances every admitted suspension by
+// one timestep (src/simulation.cu:286-329):
+//   assemble    one launch, workgroup = (suspension, target fiber);
+//   solve       lockstep GMRES(m): every iteration is one launch sequence over
+//               all suspensions (vector prep, theta matvec, two Gram-Schmidt
+//               passes, norm) and one device->host copy of every suspension's
+//               Hessenberg column; each suspension keeps its own Krylov state,
+//               restarts and tolerance, and leaves the lockstep when it converges;
+//   velocities  one launch over the stored velocity operators;
+//   update      one launch over all fibers.
+// Launches find their suspension through tables of (suspension, fiber) or
+// (suspension, 256-element block) entries rebuilt when the admitted set
+// changes.  Reductions are per suspension in a fixed order, so each result is
+// bitwise independent of what else is admitted.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/rigidfibers.h"
+#include "rf_host.h"
+#include "rf_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  int64_t id = 0;
+  int M = 0, steps = 0;
+  int64_t footprint = 0;
+  std::vector<double> pos0, ori0;      // initial state until admitted
+  std::vector<double> pos_out, ori_out;
+  std::vector<double> step_ms;
+  rf_request_info info{};
+  bool done = false;
+};
+
+enum Phase { kResidual = 0, kArnoldi = 1, kDone = 2 };
+
+struct Suspension {
+  Request* req = nullptr;
+  int M = 0, n = 0;
+  int64_t Lp = 0;
+  void* base = nullptr;  // one device allocation holding everything below
+  int64_t bytes = 0;
+  double* pos[3] = {};
+  double* ori[3] = {};
+  double* tv[2] = {};
+  double* rv[2] = {};
+  int cur = 0, prev = 1, next = 2, vcur = 0, vprev = 1;
+  bool have_history = false, have_solution = false;
+  float* T = nullptr;
+  float* Tv = nullptr;
+  double* cpart = nullptr;
+  double* b = nullptr;
+  double* x = nullptr;
+  double* V = nullptr;
+  int steps_done = 0;
+  // GMRES state of the current step
+  Phase phase = kDone;
+  int k = 0, it = 0;
+  double bnorm = 0.0, res = 0.0;
+  bool converged = false;
+  std::vector<double> H, cs, sn, g, y;
+  // operations for the next lockstep iteration
+  int zero_x = 0, ycount = 0;
+  double* svec = nullptr;
+  double sval = 1.0;
+  rf_status status = RF_OK;
+};
+
+int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
+
+template <typename T>
+bool grow_device(T** p, size_t* cap, size_t need) {
+  if (need <= *cap) return true;
+  size_t c = std::max<size_t>(need, 2 * *cap);
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc(p, c * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  *cap = c;
+  return true;
+}
+
+template <typename T>
+bool grow_host(T** p, size_t* cap, size_t need) {
+  if (need <= *cap) return true;
+  size_t c = std::max<size_t>(need, 2 * *cap);
+  if (*p) (void)hipHostFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipHostMalloc(reinterpret_cast<void**>(p), c * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  *cap = c;
+  return true;
+}
+
+}  // namespace
+
+struct rf_server {
+  rf_server_config cfg{};
+  rf::PhysConsts pc{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int N = 0, NQI = 0, m = 0, kdim = 0;
+  bool analytical = false;
+  Clock::time_point t0;
+
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request*> queue;
+  std::map<int64_t, Request*> requests;
+  int64_t next_id = 1;
+  bool stop = false;
+  std::thread worker;
+
+  std::vector<Suspension*> active;  // worker thread only
+  int64_t budget = 0, used = 0, peak = 0;
+  int64_t ticks = 0, fiber_steps = 0, completed = 0, n_active = 0;
+  double busy = 0.0;
+
+  // launch tables: device and pinned host mirrors
+  rf::SusDev* d_sus = nullptr;
+  rf::SusDev* h_sus = nullptr;
+  rf::SusIter* d_it = nullptr;
+  rf::SusIter* h_it = nullptr;
+  int2* d_targets = nullptr;
+  int2* h_targets = nullptr;
+  int2* d_vblocks = nullptr;
+  int2* h_vblocks = nullptr;
+  int2* d_fblocks = nullptr;
+  int2* h_fblocks = nullptr;
+  double* d_part = nullptr;
+  double* d_hout = nullptr;
+  double* h_hout = nullptr;
+  double* d_ycoef = nullptr;
+  double* h_ycoef = nullptr;
+  size_t c_dsus = 0, c_hsus = 0, c_dit = 0, c_hit = 0, c_dt = 0, c_ht = 0, c_dv = 0, c_hv = 0, c_df = 0, c_hf = 0,
+         c_part = 0, c_dh = 0, c_hh = 0, c_dy = 0, c_hy = 0;
+  int ntargets = 0, nvb = 0, nfb = 0;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+};
+
+namespace {
+
+int64_t footprint(const rf_server* S, int M, int64_t* Lp_out) {
+  const int64_t n = 3LL * S->N * M;
+  const int64_t Lp = rf::theta_stride(M, S->N);
+  const int64_t NC = 3 + 6 * (S->N - 1);
+  int64_t b = 0;
+  b += 10 * align256(3LL * M * 8);                 // pos[3], ori[3], tv[2], rv[2]
+  b += align256(static_cast<int64_t>(M) * NC * Lp * 4);  // theta system
+  b += align256(static_cast<int64_t>(M) * 12 * Lp * 4);  // velocity operator
+  b += align256(6LL * M * 8);                      // velocity constants
+  b += 2 * align256(n * 8);                        // b, x
+  b += align256(static_cast<int64_t>(S->kdim) * n * 8);  // Krylov basis
+  if (Lp_out) *Lp_out = Lp;
+  return b;
+}
+
+// Device allocation and upload of an admitted request (worker thread).
+rf_status admit(rf_server* S, Request* r, Suspension** out) {
+  auto* s = new (std::nothrow) Suspension();
+  if (!s) return RF_ERR_OUT_OF_MEMORY;
+  s->req = r;
+  s->M = r->M;
+  s->n = 3 * S->N * r->M;
+  s->bytes = footprint(S, r->M, &s->Lp);
+  if (hipMalloc(&s->base, s->bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return RF_ERR_OUT_OF_MEMORY;
+  }
+  char* p = static_cast<char*>(s->base);
+  auto take = [&](int64_t bytes) {
+    char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  const int64_t f3 = 3LL * s->M * 8;
+  for (int k = 0; k < 3; ++k) s->pos[k] = reinterpret_cast<double*>(take(f3));
+  for (int k = 0; k < 3; ++k) s->ori[k] = reinterpret_cast<double*>(take(f3));
+  for (int k = 0; k < 2; ++k) s->tv[k] = reinterpret_cast<double*>(take(f3));
+  for (int k = 0; k < 2; ++k) s->rv[k] = reinterpret_cast<double*>(take(f3));
+  const int64_t NC = 3 + 6 * (S->N - 1);
+  s->T = reinterpret_cast<float*>(take(static_cast<int64_t>(s->M) * NC * s->Lp * 4));
+  s->Tv = reinterpret_cast<float*>(take(static_cast<int64_t>(s->M) * 12 * s->Lp * 4));
+  s->cpart = reinterpret_cast<double*>(take(6LL * s->M * 8));
+  s->b = reinterpret_cast<double*>(take(static_cast<int64_t>(s->n) * 8));
+  s->x = reinterpret_cast<double*>(take(static_cast<int64_t>(s->n) * 8));
+  s->V = reinterpret_cast<double*>(take(static_cast<int64_t>(S->kdim) * s->n * 8));
+  if (hipMemcpyAsync(s->pos[0], r->pos0.data(), f3, hipMemcpyHostToDevice, S->stream) != hipSuccess ||
+      hipMemcpyAsync(s->ori[0], r->ori0.data(), f3, hipMemcpyHostToDevice, S->stream) != hipSuccess ||
+      hipStreamSynchronize(S->stream) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(s->base);
+    delete s;
+    return RF_ERR_DEVICE;
+  }
+  r->pos0.clear();
+  r->pos0.shrink_to_fit();
+  r->ori0.clear();
+  r->ori0.shrink_to_fit();
+  s->H.assign(static_cast<size_t>(S->m + 1) * S->m, 0.0);
+  s->cs.assign(S->m, 0.0);
+  s->sn.assign(S->m, 0.0);
+  s->g.assign(S->m + 1, 0.0);
+  s->y.assign(S->m, 0.0);
+  *out = s;
+  return RF_OK;
+}
+
+// Tables of the admitted set (rebuilt every tick: the state pointers rotate).
+bool build_tables(rf_server* S) {
+  const int ns = static_cast<int>(S->active.size());
+  int nt = 0, nv = 0, nf = 0;
+  for (auto* s : S->active) {
+    nt += s->M;
+    nv += (s->n + 255) / 256;
+    nf += (s->M + 255) / 256;
+  }
+  if (!grow_host(&S->h_sus, &S->c_hsus, ns) || !grow_device(&S->d_sus, &S->c_dsus, ns) ||
+      !grow_host(&S->h_it, &S->c_hit, ns) || !grow_device(&S->d_it, &S->c_dit, ns) ||
+      !grow_host(&S->h_targets, &S->c_ht, nt) || !grow_device(&S->d_targets, &S->c_dt, nt) ||
+      !grow_host(&S->h_vblocks, &S->c_hv, nv) || !grow_device(&S->d_vblocks, &S->c_dv, nv) ||
+      !grow_host(&S->h_fblocks, &S->c_hf, nf) || !grow_device(&S->d_fblocks, &S->c_df, nf) ||
+      !grow_device(&S->d_part, &S->c_part, static_cast<size_t>(nv) * S->kdim) ||
+      !grow_host(&S->h_hout, &S->c_hh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
+      !grow_device(&S->d_hout, &S->c_dh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
+      !grow_host(&S->h_ycoef, &S->c_hy, static_cast<size_t>(ns) * S->kdim) ||
+      !grow_device(&S->d_ycoef, &S->c_dy, static_cast<size_t>(ns) * S->kdim))
+    return false;
+  int ti = 0, vi = 0, fi = 0;
+  for (int q = 0; q < ns; ++q) {
+    Suspension* s = S->active[q];
+    rf::SusDev& d = S->h_sus[q];
+    d.M = s->M;
+    d.n = s->n;
+    d.Lp = s->Lp;
+    d.vb_first = vi;
+    d.vb_count = (s->n + 255) / 256;
+    d.pos = s->pos[s->cur];
+    d.ori = s->ori[s->cur];
+    d.T = s->T;
+    d.Tv = s->Tv;
+    d.cpart = s->cpart;
+    d.b = s->b;
+    d.x = s->x;
+    d.V = s->V;
+    d.tv = s->tv[s->vcur];
+    d.rv = s->rv[s->vcur];
+    d.pos_prev = s->pos[s->prev];
+    d.ori_prev = s->ori[s->prev];
+    d.tv_prev = s->tv[s->vprev];
+    d.rv_prev = s->rv[s->vprev];
+    d.pos_next = s->pos[s->next];
+    d.ori_next = s->ori[s->next];
+    d.first = s->have_history ? 0 : 1;
+    for (int i = 0; i < s->M; ++i) S->h_targets[ti++] = make_int2(q, i);
+    for (int b = 0; b < d.vb_count; ++b) S->h_vblocks[vi++] = make_int2(q, b);
+    for (int b = 0; b < (s->M + 255) / 256; ++b) S->h_fblocks[fi++] = make_int2(q, b);
+  }
+  S->ntargets = nt;
+  S->nvb = nv;
+  S->nfb = nf;
+  hipStream_t st = S->stream;
+  return hipMemcpyAsync(S->d_sus, S->h_sus, ns * sizeof(rf::SusDev), hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(S->d_targets, S->h_targets, nt * sizeof(int2), hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(S->d_vblocks, S->h_vblocks, nv * sizeof(int2), hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(S->d_fblocks, S->h_fblocks, nf * sizeof(int2), hipMemcpyHostToDevice, st) == hipSuccess;
+}
+
+// One lockstep iteration: upload the per-suspension descriptors, run the
+// launch sequence, bring every Hessenberg column back.
+bool lockstep_iteration(rf_server* S, int kmax) {
+  const int ns = static_cast<int>(S->active.size());
+  const int hs = 3 * (S->kdim + 1);
+  hipStream_t st = S->stream;
+  if (hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(S->d_ycoef, S->h_ycoef, static_cast<size_t>(ns) * S->kdim * sizeof(double),
+                     hipMemcpyHostToDevice, st) != hipSuccess)
+    return false;
+  if (rf::launch_iter_prep_batch(S->nvb, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
+  if (rf::launch_gemv_theta_batch(S->N, S->ntargets, S->d_sus, S->d_it, S->d_targets, S->pc, st) != hipSuccess)
+    return false;
+  for (int ph = 0; ph < 2; ++ph) {
+    if (kmax > 0) {
+      if (rf::launch_dots_batch(S->nvb, ns, kmax, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
+          hipSuccess)
+        return false;
+      if (rf::launch_orth_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
+    }
+  }
+  if (rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) != hipSuccess)
+    return false;
+  if (hipMemcpyAsync(S->h_hout, S->d_hout, static_cast<size_t>(ns) * hs * sizeof(double), hipMemcpyDeviceToHost,
+                     st) != hipSuccess)
+    return false;
+  return hipStreamSynchronize(st) == hipSuccess;
+}
+
+// GMRES(m) for every admitted suspension (the same algorithm as the
+// single-simulation solver: true residual at each restart, classical
+// Gram-Schmidt twice, Givens rotations on the host).
+bool solve_lockstep(rf_server* S) {
+  const int ns = static_cast<int>(S->active.size());
+  const int hs = 3 * (S->kdim + 1);
+  const int m = S->m;
+  const double tol = S->cfg.base.gmres_tolerance;
+  const int maxit = S->cfg.base.gmres_max_iterations;
+  const bool warm_ok = S->cfg.base.initial_guess == RF_GUESS_PREVIOUS;
+  // ||b|| of every suspension
+  for (int q = 0; q < ns; ++q) {
+    Suspension* s = S->active[q];
+    rf::SusIter& c = S->h_it[q];
+    c = rf::SusIter{};
+    c.w = s->b;
+    c.K = 0;
+    c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
+    c.ycoef = S->d_ycoef + static_cast<int64_t>(q) * S->kdim;
+  }
+  {
+    hipStream_t st = S->stream;
+    if (hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
+        rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(S->h_hout, S->d_hout, static_cast<size_t>(ns) * hs * sizeof(double), hipMemcpyDeviceToHost,
+                       st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      return false;
+  }
+  for (int q = 0; q < ns; ++q) {
+    Suspension* s = S->active[q];
+    s->bnorm = std::sqrt(S->h_hout[static_cast<int64_t>(q) * hs + 2 * (S->kdim + 1)]);
+    s->it = 0;
+    s->converged = false;
+    s->ycount = 0;
+    s->svec = nullptr;
+    s->zero_x = !(warm_ok && s->have_solution);
+    if (!std::isfinite(s->bnorm)) {
+      s->status = RF_ERR_NONFINITE;
+      s->phase = kDone;
+    } else if (s->bnorm == 0.0) {
+      s->zero_x = 1;
+      s->converged = true;
+      s->phase = kDone;
+    } else {
+      s->phase = kResidual;
+    }
+  }
+  while (true) {
+    bool work = false;
+    int kmax = 0;
+    for (int q = 0; q < ns; ++q) {
+      Suspension* s = S->active[q];
+      rf::SusIter& c = S->h_it[q];
+      c = rf::SusIter{};
+      c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
+      c.zero_x = s->zero_x;
+      c.ycount = s->ycount;
+      c.ycoef = S->d_ycoef + static_cast<int64_t>(q) * S->kdim;
+      for (int i = 0; i < s->ycount; ++i) S->h_ycoef[static_cast<int64_t>(q) * S->kdim + i] = s->y[i];
+      c.svec = s->svec;
+      c.sval = s->sval;
+      if (s->phase == kResidual) {
+        c.gx = s->x;
+        c.gy = s->V;
+        c.gmode = 1;
+        c.w = s->V;
+        c.K = 0;
+      } else if (s->phase == kArnoldi) {
+        c.gx = s->V + static_cast<int64_t>(s->k) * s->n;
+        c.gy = s->V + static_cast<int64_t>(s->k + 1) * s->n;
+        c.gmode = 0;
+        c.w = c.gy;
+        c.K = s->k + 1;
+        kmax = std::max(kmax, c.K);
+      }
+      if (s->phase != kDone || c.zero_x || c.ycount || c.svec) work = true;
+    }
+    if (!work) break;
+    if (!lockstep_iteration(S, kmax)) return false;
+    for (int q = 0; q < ns; ++q) {
+      Suspension* s = S->active[q];
+      const double* h = S->h_hout + static_cast<int64_t>(q) * hs;
+      s->zero_x = 0;
+      s->ycount = 0;
+      s->svec = nullptr;
+      if (s->phase == kResidual) {
+        const double beta = std::sqrt(h[2 * (S->kdim + 1)]);
+        s->res = beta;
+        if (!std::isfinite(beta)) {
+          s->status = RF_ERR_NONFINITE;
+          s->phase = kDone;
+        } else if (beta / s->bnorm <= tol) {
+          s->converged = true;
+          s->phase = kDone;
+        } else if (s->it >= maxit) {
+          s->phase = kDone;
+        } else {
+          s->svec = s->V;
+          s->sval = 1.0 / beta;
+          std::fill(s->g.begin(), s->g.end(), 0.0);
+          s->g[0] = beta;
+          s->k = 0;
+          s->phase = kArnoldi;
+        }
+      } else if (s->phase == kArnoldi) {
+        const int k = s->k;
+        ++s->it;
+        const double hn = std::sqrt(h[2 * (S->kdim + 1)]);
+        if (!std::isfinite(hn)) {
+          s->status = RF_ERR_NONFINITE;
+          s->phase = kDone;
+          continue;
+        }
+        double* Hk = s->H.data() + static_cast<size_t>(k) * (m + 1);
+        for (int i = 0; i <= k; ++i) Hk[i] = h[i] + h[(S->kdim + 1) + i];
+        Hk[k + 1] = hn;
+        for (int i = 0; i < k; ++i) {
+          const double a0 = Hk[i], a1 = Hk[i + 1];
+          Hk[i] = s->cs[i] * a0 + s->sn[i] * a1;
+          Hk[i + 1] = -s->sn[i] * a0 + s->cs[i] * a1;
+        }
+        const double a0 = Hk[k], a1 = Hk[k + 1];
+        const double rr = std::hypot(a0, a1);
+        s->cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
+        s->sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
+        Hk[k] = rr;
+        Hk[k + 1] = 0.0;
+        s->g[k + 1] = -s->sn[k] * s->g[k];
+        s->g[k] = s->cs[k] * s->g[k];
+        s->res = std::fabs(s->g[k + 1]);
+        if (s->res / s->bnorm <= tol || hn == 0.0 || k + 1 == m || s->it >= maxit) {
+          const int ku = k + 1;
+          for (int i = ku - 1; i >= 0; --i) {
+            double acc = s->g[i];
+            for (int j = i + 1; j < ku; ++j) acc -= s->H[i + static_cast<size_t>(j) * (m + 1)] * s->y[j];
+            s->y[i] = acc / s->H[i + static_cast<size_t>(i) * (m + 1)];
+          }
+          s->ycount = ku;
+          s->phase = kResidual;  // x += V y, then the true residual
+        } else {
+          s->svec = s->V + static_cast<int64_t>(k + 1) * s->n;
+          s->sval = 1.0 / hn;
+          s->k = k + 1;
+        }
+      }
+    }
+  }
+  for (auto* s : S->active) {
+    s->have_solution = true;
+    s->req->info.gmres_iterations += s->it;
+    if (!s->converged && s->status == RF_OK) s->status = RF_WARN_NOT_CONVERGED;
+  }
+  return true;
+}
+
+// One timestep of every admitted suspension.
+bool tick(rf_server* S) {
+  hipStream_t st = S->stream;
+  if (!build_tables(S)) return false;
+  if (rf::launch_assemble_theta_batch(S->N, S->NQI, S->analytical, S->ntargets, S->d_sus, S->d_targets, S->pc, st) !=
+      hipSuccess)
+    return false;
+  if (!solve_lockstep(S)) return false;
+  if (rf::launch_velocities_stored_batch(S->N, S->ntargets, S->d_sus, S->d_targets, S->pc, st) != hipSuccess)
+    return false;
+  if (rf::launch_update_batch(S->nfb, S->d_sus, S->d_fblocks, S->cfg.base.timestep, st) != hipSuccess) return false;
+  if (hipStreamSynchronize(st) != hipSuccess) return false;
+  for (auto* s : S->active) {  // rf_advance
+    const int old_prev = s->prev;
+    s->prev = s->cur;
+    s->cur = s->next;
+    s->next = old_prev;
+    std::swap(s->vcur, s->vprev);
+    s->have_history = true;
+    ++s->steps_done;
+  }
+  return true;
+}
+
+// Copies the final state, frees the suspension, publishes the result.
+void retire(rf_server* S, Suspension* s, rf_status status) {
+  Request* r = s->req;
+  const size_t f3 = 3 * static_cast<size_t>(s->M);
+  r->pos_out.assign(f3, 0.0);
+  r->ori_out.assign(f3, 0.0);
+  if (status == RF_OK || status == RF_WARN_NOT_CONVERGED) {
+    if (hipMemcpy(r->pos_out.data(), s->pos[s->cur], f3 * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(r->ori_out.data(), s->ori[s->cur], f3 * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+      (void)hipGetLastError();
+      status = RF_ERR_DEVICE;
+    }
+    for (size_t i = 0; i < f3 && status != RF_ERR_DEVICE; ++i)
+      if (!std::isfinite(r->pos_out[i]) || !std::isfinite(r->ori_out[i])) status = RF_ERR_NONFINITE;
+  }
+  (void)hipFree(s->base);
+  std::lock_guard<std::mutex> lk(S->mu);
+  S->used -= s->bytes;
+  --S->n_active;
+  r->info.status = status;
+  r->info.finished = S->now();
+  r->done = true;
+  ++S->completed;
+  delete s;
+  S->cv_done.notify_all();
+}
+
+void worker_main(rf_server* S) {
+  (void)hipSetDevice(S->device);
+  while (true) {
+    std::vector<Request*> admitted;
+    {
+      std::unique_lock<std::mutex> lk(S->mu);
+      if (S->active.empty())
+        S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+      if (S->stop) break;
+      while (!S->queue.empty() &&
+             static_cast<int>(S->active.size() + admitted.size()) < S->cfg.max_active &&
+             S->used + S->queue.front()->footprint <= S->budget) {
+        Request* r = S->queue.front();
+        S->queue.pop_front();
+        S->used += r->footprint;
+        S->peak = std::max(S->peak, S->used);
+        r->info.admitted = S->now();
+        ++S->n_active;
+        admitted.push_back(r);
+      }
+    }
+    for (Request* r : admitted) {
+      Suspension* s = nullptr;
+      const rf_status st = admit(S, r, &s);
+      if (st == RF_OK) {
+        S->active.push_back(s);
+      } else {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->used -= r->footprint;
+        --S->n_active;
+        r->info.status = st;
+        r->info.finished = S->now();
+        r->done = true;
+        ++S->completed;
+        S->cv_done.notify_all();
+      }
+    }
+    if (S->active.empty()) continue;
+    const auto t_start = Clock::now();
+    const bool ok = tick(S);
+    const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t_start).count();
+    std::vector<Suspension*> keep;
+    int64_t fsteps = 0;
+    for (auto* s : S->active) {
+      if (!ok) {
+        (void)hipGetLastError();
+        retire(S, s, RF_ERR_DEVICE);
+        continue;
+      }
+      s->req->step_ms.push_back(ms);
+      fsteps += s->M;
+      if (s->status != RF_OK && s->status != RF_WARN_NOT_CONVERGED) {
+        retire(S, s, s->status);
+      } else if (s->steps_done >= s->req->steps) {
+        retire(S, s, s->status);
+      } else {
+        keep.push_back(s);
+      }
+    }
+    S->active.swap(keep);
+    std::lock_guard<std::mutex> lk(S->mu);
+    ++S->ticks;
+    S->fiber_steps += fsteps;
+    S->busy += ms * 1e-3;
+  }
+  for (auto* s : S->active) retire(S, s, RF_ERR_DEVICE);  // stopped before finishing
+  S->active.clear();
+}
+
+void free_tables(rf_server* S) {
+  for (void* p : {static_cast<void*>(S->d_sus), static_cast<void*>(S->d_it), static_cast<void*>(S->d_targets),
+                  static_cast<void*>(S->d_vblocks), static_cast<void*>(S->d_fblocks), static_cast<void*>(S->d_part),
+                  static_cast<void*>(S->d_hout), static_cast<void*>(S->d_ycoef)})
+    if (p) (void)hipFree(p);
+  for (void* p : {static_cast<void*>(S->h_sus), static_cast<void*>(S->h_it), static_cast<void*>(S->h_targets),
+                  static_cast<void*>(S->h_vblocks), static_cast<void*>(S->h_fblocks), static_cast<void*>(S->h_hout),
+                  static_cast<void*>(S->h_ycoef)})
+    if (p) (void)hipHostFree(p);
+}
+
+}  // namespace
+
+extern "C" {
+
+rf_status rf_server_create(const rf_server_config* cfg, rf_server** out) {
+  if (!cfg || !out) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  rf_config base = cfg->base;
+  base.number_of_fibers = 1;  // per request
+  rf_status st = rf::check_config(&base);
+  if (st != RF_OK) return st;
+  if (base.solver != RF_SOLVER_GMRES)
+    return rf::set_error(RF_ERR_UNSUPPORTED, "the server steps its suspensions with GMRES");
+  const bool analytical = base.integration == RF_INTEGRATION_ANALYTICAL;
+  const bool vel_analytical = base.velocity_integration == RF_VELOCITY_FOLLOW
+                                  ? analytical
+                                  : base.velocity_integration == RF_VELOCITY_ANALYTICAL;
+  if (vel_analytical != analytical)
+    return rf::set_error(RF_ERR_UNSUPPORTED, "the server integrates the velocities like the system");
+  if (cfg->max_active < 1) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "max_active must be >= 1");
+  if (cfg->memory_budget < 0) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "memory_budget must be >= 0");
+  auto* S = new (std::nothrow) rf_server();
+  if (!S) return rf::set_error(RF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  S->cfg = *cfg;
+  S->cfg.base = base;
+  S->N = base.number_of_terms_in_force_expansion;
+  S->NQI = base.number_of_quadrature_intervals;
+  S->m = base.gmres_restart;
+  S->kdim = S->m + 1;
+  S->analytical = analytical;
+  rf::physical_constants(base, &S->pc);
+  S->t0 = Clock::now();
+  if (hipGetDevice(&S->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    delete S;
+    return rf::set_error(RF_ERR_DEVICE, "no HIP device or stream");
+  }
+  S->budget = cfg->memory_budget;
+  if (S->budget == 0) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipStreamDestroy(S->stream);
+      delete S;
+      return rf::set_error(RF_ERR_DEVICE, "hipMemGetInfo failed");
+    }
+    S->budget = static_cast<int64_t>(0.8 * static_cast<double>(free_b));
+  }
+  try {
+    S->worker = std::thread(worker_main, S);
+  } catch (...) {
+    (void)hipStreamDestroy(S->stream);
+    delete S;
+    return rf::set_error(RF_ERR_OUT_OF_MEMORY, "cannot start the worker thread");
+  }
+  *out = S;
+  return RF_OK;
+}
+
+void rf_server_destroy(rf_server* S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  if (S->worker.joinable()) S->worker.join();
+  (void)hipStreamSynchronize(S->stream);
+  free_tables(S);
+  for (auto& kv : S->requests) delete kv.second;
+  (void)hipStreamDestroy(S->stream);
+  delete S;
+}
+
+rf_status rf_server_submit(rf_server* S, int32_t fibers, const double* positions, const double* orientations,
+                           int32_t steps, int64_t* id) {
+  if (!S || !positions || !orientations || !id) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (fibers < 1 || steps < 1) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "fibers and steps must be >= 1");
+  const size_t f3 = 3 * static_cast<size_t>(fibers);
+  for (size_t i = 0; i < f3; ++i)
+    if (!std::isfinite(positions[i]) || !std::isfinite(orientations[i]))
+      return rf::set_error(RF_ERR_INVALID_ARGUMENT, "non-finite initial state");
+  auto* r = new (std::nothrow) Request();
+  if (!r) return rf::set_error(RF_ERR_OUT_OF_MEMORY, "host allocation failed");
+  r->M = fibers;
+  r->steps = steps;
+  r->footprint = footprint(S, fibers, nullptr);
+  if (r->footprint > S->budget) {
+    delete r;
+    return rf::set_error(RF_ERR_OUT_OF_MEMORY, "the suspension's footprint exceeds the server's memory budget");
+  }
+  r->pos0.assign(positions, positions + f3);
+  r->ori0.assign(orientations, orientations + f3);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    r->id = S->next_id++;
+    r->info.id = r->id;
+    r->info.fibers = fibers;
+    r->info.steps = steps;
+    r->info.footprint = r->footprint;
+    r->info.submitted = S->now();
+    S->requests[r->id] = r;
+    S->queue.push_back(r);
+  }
+  S->cv_work.notify_all();
+  *id = r->id;
+  return RF_OK;
+}
+
+rf_status rf_server_wait(rf_server* S, int64_t id, double timeout_s, int32_t* done) {
+  if (!S) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::unique_lock<std::mutex> lk(S->mu);
+  auto itr = S->requests.find(id);
+  if (itr == S->requests.end()) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "unknown request");
+  Request* r = itr->second;
+  if (timeout_s < 0)
+    S->cv_done.wait(lk, [&] { return r->done; });
+  else
+    S->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] { return r->done; });
+  if (done) *done = r->done ? 1 : 0;
+  return RF_OK;
+}
+
+rf_status rf_server_collect(rf_server* S, int64_t id, double* positions, double* orientations, double* step_ms,
+                            rf_request_info* info) {
+  if (!S) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  Request* r = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto itr = S->requests.find(id);
+    if (itr == S->requests.end()) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "unknown request");
+    r = itr->second;
+    if (!r->done) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "request has not finished");
+    S->requests.erase(itr);
+  }
+  const size_t f3 = 3 * static_cast<size_t>(r->M);
+  if (positions && r->pos_out.size() == f3) std::memcpy(positions, r->pos_out.data(), f3 * sizeof(double));
+  if (orientations && r->ori_out.size() == f3) std::memcpy(orientations, r->ori_out.data(), f3 * sizeof(double));
+  if (step_ms)
+    for (int k = 0; k < r->steps; ++k) step_ms[k] = k < static_cast<int>(r->step_ms.size()) ? r->step_ms[k] : 0.0;
+  if (info) *info = r->info;
+  delete r;
+  return RF_OK;
+}
+
+rf_status rf_server_get_stats(rf_server* S, rf_server_stats* st) {
+  if (!S || !st) return rf::set_error(RF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(S->mu);
+  st->ticks = S->ticks;
+  st->fiber_steps = S->fiber_steps;
+  st->completed = S->completed;
+  st->queued = static_cast<int64_t>(S->queue.size());
+  st->active = S->n_active;
+  st->busy_seconds = S->busy;
+  st->peak_bytes = S->peak;
+  st->memory_budget = S->budget;
+  return RF_OK;
+}
+
+double rf_server_clock(rf_server* S) { return S ? S->now() : 0.0; }
+
+}  // extern "C"

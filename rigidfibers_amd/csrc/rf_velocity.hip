@@ -198,12 +198,12 @@ __global__ void velocity_finalize_kernel(int M, int groups, const double* __rest
 // reduction.
 constexpr int kVelStoredThreads = 256;
 
-__global__ __launch_bounds__(kVelStoredThreads) void velocity_stored_kernel(
-    int M, int N, int64_t Lp, const float* __restrict__ Tv, const double* __restrict__ cpart, int groups,
-    const double* __restrict__ ori, const double* __restrict__ x, PhysConsts pc, double* __restrict__ tvel,
-    double* __restrict__ rvel) {
+__device__ __forceinline__ void velocity_stored_block(int i, int M, int N, int64_t Lp, const float* __restrict__ Tv,
+                                                      const double* __restrict__ cpart, int groups,
+                                                      const double* __restrict__ ori, const double* __restrict__ x,
+                                                      const PhysConsts& pc, double* __restrict__ tvel,
+                                                      double* __restrict__ rvel) {
   __shared__ double red[kVelStoredThreads / 64][6];
-  const int i = blockIdx.x;
   const float* Vi = Tv + static_cast<int64_t>(i) * 12 * Lp;
   const int L = M * N;
   double sw[3] = {0.0, 0.0, 0.0}, sv[3] = {0.0, 0.0, 0.0};
@@ -256,6 +256,31 @@ __global__ __launch_bounds__(kVelStoredThreads) void velocity_stored_kernel(
   }
 }
 
+__global__ __launch_bounds__(kVelStoredThreads) void velocity_stored_kernel(
+    int M, int N, int64_t Lp, const float* __restrict__ Tv, const double* __restrict__ cpart, int groups,
+    const double* __restrict__ ori, const double* __restrict__ x, PhysConsts pc, double* __restrict__ tvel,
+    double* __restrict__ rvel) {
+  velocity_stored_block(blockIdx.x, M, N, Lp, Tv, cpart, groups, ori, x, pc, tvel, rvel);
+}
+
+// Serving: workgroup = (suspension, target fiber); one group of velocity constants.
+__global__ __launch_bounds__(kVelStoredThreads) void velocity_stored_batch_kernel(int N,
+                                                                                  const SusDev* __restrict__ sus,
+                                                                                  const int2* __restrict__ targets,
+                                                                                  PhysConsts pc) {
+  const int2 e = targets[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  velocity_stored_block(e.y, d.M, N, d.Lp, d.Tv, d.cpart, 1, d.ori, d.x, pc, d.tv, d.rv);
+}
+
+hipError_t launch_velocities_stored_batch(int N, int ntargets, const SusDev* sus, const int2* targets,
+                                          const PhysConsts& pc, hipStream_t s) {
+  if (ntargets <= 0) return hipSuccess;
+  hipLaunchKernelGGL(velocity_stored_batch_kernel, dim3(ntargets), dim3(kVelStoredThreads), 0, s, N, sus, targets,
+                     pc);
+  return hipGetLastError();
+}
+
 hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, const double* cpart, int groups,
                                     const double* ori, const double* x, const PhysConsts& pc, double* tvel,
                                     double* rvel, hipStream_t s) {
@@ -264,14 +289,12 @@ hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, c
   return hipGetLastError();
 }
 
-__global__ void update_kernel(int first, int M, double dt, const double* __restrict__ pos_prev,
-                              const double* __restrict__ pos, const double* __restrict__ ori_prev,
-                              const double* __restrict__ ori, const double* __restrict__ tv_prev,
-                              const double* __restrict__ tv, const double* __restrict__ rv_prev,
-                              const double* __restrict__ rv, double* __restrict__ pos_next,
-                              double* __restrict__ ori_next) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= M) return;
+__device__ __forceinline__ void update_fiber(int i, int first, double dt, const double* __restrict__ pos_prev,
+                                             const double* __restrict__ pos, const double* __restrict__ ori_prev,
+                                             const double* __restrict__ ori, const double* __restrict__ tv_prev,
+                                             const double* __restrict__ tv, const double* __restrict__ rv_prev,
+                                             const double* __restrict__ rv, double* __restrict__ pos_next,
+                                             double* __restrict__ ori_next) {
   double o[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -287,6 +310,33 @@ __global__ void update_kernel(int first, int M, double dt, const double* __restr
   const double inv = 1.0 / sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
 #pragma unroll
   for (int a = 0; a < 3; ++a) ori_next[3 * i + a] = o[a] * inv;
+}
+
+__global__ void update_kernel(int first, int M, double dt, const double* __restrict__ pos_prev,
+                              const double* __restrict__ pos, const double* __restrict__ ori_prev,
+                              const double* __restrict__ ori, const double* __restrict__ tv_prev,
+                              const double* __restrict__ tv, const double* __restrict__ rv_prev,
+                              const double* __restrict__ rv, double* __restrict__ pos_next,
+                              double* __restrict__ ori_next) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < M) update_fiber(i, first, dt, pos_prev, pos, ori_prev, ori, tv_prev, tv, rv_prev, rv, pos_next, ori_next);
+}
+
+// Serving: entry (suspension, block of 256 fibers).
+__global__ __launch_bounds__(256) void update_batch_kernel(const SusDev* __restrict__ sus,
+                                                           const int2* __restrict__ fblocks, double dt) {
+  const int2 e = fblocks[blockIdx.x];
+  const SusDev& d = sus[e.x];
+  const int i = e.y * 256 + threadIdx.x;
+  if (i < d.M)
+    update_fiber(i, d.first, dt, d.pos_prev, d.pos, d.ori_prev, d.ori, d.tv_prev, d.tv, d.rv_prev, d.rv, d.pos_next,
+                 d.ori_next);
+}
+
+hipError_t launch_update_batch(int nfblocks, const SusDev* sus, const int2* fblocks, double dt, hipStream_t s) {
+  if (nfblocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(update_batch_kernel, dim3(nfblocks), dim3(256), 0, s, sus, fblocks, dt);
+  return hipGetLastError();
 }
 
 template <int N, int NQI>

@@ -49,6 +49,48 @@ def test_config_struct_layout_and_defaults():
     assert b"NULL" in lib.rf_last_error()
 
 
+def test_struct_layouts_match_the_header(tmp_path):
+    """Every struct of include/rigidfibers.h has the size and field offsets the
+    ctypes binding assumes (a C program compiled against the header prints them)."""
+    import subprocess
+
+    from rigidfibers_amd import _lib
+
+    structs = {"rf_config": _lib.RfConfig, "rf_solver_stats": _lib.RfSolverStats, "rf_timings": _lib.RfTimings,
+               "rf_server_config": _lib.RfServerConfig, "rf_request_info": _lib.RfRequestInfo,
+               "rf_server_stats": _lib.RfServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rigidfibers.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
+
+def test_request_stream_is_seeded():
+    from rigidfibers_amd.serve import request_stream, summarize
+
+    a = request_stream(count=6, m_range=(16, 40), rate=50.0, steps=3, seed=4)
+    b = request_stream(count=6, m_range=(16, 40), rate=50.0, steps=3, seed=4)
+    assert [r[1] for r in a] == [r[1] for r in b] and all(np.array_equal(x[2], y[2]) for x, y in zip(a, b))
+    assert all(16 <= r[1] <= 40 for r in a) and all(t1 > t0 for (t0, *_), (t1, *_) in zip(a, a[1:]))
+    for arrival, M, pos, ori, steps in a:
+        side = (M / 0.1) ** (1 / 3)
+        assert np.abs(pos).max() <= side / 2 and np.allclose(np.linalg.norm(ori.reshape(-1, 3), axis=1), 1.0)
+    res = [{"fibers": 10, "steps": 2, "step_ms": [1.0, 3.0], "arrival": 0.0, "admitted": 0.5, "finished": 1.0}]
+    s = summarize(res, 2.0)
+    assert s["fiber_steps"] == 20 and s["fiber_steps_per_s"] == 10.0 and s["step_ms_p50"] == 2.0
+
+
 def test_read_configuration_and_validate_override():
     from rigidfibers_amd.config import ConfigError, read_configuration
 

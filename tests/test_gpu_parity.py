@@ -172,6 +172,49 @@ def test_stored_velocities_follow_state_changes(oracle64):
     assert rel_err(v, v_ref) < 1e-6 and rel_err(w, w_ref) < 1e-6
 
 
+@pytest.mark.parametrize("analytical", [False, True])
+def test_m2000_sampled_rows_against_oracle(oracle64, oracle32, analytical):
+    """M = 2000 fibers in a cube of side 40 (n = 30 000): the rows of A and b of
+    three groups of target fibers against the FP64 oracle's rows (bound: the
+    larger of 2e-6 and 3x the FP32 oracle's error on the same rows), the
+    residual of the GPU's GMRES solution on those rows, and the velocities of
+    the sampled fibers from the GPU's solution against the oracle's."""
+    from oracle.pyoracle import make_params
+    from rigidfibers_amd.scenes import cube
+
+    M = 2000
+    pos, ori = cube(M, 40.0, min_dist=0.4, seed=2000)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-6
+    with _sim(params, pos, ori, integration="analytical" if analytical else "numerical") as sim:
+        sim.assemble_system()
+        A, b, _ = sim.system(with_matrix=True)
+        sim.solve_system()
+        st = sim.solver_stats()
+        _, _, x = sim.system()
+        sim.update_velocities()
+        v, w = sim.velocities()
+    assert st["converged"]
+    p = make_params(M)
+    bnorm = np.linalg.norm(b)
+    for i0 in (0, 997, 1997):
+        i1 = i0 + 3
+        rows = slice(15 * i0, 15 * i1)
+        Ar, br = oracle64.assemble_rows(p, pos, ori, i0, i1, analytical)
+        A32, b32 = oracle32.assemble_rows(p, pos, ori, i0, i1, analytical)
+        ea, eb = rel_err(A[rows], Ar), rel_err(b[rows], br)
+        ra, rb = rel_err(A32, Ar), rel_err(b32, br)
+        r = br - Ar @ x
+        vr, wr = oracle64.velocities_rows(p, pos, ori, x, i0, i1, analytical)
+        ev, ew = rel_err(v[3 * i0:3 * i1], vr), rel_err(w[3 * i0:3 * i1], wr)
+        print(f"rows {i0}..{i1}: A {ea:.1e} (FP32 oracle {ra:.1e}) b {eb:.1e} (FP32 oracle {rb:.1e}) "
+              f"residual {np.linalg.norm(r) / bnorm:.1e} v {ev:.1e} w {ew:.1e}")
+        assert ea < max(2e-6, 3 * ra) and eb < max(2e-6, 3 * rb)
+        assert np.linalg.norm(r) <= 1e-5 * bnorm
+        assert ev < 1e-6 and ew < 1e-6
+
+
 def test_single_fiber_known_answer():
     """M = 1: A = I, b = 0, x = 0, v = (0.5/d)((d+2) F + (d-2)(t.F) t), w = 0
     (src/kernels/reset_velocities.cu:22-38)."""
@@ -218,7 +261,7 @@ def test_ring_symmetry():
 
 @pytest.mark.parametrize("solver,M", [("gmres", 60), ("bicgstab", 60), ("direct", 60), ("direct", 230),
                                       ("mixed-bf16", 60), ("mixed-fp16", 60), ("mixed-bf16", 230),
-                                      ("mixed-fp16", 230)])
+                                      ("mixed-fp16", 230), ("gmres-ir", 60), ("gmres-ir", 230)])
 def test_solvers_agree_with_oracle(oracle64, solver, M):
     """x of each solver against the oracle's LU solution of the oracle's system
     (n = 900 and 3450: one and several 256-column outer LU panels with ragged
@@ -240,6 +283,30 @@ def test_solvers_agree_with_oracle(oracle64, solver, M):
     print(solver, st, rel_err(x, x_ref))
     assert st["converged"]
     assert rel_err(x, x_ref) < 1e-6
+
+
+def test_gmres_ir_uses_the_fp16_copy():
+    """GMRES-IR runs its inner iterations on the FP16 copy and reaches the
+    outer tolerance on the FP32 system; its states follow plain GMRES."""
+    M = 120
+    pos, ori = random_scene(M, side=6.0, seed=17, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-10
+    params.GMRES_RESTART = 30
+    out = {}
+    for solver in ("gmres", "gmres-ir"):
+        with _sim(params, pos, ori, solver=solver) as sim:
+            for t in range(3):
+                sim.step(t)
+            st, tm = sim.solver_stats(), sim.timings()
+            out[solver] = sim.state()
+        assert st["converged"] and st["relative_residual"] <= 1e-10
+        if solver == "gmres-ir":
+            assert tm["gemv_half_launches"] > 0 and tm["gemv_launches"] >= 2
+    ep, eo = rel_err(out["gmres-ir"][0], out["gmres"][0]), rel_err(out["gmres-ir"][1], out["gmres"][1])
+    print(f"gmres-ir vs gmres after 3 steps: pos {ep:.1e} ori {eo:.1e}")
+    assert ep < 1e-9 and eo < 1e-9
 
 
 def test_ten_steps_against_oracle(oracle64):

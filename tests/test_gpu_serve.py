@@ -1,0 +1,102 @@
+"""Serving engine (rf_server_*): continuous batching of independent
+suspensions on one GPU.
+
+  * each result is bitwise the same served alone and served in a mix;
+  * a served suspension follows the single-simulation path (Simulation) to
+    the GMRES tolerance;
+  * admission respects the memory budget and the slot count, and a request
+    whose footprint exceeds the budget is refused.
+"""
+import numpy as np
+import pytest
+
+from helpers import ref_config, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(tol=1e-8):
+    p = ref_config(validate=False)
+    p.GMRES_TOLERANCE = tol
+    return p
+
+
+def _requests(sizes, steps, seed=7):
+    from rigidfibers_amd.scenes import cube
+
+    return [(M, *cube(M, (10.0 * M) ** (1 / 3), min_dist=0.4, seed=seed + k), steps) for k, M in enumerate(sizes)]
+
+
+def _serve(server, reqs):
+    ids = [(server.submit(pos, ori, steps), M, steps) for M, pos, ori, steps in reqs]
+    out = []
+    for rid, M, steps in ids:
+        assert server.wait(rid, timeout=120.0)
+        out.append(server.collect(rid, M, steps))
+    return out
+
+
+def test_result_same_alone_and_in_mix():
+    from rigidfibers_amd.serve import Server
+
+    reqs = _requests([23, 64, 17, 120, 45, 90], steps=4)
+    with Server(_params(), max_active=8) as srv:
+        alone = [_serve(srv, [r])[0] for r in reqs]
+        mixed = _serve(srv, reqs)
+        st = srv.stats()
+    for a, b in zip(alone, mixed):
+        assert a["info"]["status"] == 0 and b["info"]["status"] == 0
+        assert np.array_equal(a["positions"], b["positions"]) and np.array_equal(a["orientations"], b["orientations"])
+        assert a["info"]["gmres_iterations"] == b["info"]["gmres_iterations"]
+    assert st["completed"] == 2 * len(reqs) and st["queued"] == 0 and st["active"] == 0
+
+
+@pytest.mark.parametrize("integration", ["numerical", "analytical"])
+def test_served_matches_simulation(integration):
+    from rigidfibers_amd.serve import Server
+    from rigidfibers_amd.simulation import Simulation
+
+    params = _params(tol=1e-10)
+    reqs = _requests([37, 80], steps=5, seed=11)
+    with Server(params, integration=integration, max_active=4) as srv:
+        served = _serve(srv, reqs)
+    for (M, pos, ori, steps), r in zip(reqs, served):
+        p = _params(tol=1e-10)
+        p.NUMBER_OF_FIBERS = M
+        with Simulation(p, pos, ori, solver="gmres", integration=integration) as sim:
+            for t in range(steps):
+                sim.step(t)
+            ps, os_ = sim.state()
+        ep, eo = rel_err(r["positions"], ps), rel_err(r["orientations"], os_)
+        print(f"M={M} {integration}: pos {ep:.1e} ori {eo:.1e}")
+        assert ep < 1e-8 and eo < 1e-8
+
+
+def test_memory_budget_and_slots():
+    from rigidfibers_amd.serve import Server
+
+    reqs = _requests([60, 60, 60, 60], steps=3, seed=3)
+    with Server(_params(), max_active=8) as srv:  # learn the footprint
+        fp = _serve(srv, reqs[:1])[0]["info"]["footprint"]
+    budget = int(1.5 * fp)  # room for one suspension at a time
+    with Server(_params(), memory_budget=budget, max_active=8) as srv:
+        out = _serve(srv, reqs)
+        st = srv.stats()
+    assert st["peak_bytes"] <= budget and st["memory_budget"] == budget
+    spans = sorted((r["info"]["admitted"], r["info"]["finished"]) for r in out)
+    for (a0, f0), (a1, f1) in zip(spans, spans[1:]):
+        assert a1 >= f0 - 1e-6  # never two admitted together
+    with Server(_params(), max_active=2) as srv:
+        out = _serve(srv, _requests([30, 31, 32, 33, 34], steps=2, seed=5))
+    assert all(r["info"]["status"] == 0 for r in out)
+
+
+def test_oversized_request_refused():
+    from rigidfibers_amd import _lib
+    from rigidfibers_amd.serve import Server
+
+    (M, pos, ori, steps), = _requests([50], steps=1)
+    with Server(_params(), memory_budget=1 << 20, max_active=4) as srv:
+        with pytest.raises(_lib.RfError) as e:
+            srv.submit(pos, ori, steps)
+    assert e.value.code == _lib.RF_ERR_OUT_OF_MEMORY
