@@ -25,6 +25,10 @@ Also reported:
                 refinement (to 1e-10) on the same scene: fiber-steps/s, the
                 speed-up, and the relative position error against the FP32-LU
                 run after the same steps;
+  serving       the serving stream with continuous batching against the same
+                stream one request at a time: p50/p99 step and request latency;
+  large         M = 10 000: GMRES-IR on the FP16 theta copy against plain
+                GMRES on the FP32 theta system (speed-up, position difference);
   cpu_baseline  the FP32 OpenMP port of the reference Fortran (oracle build
                 librf_oracle32.so) on a bounded sample: the rows of S target
                 fibers assembled against all 5000 sources, their velocities,
@@ -173,6 +177,59 @@ def serving(args):
     return out
 
 
+def large_suspension(args):
+    """M = 10 000 fibers in a cube of side 60 (n = 150 000), numerical
+    integrals, GMRES restart 30 to 1e-8, 5 steps: GMRES-IR on the FP16 theta
+    copy (FP32 residuals) against plain GMRES on the FP32 theta system."""
+    from rigidfibers_amd.config import Parameters
+    from rigidfibers_amd.scenes import cube
+    from rigidfibers_amd.simulation import Simulation
+
+    M = args.large_fibers
+    pos, ori = cube(M, 60.0 * (M / 10000.0) ** (1.0 / 3.0), min_dist=0.4, seed=args.seed)
+    params = Parameters(NUMBER_OF_FIBERS=M, TIMESTEP=0.1, SLENDERNESS=0.01, NUMBER_OF_TERMS_IN_FORCE_EXPANSION=5,
+                        NUMBER_OF_QUADRATURE_INTERVALS=8, GMRES_RESTART=30, GMRES_MAX_ITERATIONS=2000,
+                        GMRES_TOLERANCE=1e-8)
+    out, ref = {"workload": f"M={M} cube of side {60.0 * (M / 10000.0) ** (1.0 / 3.0):g}, numerical, GMRES(30) "
+                            f"tol 1e-8, 1 warm-up + {args.large_steps} timed steps"}, None
+    for solver in ("gmres", "gmres-ir"):
+        with Simulation(params, pos, ori, solver=solver, integration="numerical") as sim:
+            iters, gemv, gemv_n, half, half_n, solve = [], 0.0, 0, 0.0, 0, 0.0
+            sim.step(0)  # untimed warm-up (cold GMRES start)
+            t0 = time.perf_counter()
+            for t in range(1, args.large_steps + 1):
+                sim.step(t)
+                tm = sim.timings()
+                iters.append(sim.solver_stats()["iterations"])
+                gemv += tm["gemv"]
+                gemv_n += tm["gemv_launches"]
+                half += tm["gemv_half"]
+                half_n += tm["gemv_half_launches"]
+                solve += tm["solve_system"]
+            elapsed = time.perf_counter() - t0
+            p_end, o_end = sim.state()
+            dev = sim.device_bytes()
+        per32 = float(M) * 27 * M * 5 * 4
+        entry = {"fiber_steps_per_s": round(M * args.large_steps / elapsed, 1),
+                 "ms_per_step": round(1e3 * elapsed / args.large_steps, 2),
+                 "solve_ms_per_step": round(solve / args.large_steps, 2), "iterations_per_step": iters,
+                 "fp32_matvecs": gemv_n, "fp32_matvec_ms": round(gemv / max(gemv_n, 1), 4),
+                 "fp32_matvec_GBps": round(per32 / (gemv / max(gemv_n, 1) * 1e-3) / 1e9, 1) if gemv_n else None,
+                 "device_bytes": dev}
+        if half_n:
+            per16 = per32 / 2 + float(M) * M * 4
+            entry.update({"fp16_matvecs": half_n, "fp16_matvec_ms": round(half / half_n, 4),
+                          "fp16_matvec_GBps": round(per16 / (half / half_n * 1e-3) / 1e9, 1)})
+        if ref is None:
+            ref = (p_end, o_end, elapsed)
+        else:
+            entry["speedup_vs_fp32_gmres"] = round(ref[2] / elapsed, 3)
+            entry["pos_rel_err_vs_fp32_gmres"] = rel_err(p_end, ref[0])
+            entry["ori_rel_err_vs_fp32_gmres"] = rel_err(o_end, ref[1])
+        out[solver.replace("-", "_")] = entry
+    return out
+
+
 def cpu_baseline(args, pos, ori, iters_per_step):
     """FP32 OpenMP port of the reference Fortran on a bounded sample of rows."""
     import numpy as np
@@ -237,6 +294,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=64, help="suspensions stepped together")
     ap.add_argument("--serve-serial", type=int, default=512, help="requests of the one-at-a-time comparison")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--large-fibers", type=int, default=10000)
+    ap.add_argument("--large-steps", type=int, default=5)
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU-baseline sampling")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -278,6 +338,8 @@ def main():
         line["precision"] = precision_study(args, pos, ori)
     if not args.no_serving:
         line["serving"] = serving(args)
+    if not args.no_large:
+        line["large"] = large_suspension(args)
     if not args.no_cpu_baseline:
         iters = sum(g["iters"]) / len(g["iters"]) + 1.0 if args.solver in ITERATIVE else 1.0
         line["cpu_baseline"] = cpu_baseline(args, pos, ori, iters)

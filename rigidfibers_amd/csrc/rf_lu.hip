@@ -63,27 +63,34 @@ struct DiagLane {
   int zero_col;      // first column with a zero pivot (1-based), 0 if none
 };
 
+// Maximum of an unsigned over the wavefront by DPP (xor 1, xor 2, half-row and
+// row mirrors, then row broadcasts 15 and 31): lane 63 ends with the maximum.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  auto step = [](unsigned x, unsigned y) { return x > y ? x : y; };
+  v = step(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xf, 0xf, false)));
+  v = step(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xf, 0xf, false)));
+  v = step(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xf, 0xf, false)));
+  v = step(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xf, 0xf, false)));
+  v = step(v, static_cast<unsigned>(
+                  __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+  v = step(v, static_cast<unsigned>(
+                  __builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x143, 0xc, 0xf, false)));
+  return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 // Elimination step for column C (compile-time, so the row stays in registers).
 template <int C>
 __device__ __forceinline__ void diag_step(DiagLane& d, int kb, int lane) {
   if constexpr (C < kLuNb) {
     if (C >= kb) return;
-    // pivot: largest |row[C]| among the rows not yet chosen, ties to the lower
-    // row; NaN ranks highest so the order is total and every row gets a place
+    // pivot: largest |row[C]| among the rows not yet chosen.  The key orders
+    // by |value| (IEEE bits of a non-negative float order as integers, the
+    // lowest 6 mantissa bits dropped) and then by the lower row; NaN ranks
+    // highest; chosen rows have key 0, every candidate a key >= 64.
     const float a = d.row[C];
-    float v = d.used ? -1.0f : (a != a ? INFINITY : fabsf(a));
-    int idx = lane;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const float ov = __shfl_xor(v, off, 64);
-      const int oi = __shfl_xor(idx, off, 64);
-      if (ov > v || (ov == v && oi < idx)) {
-        v = ov;
-        idx = oi;
-      }
-    }
-    // every lane holds the same idx; the pivot row is read lane-to-scalar
-    const int pr = __builtin_amdgcn_readfirstlane(idx);
+    const unsigned bits = __builtin_bit_cast(unsigned, fabsf(a)) >> 6;
+    const unsigned key = d.used ? 0u : (((bits + 1u) << 6) | static_cast<unsigned>(63 - lane));
+    const int pr = 63 - static_cast<int>(wave_max_u32(key) & 63u);
     if (lane == pr) {
       d.used = true;
       d.pos = C;
@@ -164,7 +171,7 @@ __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int6
                                                     int64_t k0, int64_t K0, const int* __restrict__ perm,
                                                     T* __restrict__ Bt) {
   __shared__ float S[kLuNb][kLuNb + 1];   // [column][row]
-  __shared__ float Lr[kLuNb][kLuNb + 1];  // Lr[r][j] = L[r][j]
+  __shared__ __attribute__((aligned(16))) float Lr[kLuNb][kLuNb + 4];  // Lr[r][j] = L[r][j], 16-byte rows
   const int lane = threadIdx.x;
   const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * kLuNb;
 #pragma unroll 8
