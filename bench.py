@@ -97,6 +97,18 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
             "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps}
 
 
+def measured_traffic(kernel, M, N):
+    """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE counter
+    passes (scaled by the copy-kernel calibration), or None for other shapes."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get(f"{kernel} M={M} N={N}")
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else float(rec["fetch_bytes"] + rec["write_bytes"])
+
+
 def roofline(args, solver, g):
     M, N = args.fibers, 5
     acc = g["acc"]
@@ -109,8 +121,8 @@ def roofline(args, solver, g):
         avg_ms = acc["gemv"] / max(acc["gemv_launches"], 1)
         achieved = per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
         return {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": per_launch,
-                "avg_launch_ms": round(avg_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel, M, N),
+                "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4)}
     peak = MFMA_PEAK_TFLOPS[solver]
     achieved = acc["gemm_flop"] / (acc["gemm"] * 1e-3) / 1e12 if acc["gemm"] > 0 else 0.0
     return {"bound": "mfma", "kernel": GEMM_KERNEL[solver], "achieved": round(achieved, 1), "peak": peak,
