@@ -5,7 +5,7 @@ Workload (the configuration the headline metric is quoted on): M = 5000 fibers
 of length 2 with centres uniform in a ball of radius 35 (minimum centre
 distance 0.4, seed 2026), orientations uniform on the sphere, slenderness 0.01,
 N = 5 force terms, 8 x 3 quadrature, unit gravity, dt = 0.1, analytical inner
-integrals (FP64 recurrences), GMRES(10) to a relative residual of 1e-6 starting
+integrals (FP64 recurrences), GMRES(20) to a relative residual of 1e-6 starting
 from the previous solution.  n = 75 000 unknowns.
 
 One "step" is one call of the library's rf_step (assemble -> solve -> update
@@ -296,7 +296,7 @@ def main():
     ap.add_argument("--solver", default="gmres", choices=["gmres", "bicgstab", "direct", "mixed-bf16", "mixed-fp16"])
     ap.add_argument("--integration", default="analytical", choices=["analytical", "numerical"])
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--restart", type=int, default=10)
+    ap.add_argument("--restart", type=int, default=20)
     ap.add_argument("--matrix-format", default="auto", choices=["auto", "dense", "theta"])
     ap.add_argument("--velocity-operator", default="stored", choices=["stored", "pairwise"])
     ap.add_argument("--lu-steps", type=int, default=2, help="timed steps per solver of the precision study")
