@@ -172,13 +172,15 @@ def serving(args):
                         GMRES_TOLERANCE=1e-6)
     reqs = request_stream(count=args.serve_requests, m_range=(16, 400), density=0.1, rate=args.serve_rate,
                           steps=20, seed=args.seed)
-    with Server(params, integration="numerical", memory_budget=0, max_active=args.serve_slots) as srv:
+    with Server(params, integration="numerical", memory_budget=0, max_active=args.serve_slots,
+                eager=args.serve_eager) as srv:
         res, wall = serve_stream(srv, reqs)
         st = srv.stats()
     out = {"workload": f"{len(reqs)} suspensions, M ~ U[16,400], cube at density 0.1, 20 steps, numerical, "
                        f"GMRES tol 1e-6, Poisson {args.serve_rate:g}/s",
            "batched": summarize(res, wall),
            "server": {"ticks": st["ticks"], "busy_s": round(st["busy_seconds"], 3), "slots": args.serve_slots,
+                      "launches": "eager" if args.serve_eager else "graph",
                       "peak_bytes": st["peak_bytes"], "memory_budget": st["memory_budget"],
                       "failed": sum(1 for r in res if r["status"] not in (0, 100))}}
     if args.serve_serial > 0:
@@ -305,6 +307,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=64, help="suspensions stepped together")
     ap.add_argument("--serve-serial", type=int, default=512, help="requests of the one-at-a-time comparison")
+    ap.add_argument("--serve-eager", action="store_true",
+                    help="launch each GMRES iteration's kernels one by one instead of replaying a graph")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--large-fibers", type=int, default=10000)
     ap.add_argument("--large-steps", type=int, default=5)

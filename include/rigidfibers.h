@@ -224,7 +224,9 @@ typedef struct rf_server_config {
   rf_config base;            /* physics and GMRES settings shared by all suspensions */
   int64_t memory_budget;     /* device bytes for admitted suspensions; 0 = 80% of the free memory */
   int32_t max_active;        /* suspensions stepped together, >= 1 */
-  int32_t reserved[5];
+  int32_t eager;             /* 0: each tick captures its GMRES iteration's launches once as a HIP graph
+                                and replays it; 1: launch them one by one (same results) */
+  int32_t reserved[4];
 } rf_server_config;
 
 typedef struct rf_request_info {

@@ -92,7 +92,8 @@ class RfServerConfig(C.Structure):
         ("base", RfConfig),
         ("memory_budget", C.c_int64),
         ("max_active", C.c_int32),
-        ("reserved", C.c_int32 * 5),
+        ("eager", C.c_int32),
+        ("reserved", C.c_int32 * 4),
     ]
 
 

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -163,6 +164,8 @@ struct rf_server {
   size_t c_dsus = 0, c_hsus = 0, c_dit = 0, c_hit = 0, c_dt = 0, c_ht = 0, c_dv = 0, c_hv = 0, c_df = 0, c_hf = 0,
          c_part = 0, c_dh = 0, c_hh = 0, c_dy = 0, c_hy = 0;
   int ntargets = 0, nvb = 0, nfb = 0;
+  hipGraphExec_t gexec = nullptr;  // captured GMRES iteration of the admitted set
+  std::array<int64_t, 8> gkey{};   // the launch parameters it was captured with
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -296,9 +299,66 @@ bool build_tables(rf_server* S) {
          hipMemcpyAsync(S->d_fblocks, S->h_fblocks, nf * sizeof(int2), hipMemcpyHostToDevice, st) == hipSuccess;
 }
 
+// The kernels of one lockstep iteration over the admitted set (fixed shapes
+// within a tick: the Gram-Schmidt launches cover all m basis vectors and the
+// workgroups past a suspension's K exit).
+bool iteration_launches(rf_server* S) {
+  const int ns = static_cast<int>(S->active.size());
+  hipStream_t st = S->stream;
+  if (rf::launch_iter_prep_batch(S->nvb, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
+  if (rf::launch_gemv_theta_batch(S->N, S->ntargets, S->d_sus, S->d_it, S->d_targets, S->pc, st) != hipSuccess)
+    return false;
+  for (int ph = 0; ph < 2; ++ph) {
+    if (rf::launch_dots_batch(S->nvb, ns, S->m, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
+        hipSuccess)
+      return false;
+    if (rf::launch_orth_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
+  }
+  return rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) ==
+         hipSuccess;
+}
+
+void release_iteration(rf_server* S) {
+  if (S->gexec) (void)hipGraphExecDestroy(S->gexec);
+  S->gexec = nullptr;
+}
+
+// Captures iteration_launches as a graph.  Its launch parameters are the
+// table sizes and device addresses (the tables' contents are copied in before
+// every replay), so the graph is kept across ticks until the admitted set
+// changes them.  Kernels only: no memset or copy nodes.
+bool ensure_iteration_graph(rf_server* S) {
+  const std::array<int64_t, 8> key = {static_cast<int64_t>(S->active.size()),
+                                      S->nvb,
+                                      S->ntargets,
+                                      reinterpret_cast<int64_t>(S->d_sus),
+                                      reinterpret_cast<int64_t>(S->d_it),
+                                      reinterpret_cast<int64_t>(S->d_targets),
+                                      reinterpret_cast<int64_t>(S->d_vblocks),
+                                      reinterpret_cast<int64_t>(S->d_part)};
+  if (S->gexec && key == S->gkey) return true;
+  release_iteration(S);
+  hipStream_t st = S->stream;
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  const bool ok = iteration_launches(S);
+  if (hipStreamEndCapture(st, &g) != hipSuccess || !ok) {
+    if (g) (void)hipGraphDestroy(g);
+    return false;
+  }
+  const hipError_t e = hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    S->gexec = nullptr;
+    return false;
+  }
+  S->gkey = key;
+  return true;
+}
+
 // One lockstep iteration: upload the per-suspension descriptors, run the
-// launch sequence, bring every Hessenberg column back.
-bool lockstep_iteration(rf_server* S, int kmax) {
+// launch sequence (graph replay or eager), bring every Hessenberg column back.
+bool lockstep_iteration(rf_server* S) {
   const int ns = static_cast<int>(S->active.size());
   const int hs = 3 * (S->kdim + 1);
   hipStream_t st = S->stream;
@@ -306,19 +366,11 @@ bool lockstep_iteration(rf_server* S, int kmax) {
       hipMemcpyAsync(S->d_ycoef, S->h_ycoef, static_cast<size_t>(ns) * S->kdim * sizeof(double),
                      hipMemcpyHostToDevice, st) != hipSuccess)
     return false;
-  if (rf::launch_iter_prep_batch(S->nvb, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
-  if (rf::launch_gemv_theta_batch(S->N, S->ntargets, S->d_sus, S->d_it, S->d_targets, S->pc, st) != hipSuccess)
+  if (S->gexec) {
+    if (hipGraphLaunch(S->gexec, st) != hipSuccess) return false;
+  } else if (!iteration_launches(S)) {
     return false;
-  for (int ph = 0; ph < 2; ++ph) {
-    if (kmax > 0) {
-      if (rf::launch_dots_batch(S->nvb, ns, kmax, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
-          hipSuccess)
-        return false;
-      if (rf::launch_orth_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
-    }
   }
-  if (rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) != hipSuccess)
-    return false;
   if (hipMemcpyAsync(S->h_hout, S->d_hout, static_cast<size_t>(ns) * hs * sizeof(double), hipMemcpyDeviceToHost,
                      st) != hipSuccess)
     return false;
@@ -374,6 +426,7 @@ bool solve_lockstep(rf_server* S) {
       s->phase = kResidual;
     }
   }
+  if (!S->cfg.eager && !ensure_iteration_graph(S)) return false;
   while (true) {
     bool work = false;
     int kmax = 0;
@@ -405,7 +458,8 @@ bool solve_lockstep(rf_server* S) {
       if (s->phase != kDone || c.zero_x || c.ycount || c.svec) work = true;
     }
     if (!work) break;
-    if (!lockstep_iteration(S, kmax)) return false;
+    (void)kmax;
+    if (!lockstep_iteration(S)) return false;
     for (int q = 0; q < ns; ++q) {
       Suspension* s = S->active[q];
       const double* h = S->h_hout + static_cast<int64_t>(q) * hs;
@@ -682,6 +736,7 @@ void rf_server_destroy(rf_server* S) {
   S->cv_work.notify_all();
   if (S->worker.joinable()) S->worker.join();
   (void)hipStreamSynchronize(S->stream);
+  release_iteration(S);
   free_tables(S);
   for (auto& kv : S->requests) delete kv.second;
   (void)hipStreamDestroy(S->stream);

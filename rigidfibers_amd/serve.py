@@ -31,12 +31,14 @@ def _dptr(a):
 
 
 class Server:
-    def __init__(self, params: Parameters, integration="numerical", memory_budget=0, max_active=64, **kw):
+    def __init__(self, params: Parameters, integration="numerical", memory_budget=0, max_active=64, eager=False,
+                 **kw):
         self._lib = _lib.lib()
         cfg = _lib.RfServerConfig()
         cfg.base = make_config(params, "gmres", integration, **kw)
         cfg.memory_budget = int(memory_budget)
         cfg.max_active = int(max_active)
+        cfg.eager = 1 if eager else 0
         handle = C.c_void_p()
         _lib.check(self._lib.rf_server_create(C.byref(cfg), C.byref(handle)))
         self._h = handle

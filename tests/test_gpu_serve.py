@@ -51,6 +51,25 @@ def test_result_same_alone_and_in_mix():
     assert st["completed"] == 2 * len(reqs) and st["queued"] == 0 and st["active"] == 0
 
 
+def test_graph_replay_matches_eager_launches():
+    """Each tick replays its captured GMRES iteration graph many times (one
+    replay per lockstep iteration, host work and synchronisation between
+    replays); the results must be bitwise those of launching the same kernels
+    one by one."""
+    from rigidfibers_amd.serve import Server
+
+    reqs = _requests([23, 64, 17, 120, 45], steps=4, seed=5)
+    out = {}
+    for eager in (False, True):
+        with Server(_params(), max_active=8, eager=eager) as srv:
+            out[eager] = _serve(srv, reqs)
+            assert srv.stats()["ticks"] >= 4
+    for g, e in zip(out[False], out[True]):
+        assert g["info"]["status"] == 0 and e["info"]["status"] == 0
+        assert g["info"]["gmres_iterations"] == e["info"]["gmres_iterations"] > 4
+        assert np.array_equal(g["positions"], e["positions"]) and np.array_equal(g["orientations"], e["orientations"])
+
+
 @pytest.mark.parametrize("integration", ["numerical", "analytical"])
 def test_served_matches_simulation(integration):
     from rigidfibers_amd.serve import Server
