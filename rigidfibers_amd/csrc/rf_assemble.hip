@@ -275,20 +275,18 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
 // Column b of a symmetric 3x3 projection (v_a = S[(a, b)]) into its component
 // arrays s = xx yy zz xy xz yz at stride Lp: b = 0 writes xx xy xz, b = 1 yy yz,
 // b = 2 zz, so the three column jobs together write each component once.
-__device__ __forceinline__ void store_sym_column(float* base, int64_t Lp, int bcol, float v0, float v1, float v2) {
-  if (bcol == 0) {
-    base[0] = v0;       // xx
-    base[3 * Lp] = v1;  // xy
-    base[4 * Lp] = v2;  // xz
-  } else if (bcol == 1) {
-    base[Lp] = v1;      // yy
-    base[5 * Lp] = v2;  // yz
-  } else {
-    base[2 * Lp] = v2;  // zz
-  }
-}
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // The work of one workgroup: target fiber i, source group gy.
+//   phase 1  lane (p, q_i) writes G_m(q_i) of source slot p into LDS as row
+//            r = c*CB + (m == 0 ? p : PAIRS + p*N + m-1) of a GEMM operand
+//            (c = symmetric component; the mode-0 rows feed the RHS).
+//   phase 2  C[r][kk] = sum_q G[r][q] W[kk][q] (kk = 0..N: the projections onto
+//            w P_kk) on the matrix cores, 16 rows x 16 projections per
+//            v_mfma_f32_16x16x4f32 tile (lane group g takes q in [g Q/4, (g+1) Q/4)).
+//   phase 3  each lane writes 4 consecutive (source, mode) columns of one
+//            component array (16-byte stores); the k_i = 0 components
+//            (Theta_0 t_i) and the RHS of each source slot are formed here.
 template <int N, int NQI, bool ANALYTIC>
 __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64_t Lp, const double* __restrict__ pos,
                                                      const double* __restrict__ ori, float* __restrict__ T,
@@ -301,9 +299,18 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   constexpr int ROWS = 3 * N;
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int NR = ROWS + 6;  // RHS rows + the two velocity constants
-  static_assert(kPairThreads % Q == 0, "Q must divide the workgroup");
+  constexpr int PN = PAIRS * N;  // (source, mode) columns of one sweep
+  constexpr int CB = PAIRS + PN;  // GEMM rows per component
+  constexpr int TILES = (6 * CB + 15) / 16;
+  constexpr int RP = 16 * TILES;
+  constexpr int KQ = Q / 4;  // MFMA k-steps per tile
+  constexpr int NV = PN / 4;  // float4 chunks per component and sweep
+  constexpr int WAVES = kPairThreads / 64;
+  static_assert(kPairThreads % Q == 0 && Q % 4 == 0 && PAIRS % 4 == 0 && NM <= 16, "tile shapes");
 
-  __shared__ float lds[PAIRS][NM][6][Q];
+  __shared__ __align__(16) float G[RP * Q];
+  __shared__ __align__(16) float Pj[NM * RP];
+  __shared__ float Wl[NM * Q];
   __shared__ double bred[PAIRS][NR];
 
   const int tid = threadIdx.x;
@@ -319,14 +326,27 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   const double sq_d = node_fp64<NQI>(qi);
   const float sq = static_cast<float>(sq_d);
   float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
-
-  // RHS (and velocity-constant) sums of each source slot, kept in LDS (only
-  // PAIRS lanes update them)
-  for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
   float* Vi = Tv ? Tv + static_cast<int64_t>(i) * 12 * Lp : nullptr;
+
+  // RHS (and velocity-constant) sums of each source slot, kept in LDS
+  for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
+  if (tid == 0) {
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+#pragma unroll
+      for (int q = 0; q < Q; ++q) Wl[m * Q + q] = quad.W[m][q];
+  }
   __syncthreads();
 
-  constexpr int kJobs = PAIRS + 3 * PAIRS * N;
+  const int lane = tid & 63, col = lane & 15, g = lane >> 4;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nu = NC + (Vi ? 12 : 0);  // component arrays written per sweep
+  float wb[KQ];                       // this lane's B operand: W[col][g KQ + s]
+  {
+    const int wrow = min(col, NM - 1) * Q + g * KQ;
+#pragma unroll
+    for (int s = 0; s < KQ; ++s) wb[s] = col < NM ? Wl[wrow + s] : 0.0f;
+  }
 
   for (int j0 = jb; j0 < je; j0 += PAIRS) {
     const int j = j0 + p;
@@ -337,101 +357,97 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
 #pragma unroll
       for (int m = 0; m < NM; ++m)
 #pragma unroll
-        for (int c = 0; c < 6; ++c) lds[p][m][c][qi] = acc[m][c];
+        for (int c = 0; c < 6; ++c) G[(c * CB + (m == 0 ? p : PAIRS + p * N + m - 1)) * Q + qi] = acc[m][c];
     }
     __syncthreads();
-    for (int job = tid; job < kJobs; job += kPairThreads) {
-      if (job < PAIRS) {  // RHS job of source slot p2
-        const int p2 = job, j2 = j0 + p2;
-        if (j2 >= je || j2 == i) continue;
-        float TF[N][3], TV[3] = {0.0f, 0.0f, 0.0f};
+    {
+      for (int t = wv; t < TILES; t += WAVES) {
+        const float* a = G + (16 * t + col) * Q + g * KQ;
+        floatx4 c4 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int k = 0; k < N; ++k) TF[k][0] = TF[k][1] = TF[k][2] = 0.0f;
+        for (int s = 0; s < KQ; ++s) c4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], wb[s], c4, 0, 0, 0);
+        // c4[e] = C[16 t + 4 g + e][col]
+        if (col < NM) *reinterpret_cast<floatx4*>(Pj + col * RP + 16 * t + 4 * g) = c4;
+      }
+    }
+    __syncthreads();
+    const int npm = min(PAIRS, je - j0) * N;  // valid columns of this sweep
+    const int sself = (i - j0) * N;           // the self pair's columns, if in this sweep
+    const int64_t ecol = static_cast<int64_t>(j0) * N;
+    for (int job = tid; job < nu * NV + PAIRS; job += kPairThreads) {
+      if (job < nu * NV) {
+        const int u = job / NV;
+        const int pm0 = 4 * (job - u * NV);
+        floatx4 val;
+        if (u < 3) {  // Q0_b = (Theta_0 t_i)_b, Theta_0 = the P_1 projection
+          const float* b1 = Pj + RP + PAIRS + pm0;
+          const floatx4 x0 = *reinterpret_cast<const floatx4*>(b1 + sym_index(0, u) * CB);
+          const floatx4 x1 = *reinterpret_cast<const floatx4*>(b1 + sym_index(1, u) * CB);
+          const floatx4 x2 = *reinterpret_cast<const floatx4*>(b1 + sym_index(2, u) * CB);
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-          const float g0 = lds[p2][0][0][q], g1 = lds[p2][0][1][q], g2 = lds[p2][0][2][q];
-          const float g3 = lds[p2][0][3][q], g4 = lds[p2][0][4][q], g5 = lds[p2][0][5][q];
-          const float f0 = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
-          const float f1 = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
-          const float f2 = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
-#pragma unroll
-          for (int k = 0; k < N; ++k) {
-            TF[k][0] = fmaf(quad.W[k + 1][q], f0, TF[k][0]);
-            TF[k][1] = fmaf(quad.W[k + 1][q], f1, TF[k][1]);
-            TF[k][2] = fmaf(quad.W[k + 1][q], f2, TF[k][2]);
+          for (int e = 0; e < 4; ++e) val[e] = x0[e] * ti[0] + x1[e] * ti[1] + x2[e] * ti[2];
+        } else {
+          int kk, c;
+          if (u < NC) {  // Theta_{k_i}, k_i = 1..N-1: projection kk = k_i + 1
+            kk = 2 + (u - 3) / 6;
+            c = (u - 3) - 6 * (kk - 2);
+          } else {  // velocity operator: Theta_0 (kk = 1), then Phi_0 (kk = 0)
+            const int w = u - NC;
+            kk = w < 6 ? 1 : 0;
+            c = w < 6 ? w : w - 6;
           }
-          TV[0] = fmaf(quad.W[0][q], f0, TV[0]);
-          TV[1] = fmaf(quad.W[0][q], f1, TV[1]);
-          TV[2] = fmaf(quad.W[0][q], f2, TV[2]);
+          val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
         }
 #pragma unroll
-        for (int k = 0; k < N; ++k) {
-          const float QF = TF[k][0] * ti[0] + TF[k][1] * ti[1] + TF[k][2] * ti[2];
+        for (int e = 0; e < 4; ++e)
+          if (static_cast<unsigned>(pm0 + e - sself) < static_cast<unsigned>(N)) val[e] = 0.0f;
+        float* dst = (u < NC ? Ti + u * Lp : Vi + (u - NC) * Lp) + ecol + pm0;
+        if (pm0 + 3 < npm) {
+          *reinterpret_cast<floatx4*>(dst) = val;
+        } else {
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float v = (k == 0) ? pc.D1 * ti[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * ti[a] * QF);
-            bred[p2][3 * k + a] -= static_cast<double>(v);
-          }
+          for (int e = 0; e < 4; ++e)
+            if (pm0 + e < npm) dst[e] = val[e];
         }
-        // velocity constants: sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2
+        continue;
+      }
+      const int p2 = job - nu * NV, j2 = j0 + p2;  // RHS job of source slot p2
+      if (j2 >= je || j2 == i) continue;
+      // Projections of G_0 F/2: TF[k] onto w P_{k+1}, TV onto w
+      float TF[N][3], TV[3];
+#pragma unroll
+      for (int kk = 0; kk < NM; ++kk) {
+        const float* b0 = Pj + kk * RP + p2;
+        const float g0 = b0[0 * CB], g1 = b0[1 * CB], g2 = b0[2 * CB];
+        const float g3 = b0[3 * CB], g4 = b0[4 * CB], g5 = b0[5 * CB];
+        const float f0 = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
+        const float f1 = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
+        const float f2 = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
+        float* dst3 = kk == 0 ? TV : TF[kk - 1];
+        dst3[0] = f0;
+        dst3[1] = f1;
+        dst3[2] = f2;
+      }
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const float QF = TF[k][0] * ti[0] + TF[k][1] * ti[1] + TF[k][2] * ti[2];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          bred[p2][ROWS + a] += static_cast<double>(TV[a]);
-          bred[p2][ROWS + 3 + a] += static_cast<double>(TF[0][a]);
-        }
-        continue;
-      }
-      const int jj = job - PAIRS;
-      const int bcol = jj / (PAIRS * N);
-      const int rem = jj - bcol * (PAIRS * N);
-      const int p2 = rem / N;
-      const int m = 1 + (rem - p2 * N);
-      const int j2 = j0 + p2;
-      if (j2 >= je) continue;
-      const int64_t e = static_cast<int64_t>(j2) * N + (m - 1);
-      if (j2 == i) {
-        Ti[bcol * Lp + e] = 0.0f;
-#pragma unroll
-        for (int k = 1; k < N; ++k) store_sym_column(Ti + (3 + 6 * (k - 1)) * Lp + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
-        if (Vi) {
-          store_sym_column(Vi + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
-          store_sym_column(Vi + 6 * Lp + e, Lp, bcol, 0.0f, 0.0f, 0.0f);
-        }
-        continue;
-      }
-      const int c0 = sym_index(0, bcol), c1 = sym_index(1, bcol), c2 = sym_index(2, bcol);
-      float Th[N][3], Ph[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int k = 0; k < N; ++k) Th[k][0] = Th[k][1] = Th[k][2] = 0.0f;
-#pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const float g0 = lds[p2][m][c0][q], g1 = lds[p2][m][c1][q], g2 = lds[p2][m][c2][q];
-#pragma unroll
-        for (int k = 0; k < N; ++k) {
-          Th[k][0] = fmaf(quad.W[k + 1][q], g0, Th[k][0]);
-          Th[k][1] = fmaf(quad.W[k + 1][q], g1, Th[k][1]);
-          Th[k][2] = fmaf(quad.W[k + 1][q], g2, Th[k][2]);
-        }
-        if (Vi) {
-          Ph[0] = fmaf(quad.W[0][q], g0, Ph[0]);
-          Ph[1] = fmaf(quad.W[0][q], g1, Ph[1]);
-          Ph[2] = fmaf(quad.W[0][q], g2, Ph[2]);
+          const float v = (k == 0) ? pc.D1 * ti[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * ti[a] * QF);
+          bred[p2][3 * k + a] -= static_cast<double>(v);
         }
       }
-      // Q0_b = (Theta_0 t_i)_b
-      Ti[bcol * Lp + e] = Th[0][0] * ti[0] + Th[0][1] * ti[1] + Th[0][2] * ti[2];
+      // velocity constants: sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2
 #pragma unroll
-      for (int k = 1; k < N; ++k)
-        store_sym_column(Ti + (3 + 6 * (k - 1)) * Lp + e, Lp, bcol, Th[k][0], Th[k][1], Th[k][2]);
-      if (Vi) {
-        store_sym_column(Vi + e, Lp, bcol, Th[0][0], Th[0][1], Th[0][2]);
-        store_sym_column(Vi + 6 * Lp + e, Lp, bcol, Ph[0], Ph[1], Ph[2]);
+      for (int a = 0; a < 3; ++a) {
+        bred[p2][ROWS + a] += static_cast<double>(TV[a]);
+        bred[p2][ROWS + 3 + a] += static_cast<double>(TF[0][a]);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();  // bred complete
 
-  if (tid < ROWS) {  // the loop ended on a barrier: bred is complete
+  if (tid < ROWS) {
     double s = 0.0;
     for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
     bpart[static_cast<int64_t>(gy) * n + static_cast<int64_t>(ROWS) * i + tid] = s;
@@ -442,10 +458,21 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   }
 }
 
-// (3 waves per SIMD: the FP64 moments of the analytical form otherwise take
-// the register count just past the 168 that allows it)
+// Waves per SIMD the theta kernel asks the register allocator for: 3 (the
+// FP64 moments of the analytical form otherwise take the register count just
+// past the 168 that allows it) where four workgroups' LDS fits a CU's 160 KB,
+// else as many as the LDS allows.
+template <int N, int NQI>
+constexpr int theta_min_waves() {
+  constexpr int Q = 3 * NQI, PAIRS = kPairThreads / Q, NM = N + 1;
+  constexpr int RP = (6 * PAIRS * NM + 15) / 16 * 16;
+  constexpr int bytes = 4 * (RP * Q + NM * RP + NM * Q) + 8 * PAIRS * (3 * N + 6);
+  constexpr int blocks = (160 * 1024) / bytes;
+  return blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1);
+}
+
 template <int N, int NQI, bool ANALYTIC>
-__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_kernel(
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI>(), 8))) void assemble_theta_kernel(
     int M, int64_t Lp, const double* __restrict__ pos, const double* __restrict__ ori, float* __restrict__ T,
     double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart) {
   assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart);
@@ -454,7 +481,7 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3,
 // Serving: workgroup = (suspension, target fiber), all sources in one group,
 // so b and the velocity constants are written directly.
 template <int N, int NQI, bool ANALYTIC>
-__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void assemble_theta_batch_kernel(
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI>(), 8))) void assemble_theta_batch_kernel(
     const SusDev* __restrict__ sus, const int2* __restrict__ targets, PhysConsts pc) {
   const int2 e = targets[blockIdx.x];
   const SusDev& d = sus[e.x];

