@@ -68,7 +68,8 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
                         GMRES_MAX_ITERATIONS=1000, GMRES_TOLERANCE=args.tol, BICGSTAB_TOLERANCE=args.tol)
     matrix_format = args.matrix_format if solver in ITERATIVE else "auto"
     sim = Simulation(params, pos, ori, solver=solver, integration=args.integration, matrix_format=matrix_format,
-                     velocity_operator=args.velocity_operator, refinement_tolerance=1e-10)
+                     velocity_operator=args.velocity_operator, refinement_tolerance=1e-10,
+                     initial_guess=args.initial_guess)
     try:
         for t in range(warmup):
             sim.step(t)
@@ -301,6 +302,8 @@ def main():
     ap.add_argument("--restart", type=int, default=20)
     ap.add_argument("--matrix-format", default="auto", choices=["auto", "dense", "theta"])
     ap.add_argument("--velocity-operator", default="stored", choices=["stored", "pairwise"])
+    ap.add_argument("--initial-guess", default="previous", choices=["previous", "zero", "extrapolated"],
+                    help="iterative solvers' start: the previous solution or 2 x_n - x_{n-1}")
     ap.add_argument("--lu-steps", type=int, default=2, help="timed steps per solver of the precision study")
     ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)

@@ -71,8 +71,14 @@ typedef enum rf_velocity_integration {
 /* Initial guess of the iterative solvers.  The shipped Fortran and ViennaCL
  * start from zero (fortran/solve_system.incl:16-17); the reference's golden
  * vectors for --gmres (tests/reference/100_*_gmres) start each step from the
- * previous step's solution, which also saves iterations. */
-typedef enum rf_initial_guess { RF_GUESS_ZERO = 0, RF_GUESS_PREVIOUS = 1 } rf_initial_guess;
+ * previous step's solution, which also saves iterations.  EXTRAPOLATED starts
+ * from 2 x_n - x_{n-1} once two solutions exist (the previous one before
+ * that): along a trajectory its error is O(dt^2) against O(dt). */
+typedef enum rf_initial_guess {
+  RF_GUESS_ZERO = 0,
+  RF_GUESS_PREVIOUS = 1,
+  RF_GUESS_EXTRAPOLATED = 2
+} rf_initial_guess;
 
 /* Storage of A on the device.  DENSE: n x n FP32 column-major, the
  * reference's layout (src/simulation.cu:64-110).  THETA: each 3x3 block kept

@@ -23,7 +23,7 @@ RF_WARN_NOT_CONVERGED = 100
 SOLVERS = {"gmres": 0, "bicgstab": 1, "direct": 2, "magma": 2, "mixed-bf16": 3, "mixed-fp16": 4, "gmres-ir": 5}
 INTEGRATIONS = {"numerical": 0, "analytical": 1}
 VELOCITY_INTEGRATIONS = {"follow": 0, "numerical": 1, "analytical": 2}
-INITIAL_GUESSES = {"zero": 0, "previous": 1}
+INITIAL_GUESSES = {"zero": 0, "previous": 1, "extrapolated": 2}
 MATRIX_FORMATS = {"auto": 0, "dense": 1, "theta": 2}
 VELOCITY_OPERATORS = {"stored": 0, "pairwise": 1}
 

@@ -374,7 +374,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     const int npm = min(PAIRS, je - j0) * N;  // valid columns of this sweep
     const int sself = (i - j0) * N;           // the self pair's columns, if in this sweep
     const int64_t ecol = static_cast<int64_t>(j0) * N;
-    for (int job = tid; job < nu * NV + PAIRS; job += kPairThreads) {
+    for (int job = tid; job < nu * NV + PAIRS * (N + 1); job += kPairThreads) {
       if (job < nu * NV) {
         const int u = job / NV;
         const int pm0 = 4 * (job - u * NV);
@@ -411,37 +411,47 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         }
         continue;
       }
-      const int p2 = job - nu * NV, j2 = j0 + p2;  // RHS job of source slot p2
+      // RHS job (source slot p2, row block k): k < N forms rows 3k..3k+2 of
+      // b from the projection of G_0 F/2 onto w P_{k+1}; k = N the velocity
+      // constants sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2.
+      const int rj = job - nu * NV;
+      const int p2 = rj / (N + 1), k = rj - p2 * (N + 1), j2 = j0 + p2;
       if (j2 >= je || j2 == i) continue;
-      // Projections of G_0 F/2: TF[k] onto w P_{k+1}, TV onto w
-      float TF[N][3], TV[3];
-#pragma unroll
-      for (int kk = 0; kk < NM; ++kk) {
+      auto proj = [&](int kk, float f[3]) {  // sum_b Theta^(0)_kk[a][b] F_b / 2
         const float* b0 = Pj + kk * RP + p2;
         const float g0 = b0[0 * CB], g1 = b0[1 * CB], g2 = b0[2 * CB];
         const float g3 = b0[3 * CB], g4 = b0[4 * CB], g5 = b0[5 * CB];
-        const float f0 = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
-        const float f1 = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
-        const float f2 = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
-        float* dst3 = kk == 0 ? TV : TF[kk - 1];
-        dst3[0] = f0;
-        dst3[1] = f1;
-        dst3[2] = f2;
-      }
+        f[0] = g0 * pc.Fh[0] + g3 * pc.Fh[1] + g4 * pc.Fh[2];
+        f[1] = g3 * pc.Fh[0] + g1 * pc.Fh[1] + g5 * pc.Fh[2];
+        f[2] = g4 * pc.Fh[0] + g5 * pc.Fh[1] + g2 * pc.Fh[2];
+      };
+      double* br = bred[p2];
+      if (k < N) {
+        float TF[3];
+        proj(k + 1, TF);
+        float gam = 0.0f, eig = 0.0f;
 #pragma unroll
-      for (int k = 0; k < N; ++k) {
-        const float QF = TF[k][0] * ti[0] + TF[k][1] * ti[1] + TF[k][2] * ti[2];
+        for (int kk = 1; kk < N; ++kk)
+          if (k == kk) {
+            gam = pc.gamma[kk];
+            eig = pc.eigen[kk];
+          }
+        const float QF = TF[0] * ti[0] + TF[1] * ti[1] + TF[2] * ti[2];
+        double v[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+          v[a] = static_cast<double>((k == 0) ? pc.D1 * ti[a] * QF : gam * (TF[a] - eig * ti[a] * QF));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) br[3 * k + a] -= v[a];
+      } else {
+        float TV[3], T1[3];
+        proj(0, TV);
+        proj(1, T1);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const float v = (k == 0) ? pc.D1 * ti[a] * QF : pc.gamma[k] * (TF[k][a] - pc.eigen[k] * ti[a] * QF);
-          bred[p2][3 * k + a] -= static_cast<double>(v);
+          br[ROWS + a] += static_cast<double>(TV[a]);
+          br[ROWS + 3 + a] += static_cast<double>(T1[a]);
         }
-      }
-      // velocity constants: sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        bred[p2][ROWS + a] += static_cast<double>(TV[a]);
-        bred[p2][ROWS + 3 + a] += static_cast<double>(TF[0][a]);
       }
     }
   }
@@ -458,21 +468,21 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   }
 }
 
-// Waves per SIMD the theta kernel asks the register allocator for: 3 (the
-// FP64 moments of the analytical form otherwise take the register count just
-// past the 168 that allows it) where four workgroups' LDS fits a CU's 160 KB,
-// else as many as the LDS allows.
-template <int N, int NQI>
+// Waves per SIMD the theta kernel asks the register allocator for: as many
+// as the workgroups whose LDS fits a CU's 160 KB can use, at most 3 for the
+// analytical form (its FP64 moments need more than the 128 registers of 4
+// waves; 3 waves allow 168).
+template <int N, int NQI, bool ANALYTIC>
 constexpr int theta_min_waves() {
   constexpr int Q = 3 * NQI, PAIRS = kPairThreads / Q, NM = N + 1;
   constexpr int RP = (6 * PAIRS * NM + 15) / 16 * 16;
   constexpr int bytes = 4 * (RP * Q + NM * RP + NM * Q) + 8 * PAIRS * (3 * N + 6);
   constexpr int blocks = (160 * 1024) / bytes;
-  return blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1);
+  return (blocks >= 5 && !ANALYTIC) ? 4 : (blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1));
 }
 
 template <int N, int NQI, bool ANALYTIC>
-__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI>(), 8))) void assemble_theta_kernel(
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI, ANALYTIC>(), 8))) void assemble_theta_kernel(
     int M, int64_t Lp, const double* __restrict__ pos, const double* __restrict__ ori, float* __restrict__ T,
     double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart) {
   assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart);
@@ -481,7 +491,7 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(th
 // Serving: workgroup = (suspension, target fiber), all sources in one group,
 // so b and the velocity constants are written directly.
 template <int N, int NQI, bool ANALYTIC>
-__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI>(), 8))) void assemble_theta_batch_kernel(
+__global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI, ANALYTIC>(), 8))) void assemble_theta_batch_kernel(
     const SusDev* __restrict__ sus, const int2* __restrict__ targets, PhysConsts pc) {
   const int2 e = targets[blockIdx.x];
   const SusDev& d = sus[e.x];

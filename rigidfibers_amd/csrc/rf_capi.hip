@@ -72,6 +72,8 @@ struct rf_simulation {
   bool have_history = false;
   bool have_next = false;
   bool have_solution = false;  // x holds a previous solution (initial guess)
+  bool have_previous = false;  // xprev holds the solution before it
+  double* xprev = nullptr;     // RF_GUESS_EXTRAPOLATED only
 
   float* A = nullptr;     // dense system (direct solvers / RF_MATRIX_DENSE)
   bool theta = false;     // system held in theta form (T) instead of A
@@ -155,7 +157,8 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
   if (c->matrix_format < RF_MATRIX_AUTO || c->matrix_format > RF_MATRIX_THETA)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown matrix format");
-  if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS)
+  if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS &&
+      c->initial_guess != RF_GUESS_EXTRAPOLATED)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
   if (c->velocity_integration < RF_VELOCITY_FOLLOW || c->velocity_integration > RF_VELOCITY_ANALYTICAL)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity integration");
@@ -361,8 +364,29 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
   return RF_OK;
 }
 
+// Initial guess of the iterative solvers in s->x, which holds the previous
+// solution x_n when there is one: PREVIOUS keeps it, EXTRAPOLATED replaces it
+// by 2 x_n - x_{n-1} once x_{n-1} is known and keeps x_n for the next solve.
+// *warm says whether x is a guess (else the solver starts from zero).
+rf_status initial_guess(rf_simulation* s, bool* warm) {
+  *warm = s->cfg.initial_guess != RF_GUESS_ZERO && s->have_solution;
+  if (*warm && s->cfg.initial_guess == RF_GUESS_EXTRAPOLATED) {
+    const int64_t n = s->n;
+    if (s->have_previous) {
+      RF_HIP(rf::axpby(n, 1.0, s->x, -1.0, s->xprev, s->stream));  // d = x_n - x_{n-1}
+      RF_HIP(rf::axpby(n, 1.0, s->xprev, 1.0, s->x, s->stream));   // x = x_n + d
+      RF_HIP(rf::axpby(n, 1.0, s->x, -1.0, s->xprev, s->stream));  // xprev = x - d = x_n
+    } else {
+      RF_HIP(hipMemcpyAsync(s->xprev, s->x, n * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    }
+    s->have_previous = true;
+  }
+  return RF_OK;
+}
+
 rf_status solve_gmres(rf_simulation* s) {
-  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  bool warm = false;
+  RF_TRY(initial_guess(s, &warm));
   return gmres_core(s, s->b, s->x, warm, s->cfg.gmres_tolerance, s->cfg.gmres_max_iterations, false, &s->stats);
 }
 
@@ -379,7 +403,8 @@ rf_status solve_gmres_ir(rf_simulation* s) {
   double* d = s->work + n;
   double bnorm = 0.0;
   RF_TRY(device_norm(s, s->b, &bnorm));
-  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  bool warm = false;
+  RF_TRY(initial_guess(s, &warm));
   if (!warm || bnorm == 0.0) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
   s->stats = rf_solver_stats{0, 0, 0.0};
   if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
@@ -422,7 +447,8 @@ rf_status solve_bicgstab(rf_simulation* s) {
   double* t = s->work + 5 * n;
   double bnorm = 0.0;
   RF_TRY(device_norm(s, s->b, &bnorm));
-  const bool warm = s->cfg.initial_guess == RF_GUESS_PREVIOUS && s->have_solution;
+  bool warm = false;
+  RF_TRY(initial_guess(s, &warm));
   s->stats = rf_solver_stats{0, 0, 0.0};
   if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
   if (bnorm == 0.0 || !warm) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
@@ -623,6 +649,7 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
   }
   if (st == RF_OK) st = s->alloc(&s->b, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->x, static_cast<size_t>(n));
+  if (st == RF_OK && cfg->initial_guess == RF_GUESS_EXTRAPOLATED) st = s->alloc(&s->xprev, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->bpart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
   if (st == RF_OK) st = s->alloc(&s->vpart, static_cast<size_t>(rf::max_pair_groups(M)) * M * 6);
   if (st == RF_OK) st = s->alloc(&s->wf, static_cast<size_t>(M) * 3 * 3 * s->NQI);

@@ -72,7 +72,7 @@ def _add_common(p, platform, command):
         dims.add_argument("--D2", action="store_true")
         dims.add_argument("--D3", action="store_true")
     p.add_argument("--velocity-integration", choices=["follow", "numerical", "analytical"], default="follow")
-    p.add_argument("--initial-guess", choices=["previous", "zero"], default="previous")
+    p.add_argument("--initial-guess", choices=["previous", "zero", "extrapolated"], default="previous")
     p.add_argument("--output-dir", default=".")
     if command == "validate":
         p.add_argument("--reference-dir", default=None,

@@ -333,6 +333,28 @@ def test_ten_steps_against_oracle(oracle64):
     assert ep < 1e-5 and eo < 1e-5
 
 
+@pytest.mark.parametrize("solver", ["gmres", "bicgstab", "gmres-ir"])
+def test_extrapolated_guess(oracle64, solver):
+    """Starting each solve from 2 x_n - x_{n-1}: every step converges, and
+    after 10 steps the state is the one the oracle's direct solves give."""
+    from oracle.pyoracle import make_params
+
+    M, pos, ori = ref100()
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-8
+    params.BICGSTAB_TOLERANCE = 1e-8
+    with _sim(params, pos, ori, solver=solver, initial_guess="extrapolated") as sim:
+        for t in range(10):
+            sim.step(t)
+            assert sim.solver_stats()["converged"]
+        p, o = sim.state()
+    p_ref, o_ref = oracle64.simulate(make_params(M), pos, ori, 10, solver="direct")
+    ep, eo = rel_err(p, p_ref), rel_err(o, o_ref)
+    print(f"{solver} extrapolated, 10 steps: pos {ep:.1e} ori {eo:.1e}")
+    assert ep < 1e-5 and eo < 1e-5
+
+
 @pytest.mark.parametrize("analytical", [False, True])
 def test_theta_form_matches_dense(analytical):
     """The theta-form system (the iterative solvers' storage) expands to the
