@@ -281,9 +281,11 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 //   phase 1  lane (p, q_i) writes G_m(q_i) of source slot p into LDS as row
 //            r = c*CB + (m == 0 ? p : PAIRS + p*N + m-1) of a GEMM operand
 //            (c = symmetric component; the mode-0 rows feed the RHS).
-//   phase 2  C[r][kk] = sum_q G[r][q] W[kk][q] (kk = 0..N: the projections onto
-//            w P_kk) on the matrix cores, 16 rows x 16 projections per
-//            v_mfma_f32_16x16x4f32 tile (lane group g takes q in [g Q/4, (g+1) Q/4)).
+//   phase 2  lane r forms C[r][kk] = sum_q W[kk][q] G[r][q] (kk = 0..N: the
+//            projections onto w P_kk) for all kk from the mirror sums
+//            E = G(q) + G(-q) and differences O = G(q) - G(-q): the weights
+//            satisfy W[kk][-q] = (-1)^kk W[kk][q] exactly, so even kk take E,
+//            odd kk take O, and the table entries are immediate operands.
 //   phase 3  each lane writes 4 consecutive (source, mode) columns of one
 //            component array (16-byte stores); the k_i = 0 components
 //            (Theta_0 t_i) and the RHS of each source slot are formed here.
@@ -301,16 +303,12 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   constexpr int NR = ROWS + 6;  // RHS rows + the two velocity constants
   constexpr int PN = PAIRS * N;  // (source, mode) columns of one sweep
   constexpr int CB = PAIRS + PN;  // GEMM rows per component
-  constexpr int TILES = (6 * CB + 15) / 16;
-  constexpr int RP = 16 * TILES;
-  constexpr int KQ = Q / 4;  // MFMA k-steps per tile
+  constexpr int RP = 6 * CB;  // GEMM rows
   constexpr int NV = PN / 4;  // float4 chunks per component and sweep
-  constexpr int WAVES = kPairThreads / 64;
-  static_assert(kPairThreads % Q == 0 && Q % 4 == 0 && PAIRS % 4 == 0 && NM <= 16, "tile shapes");
+  static_assert(kPairThreads % Q == 0 && Q % 2 == 0 && PAIRS % 4 == 0, "sweep shapes");
 
   __shared__ __align__(16) float G[RP * Q];
   __shared__ __align__(16) float Pj[NM * RP];
-  __shared__ float Wl[NM * Q];
   __shared__ double bred[PAIRS][NR];
 
   const int tid = threadIdx.x;
@@ -330,23 +328,9 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
 
   // RHS (and velocity-constant) sums of each source slot, kept in LDS
   for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
-  if (tid == 0) {
-#pragma unroll
-    for (int m = 0; m < NM; ++m)
-#pragma unroll
-      for (int q = 0; q < Q; ++q) Wl[m * Q + q] = quad.W[m][q];
-  }
   __syncthreads();
 
-  const int lane = tid & 63, col = lane & 15, g = lane >> 4;
-  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nu = NC + (Vi ? 12 : 0);  // component arrays written per sweep
-  float wb[KQ];                       // this lane's B operand: W[col][g KQ + s]
-  {
-    const int wrow = min(col, NM - 1) * Q + g * KQ;
-#pragma unroll
-    for (int s = 0; s < KQ; ++s) wb[s] = col < NM ? Wl[wrow + s] : 0.0f;
-  }
 
   for (int j0 = jb; j0 < je; j0 += PAIRS) {
     const int j = j0 + p;
@@ -360,14 +344,21 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         for (int c = 0; c < 6; ++c) G[(c * CB + (m == 0 ? p : PAIRS + p * N + m - 1)) * Q + qi] = acc[m][c];
     }
     __syncthreads();
-    {
-      for (int t = wv; t < TILES; t += WAVES) {
-        const float* a = G + (16 * t + col) * Q + g * KQ;
-        floatx4 c4 = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int r = tid; r < RP; r += kPairThreads) {
+      const float* gr = G + r * Q;
+      float E[Q / 2], O[Q / 2];
 #pragma unroll
-        for (int s = 0; s < KQ; ++s) c4 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], wb[s], c4, 0, 0, 0);
-        // c4[e] = C[16 t + 4 g + e][col]
-        if (col < NM) *reinterpret_cast<floatx4*>(Pj + col * RP + 16 * t + 4 * g) = c4;
+      for (int h = 0; h < Q / 2; ++h) {
+        const float x = gr[h], y = gr[Q - 1 - h];
+        E[h] = x + y;
+        O[h] = x - y;
+      }
+#pragma unroll
+      for (int kk = 0; kk < NM; ++kk) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int h = 0; h < Q / 2; ++h) acc = fmaf(quad.W[kk][h], (kk & 1) ? O[h] : E[h], acc);
+        Pj[kk * RP + r] = acc;
       }
     }
     __syncthreads();
@@ -475,8 +466,8 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
 template <int N, int NQI, bool ANALYTIC>
 constexpr int theta_min_waves() {
   constexpr int Q = 3 * NQI, PAIRS = kPairThreads / Q, NM = N + 1;
-  constexpr int RP = (6 * PAIRS * NM + 15) / 16 * 16;
-  constexpr int bytes = 4 * (RP * Q + NM * RP + NM * Q) + 8 * PAIRS * (3 * N + 6);
+  constexpr int RP = 6 * PAIRS * NM;
+  constexpr int bytes = 4 * (RP * Q + NM * RP) + 8 * PAIRS * (3 * N + 6);
   constexpr int blocks = (160 * 1024) / bytes;
   return (blocks >= 5 && !ANALYTIC) ? 4 : (blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1));
 }
