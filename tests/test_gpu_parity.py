@@ -333,6 +333,54 @@ def test_ten_steps_against_oracle(oracle64):
     assert ep < 1e-5 and eo < 1e-5
 
 
+def test_cube_m100_ten_steps(oracle64):
+    """M = 100 centres uniform in [-5, 5]^3 (min distance 0.4), numerical
+    integrals, GMRES to 1e-6, 10 steps: the state against the FP64 oracle's
+    direct solves."""
+    from oracle.pyoracle import make_params
+    from rigidfibers_amd.scenes import cube
+
+    M = 100
+    pos, ori = cube(M, 10.0, min_dist=0.4, seed=100)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-6
+    with _sim(params, pos, ori, solver="gmres", integration="numerical") as sim:
+        for t in range(10):
+            sim.step(t)
+            assert sim.solver_stats()["converged"]
+        p, o = sim.state()
+    p_ref, o_ref = oracle64.simulate(make_params(M), pos, ori, 10, solver="direct")
+    ep, eo = rel_err(p, p_ref), rel_err(o, o_ref)
+    print(f"cube M=100, 10 steps: pos {ep:.1e} ori {eo:.1e}")
+    assert ep < 1e-5 and eo < 1e-5
+
+
+@pytest.mark.parametrize("analytical", [False, True])
+def test_cube_m2000_gmres_and_lu_trajectories(analytical):
+    """M = 2000 in a cube of side 40 (n = 30 000), 20 steps: GMRES to 1e-8 on
+    the theta form and the FP32 LU with FP64 refinement on the dense matrix
+    follow the same trajectory."""
+    from rigidfibers_amd.scenes import cube
+
+    M = 2000
+    pos, ori = cube(M, 40.0, min_dist=0.4, seed=2020)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-8
+    integration = "analytical" if analytical else "numerical"
+    out = {}
+    for solver in ("gmres", "direct"):
+        with _sim(params, pos, ori, solver=solver, integration=integration, refinement_tolerance=1e-10) as sim:
+            for t in range(20):
+                sim.step(t)
+                assert sim.solver_stats()["converged"]
+            out[solver] = sim.state()
+    ep, eo = rel_err(out["gmres"][0], out["direct"][0]), rel_err(out["gmres"][1], out["direct"][1])
+    print(f"cube M=2000 {integration}, 20 steps, GMRES vs LU: pos {ep:.1e} ori {eo:.1e}")
+    assert ep < 1e-7 and eo < 1e-6
+
+
 @pytest.mark.parametrize("solver", ["gmres", "bicgstab", "gmres-ir"])
 def test_extrapolated_guess(oracle64, solver):
     """Starting each solve from 2 x_n - x_{n-1}: every step converges, and
