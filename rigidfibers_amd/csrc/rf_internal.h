@@ -107,6 +107,25 @@ struct SusIter {
   double* hout;                    // 3 (kdim + 1): <V_j, w> twice, ||w||^2
 };
 
+// Device-side GMRES(m) state of one suspension in the serving lockstep: the
+// control step after each iteration reads the iteration's Hessenberg column
+// (SusIter::hout), advances this state and writes the next SusIter.
+struct SusCtl {
+  int phase;   // 0: true residual next, 1: Arnoldi step next, 2: done
+  int k;       // Arnoldi column of the current cycle
+  int it;      // Arnoldi steps so far
+  int converged;
+  int nonfinite;
+  double bnorm, res;
+  double* H;   // (m + 1) x m Hessenberg matrix, column-major
+  double* cs;  // m Givens cosines
+  double* sn;  // m Givens sines
+  double* g;   // m + 1 rotated right-hand side
+  double* y;   // m coefficients of x += V y
+};
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, double tol, int maxit, const SusDev* sus, SusIter* it,
+                                      SusCtl* ctl, int* active, hipStream_t s);
+
 hipError_t launch_assemble_theta_batch(int N, int NQI, bool analytical, int ntargets, const SusDev* sus,
                                        const int2* targets, const PhysConsts& pc, hipStream_t s);
 hipError_t launch_gemv_theta_batch(int N, int ntargets, const SusDev* sus, const SusIter* it, const int2* targets,

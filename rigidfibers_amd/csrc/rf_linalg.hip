@@ -446,6 +446,113 @@ __global__ __launch_bounds__(kVecBlock) void orth_batch_kernel(int phase, int kd
   c.w[r] -= s;
 }
 
+// Control step of the serving lockstep (one lane per suspension), the same
+// GMRES(m) as the single-simulation solver: true residual at every restart,
+// classical Gram-Schmidt twice (the two passes' coefficients summed), Givens
+// rotations, stop on ||r|| <= tol ||b||.  Writes the next iteration's
+// operations into it[q] and whether it has any into active[q].
+__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int ns, int m, int kdim, double tol, int maxit,
+                                                                 const SusDev* __restrict__ sus,
+                                                                 SusIter* __restrict__ it, SusCtl* __restrict__ ctl,
+                                                                 int* __restrict__ active) {
+  const int q = blockIdx.x * 64 + threadIdx.x;
+  if (q >= ns) return;
+  const SusDev& d = sus[q];
+  SusIter& c = it[q];
+  SusCtl& C = ctl[q];
+  const double* h = c.hout;
+  int ycount = 0;
+  double* svec = nullptr;
+  double sval = 1.0;
+  if (C.phase == 0) {  // h holds ||b - A x||^2
+    const double beta = sqrt(h[2 * (kdim + 1)]);
+    C.res = beta;
+    if (!isfinite(beta)) {
+      C.nonfinite = 1;
+      C.phase = 2;
+    } else if (beta / C.bnorm <= tol) {
+      C.converged = 1;
+      C.phase = 2;
+    } else if (C.it >= maxit) {
+      C.phase = 2;
+    } else {
+      svec = d.V;
+      sval = 1.0 / beta;
+      for (int i = 0; i <= m; ++i) C.g[i] = 0.0;
+      C.g[0] = beta;
+      C.k = 0;
+      C.phase = 1;
+    }
+  } else if (C.phase == 1) {  // h holds <V_j, w> of both passes and ||w||^2
+    const int k = C.k;
+    ++C.it;
+    const double hn = sqrt(h[2 * (kdim + 1)]);
+    if (!isfinite(hn)) {
+      C.nonfinite = 1;
+      C.phase = 2;
+    } else {
+      double* Hk = C.H + static_cast<int64_t>(k) * (m + 1);
+      for (int i = 0; i <= k; ++i) Hk[i] = h[i] + h[(kdim + 1) + i];
+      Hk[k + 1] = hn;
+      for (int i = 0; i < k; ++i) {
+        const double a0 = Hk[i], a1 = Hk[i + 1];
+        Hk[i] = C.cs[i] * a0 + C.sn[i] * a1;
+        Hk[i + 1] = -C.sn[i] * a0 + C.cs[i] * a1;
+      }
+      const double a0 = Hk[k], a1 = Hk[k + 1];
+      const double rr = hypot(a0, a1);
+      C.cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
+      C.sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
+      Hk[k] = rr;
+      Hk[k + 1] = 0.0;
+      C.g[k + 1] = -C.sn[k] * C.g[k];
+      C.g[k] = C.cs[k] * C.g[k];
+      C.res = fabs(C.g[k + 1]);
+      if (C.res / C.bnorm <= tol || hn == 0.0 || k + 1 == m || C.it >= maxit) {
+        const int ku = k + 1;  // y = H^-1 g, then x += V y and the true residual
+        for (int i = ku - 1; i >= 0; --i) {
+          double acc = C.g[i];
+          for (int j = i + 1; j < ku; ++j) acc -= C.H[i + static_cast<int64_t>(j) * (m + 1)] * C.y[j];
+          C.y[i] = acc / C.H[i + static_cast<int64_t>(i) * (m + 1)];
+        }
+        ycount = ku;
+        C.phase = 0;
+      } else {
+        svec = d.V + static_cast<int64_t>(k + 1) * d.n;
+        sval = 1.0 / hn;
+        C.k = k + 1;
+      }
+    }
+  }
+  SusIter nx{};
+  nx.hout = c.hout;
+  nx.ycount = ycount;
+  nx.ycoef = C.y;
+  nx.svec = svec;
+  nx.sval = sval;
+  if (C.phase == 0) {
+    nx.gx = d.x;
+    nx.gy = d.V;
+    nx.gmode = 1;
+    nx.w = d.V;
+  } else if (C.phase == 1) {
+    nx.gx = d.V + static_cast<int64_t>(C.k) * d.n;
+    nx.gy = d.V + static_cast<int64_t>(C.k + 1) * d.n;
+    nx.w = nx.gy;
+    nx.K = C.k + 1;
+  }
+  c = nx;
+  active[q] = (C.phase != 2 || ycount > 0 || svec != nullptr) ? 1 : 0;
+}
+
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, double tol, int maxit, const SusDev* sus, SusIter* it,
+                                      SusCtl* ctl, int* active, hipStream_t s) {
+  if (ns <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gmres_control_batch_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, ns, m, kdim, tol, maxit, sus,
+                     it, ctl, active);
+  return hipGetLastError();
+}
+
 hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it, const int2* vblocks,
                                   hipStream_t s) {
   if (nvb <= 0) return hipSuccess;

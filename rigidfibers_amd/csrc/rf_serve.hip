@@ -55,8 +55,6 @@ struct Request {
   bool done = false;
 };
 
-enum Phase { kResidual = 0, kArnoldi = 1, kDone = 2 };
-
 struct Suspension {
   Request* req = nullptr;
   int M = 0, n = 0;
@@ -76,16 +74,6 @@ struct Suspension {
   double* x = nullptr;
   double* V = nullptr;
   int steps_done = 0;
-  // GMRES state of the current step
-  Phase phase = kDone;
-  int k = 0, it = 0;
-  double bnorm = 0.0, res = 0.0;
-  bool converged = false;
-  std::vector<double> H, cs, sn, g, y;
-  // operations for the next lockstep iteration
-  int zero_x = 0, ycount = 0;
-  double* svec = nullptr;
-  double sval = 1.0;
   rf_status status = RF_OK;
 };
 
@@ -159,13 +147,17 @@ struct rf_server {
   double* d_part = nullptr;
   double* d_hout = nullptr;
   double* h_hout = nullptr;
-  double* d_ycoef = nullptr;
-  double* h_ycoef = nullptr;
   size_t c_dsus = 0, c_hsus = 0, c_dit = 0, c_hit = 0, c_dt = 0, c_ht = 0, c_dv = 0, c_hv = 0, c_df = 0, c_hf = 0,
-         c_part = 0, c_dh = 0, c_hh = 0, c_dy = 0, c_hy = 0;
+         c_part = 0, c_dh = 0, c_hh = 0;
   int ntargets = 0, nvb = 0, nfb = 0;
-  hipGraphExec_t gexec = nullptr;  // captured GMRES iteration of the admitted set
-  std::array<int64_t, 8> gkey{};   // the launch parameters it was captured with
+  rf::SusCtl* d_ctl = nullptr;
+  rf::SusCtl* h_ctl = nullptr;
+  double* d_ctlmem = nullptr;  // Hessenberg, Givens and y of every suspension
+  int* d_act = nullptr;
+  int* h_act = nullptr;
+  size_t c_dctl = 0, c_hctl = 0, c_cmem = 0, c_dact = 0, c_hact = 0;
+  hipGraphExec_t gexec = nullptr;  // captured round of GMRES iterations of the admitted set
+  std::array<int64_t, 10> gkey{};  // the launch parameters it was captured with
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 };
@@ -230,11 +222,6 @@ rf_status admit(rf_server* S, Request* r, Suspension** out) {
   r->pos0.shrink_to_fit();
   r->ori0.clear();
   r->ori0.shrink_to_fit();
-  s->H.assign(static_cast<size_t>(S->m + 1) * S->m, 0.0);
-  s->cs.assign(S->m, 0.0);
-  s->sn.assign(S->m, 0.0);
-  s->g.assign(S->m + 1, 0.0);
-  s->y.assign(S->m, 0.0);
   *out = s;
   return RF_OK;
 }
@@ -256,8 +243,10 @@ bool build_tables(rf_server* S) {
       !grow_device(&S->d_part, &S->c_part, static_cast<size_t>(nv) * S->kdim) ||
       !grow_host(&S->h_hout, &S->c_hh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
       !grow_device(&S->d_hout, &S->c_dh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
-      !grow_host(&S->h_ycoef, &S->c_hy, static_cast<size_t>(ns) * S->kdim) ||
-      !grow_device(&S->d_ycoef, &S->c_dy, static_cast<size_t>(ns) * S->kdim))
+      !grow_host(&S->h_ctl, &S->c_hctl, ns) || !grow_device(&S->d_ctl, &S->c_dctl, ns) ||
+      !grow_device(&S->d_ctlmem, &S->c_cmem,
+                   static_cast<size_t>(ns) * ((S->m + 1) * static_cast<size_t>(S->m) + 4 * S->m + 1)) ||
+      !grow_host(&S->h_act, &S->c_hact, ns) || !grow_device(&S->d_act, &S->c_dact, ns))
     return false;
   int ti = 0, vi = 0, fi = 0;
   for (int q = 0; q < ns; ++q) {
@@ -299,9 +288,13 @@ bool build_tables(rf_server* S) {
          hipMemcpyAsync(S->d_fblocks, S->h_fblocks, nf * sizeof(int2), hipMemcpyHostToDevice, st) == hipSuccess;
 }
 
-// The kernels of one lockstep iteration over the admitted set (fixed shapes
-// within a tick: the Gram-Schmidt launches cover all m basis vectors and the
-// workgroups past a suspension's K exit).
+// Lockstep iterations per round: a round is replayed as one graph and ends
+// with the round's only device->host copy (the suspensions still working).
+constexpr int kRound = 4;
+
+// The kernels of one lockstep iteration over the admitted set and its control
+// step (fixed shapes within a tick: the Gram-Schmidt launches cover all m
+// basis vectors and the workgroups past a suspension's K exit).
 bool iteration_launches(rf_server* S) {
   const int ns = static_cast<int>(S->active.size());
   hipStream_t st = S->stream;
@@ -314,8 +307,17 @@ bool iteration_launches(rf_server* S) {
       return false;
     if (rf::launch_orth_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
   }
-  return rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) ==
-         hipSuccess;
+  if (rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) != hipSuccess)
+    return false;
+  return rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->cfg.base.gmres_tolerance,
+                                        S->cfg.base.gmres_max_iterations, S->d_sus, S->d_it, S->d_ctl, S->d_act,
+                                        st) == hipSuccess;
+}
+
+bool round_launches(rf_server* S) {
+  for (int r = 0; r < kRound; ++r)
+    if (!iteration_launches(S)) return false;
+  return true;
 }
 
 void release_iteration(rf_server* S) {
@@ -323,25 +325,28 @@ void release_iteration(rf_server* S) {
   S->gexec = nullptr;
 }
 
-// Captures iteration_launches as a graph.  Its launch parameters are the
-// table sizes and device addresses (the tables' contents are copied in before
-// every replay), so the graph is kept across ticks until the admitted set
-// changes them.  Kernels only: no memset or copy nodes.
+// Captures round_launches as a graph.  Its launch parameters are the table
+// sizes and device addresses (the tables' contents are written before a
+// round or by the control steps inside it), so the graph is kept across
+// ticks until the admitted set changes them.  Kernels only: no memset or
+// copy nodes.
 bool ensure_iteration_graph(rf_server* S) {
-  const std::array<int64_t, 8> key = {static_cast<int64_t>(S->active.size()),
-                                      S->nvb,
-                                      S->ntargets,
-                                      reinterpret_cast<int64_t>(S->d_sus),
-                                      reinterpret_cast<int64_t>(S->d_it),
-                                      reinterpret_cast<int64_t>(S->d_targets),
-                                      reinterpret_cast<int64_t>(S->d_vblocks),
-                                      reinterpret_cast<int64_t>(S->d_part)};
+  const std::array<int64_t, 10> key = {static_cast<int64_t>(S->active.size()),
+                                       S->nvb,
+                                       S->ntargets,
+                                       reinterpret_cast<int64_t>(S->d_sus),
+                                       reinterpret_cast<int64_t>(S->d_it),
+                                       reinterpret_cast<int64_t>(S->d_targets),
+                                       reinterpret_cast<int64_t>(S->d_vblocks),
+                                       reinterpret_cast<int64_t>(S->d_part),
+                                       reinterpret_cast<int64_t>(S->d_ctl),
+                                       reinterpret_cast<int64_t>(S->d_act)};
   if (S->gexec && key == S->gkey) return true;
   release_iteration(S);
   hipStream_t st = S->stream;
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
-  const bool ok = iteration_launches(S);
+  const bool ok = round_launches(S);
   if (hipStreamEndCapture(st, &g) != hipSuccess || !ok) {
     if (g) (void)hipGraphDestroy(g);
     return false;
@@ -356,37 +361,35 @@ bool ensure_iteration_graph(rf_server* S) {
   return true;
 }
 
-// One lockstep iteration: upload the per-suspension descriptors, run the
-// launch sequence (graph replay or eager), bring every Hessenberg column back.
-bool lockstep_iteration(rf_server* S) {
+// One round of lockstep iterations (graph replay or eager); *any = whether a
+// suspension still has work afterwards.
+bool lockstep_round(rf_server* S, bool* any) {
   const int ns = static_cast<int>(S->active.size());
-  const int hs = 3 * (S->kdim + 1);
   hipStream_t st = S->stream;
-  if (hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(S->d_ycoef, S->h_ycoef, static_cast<size_t>(ns) * S->kdim * sizeof(double),
-                     hipMemcpyHostToDevice, st) != hipSuccess)
-    return false;
   if (S->gexec) {
     if (hipGraphLaunch(S->gexec, st) != hipSuccess) return false;
-  } else if (!iteration_launches(S)) {
+  } else if (!round_launches(S)) {
     return false;
   }
-  if (hipMemcpyAsync(S->h_hout, S->d_hout, static_cast<size_t>(ns) * hs * sizeof(double), hipMemcpyDeviceToHost,
-                     st) != hipSuccess)
+  if (hipMemcpyAsync(S->h_act, S->d_act, ns * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
     return false;
-  return hipStreamSynchronize(st) == hipSuccess;
+  *any = false;
+  for (int q = 0; q < ns; ++q) *any = *any || S->h_act[q] != 0;
+  return true;
 }
 
 // GMRES(m) for every admitted suspension (the same algorithm as the
 // single-simulation solver: true residual at each restart, classical
-// Gram-Schmidt twice, Givens rotations on the host).
+// Gram-Schmidt twice, Givens rotations), its control on the device: the host
+// sets up ||b|| and the first operations, then replays rounds of iterations
+// until no suspension has work left.
 bool solve_lockstep(rf_server* S) {
   const int ns = static_cast<int>(S->active.size());
   const int hs = 3 * (S->kdim + 1);
   const int m = S->m;
-  const double tol = S->cfg.base.gmres_tolerance;
   const int maxit = S->cfg.base.gmres_max_iterations;
-  const bool warm_ok = S->cfg.base.initial_guess == RF_GUESS_PREVIOUS;
+  const bool warm_ok = S->cfg.base.initial_guess != RF_GUESS_ZERO;
   // ||b|| of every suspension
   for (int q = 0; q < ns; ++q) {
     Suspension* s = S->active[q];
@@ -395,7 +398,6 @@ bool solve_lockstep(rf_server* S) {
     c.w = s->b;
     c.K = 0;
     c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
-    c.ycoef = S->d_ycoef + static_cast<int64_t>(q) * S->kdim;
   }
   {
     hipStream_t st = S->stream;
@@ -407,131 +409,65 @@ bool solve_lockstep(rf_server* S) {
         hipStreamSynchronize(st) != hipSuccess)
       return false;
   }
+  // initial control state and first operations: x = 0 unless warm, then the
+  // true residual
+  const int64_t per = static_cast<int64_t>(m + 1) * m + 4 * m + 1;
   for (int q = 0; q < ns; ++q) {
     Suspension* s = S->active[q];
-    s->bnorm = std::sqrt(S->h_hout[static_cast<int64_t>(q) * hs + 2 * (S->kdim + 1)]);
-    s->it = 0;
-    s->converged = false;
-    s->ycount = 0;
-    s->svec = nullptr;
-    s->zero_x = !(warm_ok && s->have_solution);
-    if (!std::isfinite(s->bnorm)) {
-      s->status = RF_ERR_NONFINITE;
-      s->phase = kDone;
-    } else if (s->bnorm == 0.0) {
-      s->zero_x = 1;
-      s->converged = true;
-      s->phase = kDone;
+    rf::SusCtl& C = S->h_ctl[q];
+    C = rf::SusCtl{};
+    double* mem = S->d_ctlmem + q * per;
+    C.H = mem;
+    C.cs = C.H + static_cast<int64_t>(m + 1) * m;
+    C.sn = C.cs + m;
+    C.g = C.sn + m;
+    C.y = C.g + m + 1;
+    C.bnorm = std::sqrt(S->h_hout[static_cast<int64_t>(q) * hs + 2 * (S->kdim + 1)]);
+    rf::SusIter& c = S->h_it[q];
+    c = rf::SusIter{};
+    c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
+    c.ycoef = C.y;
+    c.zero_x = !(warm_ok && s->have_solution);
+    if (!std::isfinite(C.bnorm)) {
+      C.nonfinite = 1;
+      C.phase = 2;
+    } else if (C.bnorm == 0.0) {
+      c.zero_x = 1;
+      C.converged = 1;
+      C.phase = 2;
     } else {
-      s->phase = kResidual;
+      C.phase = 0;
+      c.gx = s->x;
+      c.gy = s->V;
+      c.gmode = 1;
+      c.w = s->V;
     }
   }
+  hipStream_t st = S->stream;
+  if (hipMemcpyAsync(S->d_ctl, S->h_ctl, ns * sizeof(rf::SusCtl), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess)
+    return false;
   if (!S->cfg.eager && !ensure_iteration_graph(S)) return false;
-  while (true) {
-    bool work = false;
-    int kmax = 0;
-    for (int q = 0; q < ns; ++q) {
-      Suspension* s = S->active[q];
-      rf::SusIter& c = S->h_it[q];
-      c = rf::SusIter{};
-      c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
-      c.zero_x = s->zero_x;
-      c.ycount = s->ycount;
-      c.ycoef = S->d_ycoef + static_cast<int64_t>(q) * S->kdim;
-      for (int i = 0; i < s->ycount; ++i) S->h_ycoef[static_cast<int64_t>(q) * S->kdim + i] = s->y[i];
-      c.svec = s->svec;
-      c.sval = s->sval;
-      if (s->phase == kResidual) {
-        c.gx = s->x;
-        c.gy = s->V;
-        c.gmode = 1;
-        c.w = s->V;
-        c.K = 0;
-      } else if (s->phase == kArnoldi) {
-        c.gx = s->V + static_cast<int64_t>(s->k) * s->n;
-        c.gy = s->V + static_cast<int64_t>(s->k + 1) * s->n;
-        c.gmode = 0;
-        c.w = c.gy;
-        c.K = s->k + 1;
-        kmax = std::max(kmax, c.K);
-      }
-      if (s->phase != kDone || c.zero_x || c.ycount || c.svec) work = true;
-    }
-    if (!work) break;
-    (void)kmax;
-    if (!lockstep_iteration(S)) return false;
-    for (int q = 0; q < ns; ++q) {
-      Suspension* s = S->active[q];
-      const double* h = S->h_hout + static_cast<int64_t>(q) * hs;
-      s->zero_x = 0;
-      s->ycount = 0;
-      s->svec = nullptr;
-      if (s->phase == kResidual) {
-        const double beta = std::sqrt(h[2 * (S->kdim + 1)]);
-        s->res = beta;
-        if (!std::isfinite(beta)) {
-          s->status = RF_ERR_NONFINITE;
-          s->phase = kDone;
-        } else if (beta / s->bnorm <= tol) {
-          s->converged = true;
-          s->phase = kDone;
-        } else if (s->it >= maxit) {
-          s->phase = kDone;
-        } else {
-          s->svec = s->V;
-          s->sval = 1.0 / beta;
-          std::fill(s->g.begin(), s->g.end(), 0.0);
-          s->g[0] = beta;
-          s->k = 0;
-          s->phase = kArnoldi;
-        }
-      } else if (s->phase == kArnoldi) {
-        const int k = s->k;
-        ++s->it;
-        const double hn = std::sqrt(h[2 * (S->kdim + 1)]);
-        if (!std::isfinite(hn)) {
-          s->status = RF_ERR_NONFINITE;
-          s->phase = kDone;
-          continue;
-        }
-        double* Hk = s->H.data() + static_cast<size_t>(k) * (m + 1);
-        for (int i = 0; i <= k; ++i) Hk[i] = h[i] + h[(S->kdim + 1) + i];
-        Hk[k + 1] = hn;
-        for (int i = 0; i < k; ++i) {
-          const double a0 = Hk[i], a1 = Hk[i + 1];
-          Hk[i] = s->cs[i] * a0 + s->sn[i] * a1;
-          Hk[i + 1] = -s->sn[i] * a0 + s->cs[i] * a1;
-        }
-        const double a0 = Hk[k], a1 = Hk[k + 1];
-        const double rr = std::hypot(a0, a1);
-        s->cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
-        s->sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
-        Hk[k] = rr;
-        Hk[k + 1] = 0.0;
-        s->g[k + 1] = -s->sn[k] * s->g[k];
-        s->g[k] = s->cs[k] * s->g[k];
-        s->res = std::fabs(s->g[k + 1]);
-        if (s->res / s->bnorm <= tol || hn == 0.0 || k + 1 == m || s->it >= maxit) {
-          const int ku = k + 1;
-          for (int i = ku - 1; i >= 0; --i) {
-            double acc = s->g[i];
-            for (int j = i + 1; j < ku; ++j) acc -= s->H[i + static_cast<size_t>(j) * (m + 1)] * s->y[j];
-            s->y[i] = acc / s->H[i + static_cast<size_t>(i) * (m + 1)];
-          }
-          s->ycount = ku;
-          s->phase = kResidual;  // x += V y, then the true residual
-        } else {
-          s->svec = s->V + static_cast<int64_t>(k + 1) * s->n;
-          s->sval = 1.0 / hn;
-          s->k = k + 1;
-        }
-      }
-    }
+  // every suspension stops by itself after at most maxit Arnoldi steps and
+  // one true residual per cycle; the bound only guards the host loop
+  const int64_t max_rounds = (static_cast<int64_t>(maxit) + maxit / std::max(m, 1) + 4) / kRound + 2;
+  bool any = true;
+  for (int64_t r = 0; any; ++r) {
+    if (r >= max_rounds) return false;
+    if (!lockstep_round(S, &any)) return false;
   }
-  for (auto* s : S->active) {
+  if (hipMemcpyAsync(S->h_ctl, S->d_ctl, ns * sizeof(rf::SusCtl), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return false;
+  for (int q = 0; q < ns; ++q) {
+    Suspension* s = S->active[q];
+    const rf::SusCtl& C = S->h_ctl[q];
     s->have_solution = true;
-    s->req->info.gmres_iterations += s->it;
-    if (!s->converged && s->status == RF_OK) s->status = RF_WARN_NOT_CONVERGED;
+    s->req->info.gmres_iterations += C.it;
+    if (C.nonfinite)
+      s->status = RF_ERR_NONFINITE;
+    else if (!C.converged && s->status == RF_OK)
+      s->status = RF_WARN_NOT_CONVERGED;
   }
   return true;
 }
@@ -659,11 +595,12 @@ void worker_main(rf_server* S) {
 void free_tables(rf_server* S) {
   for (void* p : {static_cast<void*>(S->d_sus), static_cast<void*>(S->d_it), static_cast<void*>(S->d_targets),
                   static_cast<void*>(S->d_vblocks), static_cast<void*>(S->d_fblocks), static_cast<void*>(S->d_part),
-                  static_cast<void*>(S->d_hout), static_cast<void*>(S->d_ycoef)})
+                  static_cast<void*>(S->d_hout), static_cast<void*>(S->d_ctl), static_cast<void*>(S->d_ctlmem),
+                  static_cast<void*>(S->d_act)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(S->h_sus), static_cast<void*>(S->h_it), static_cast<void*>(S->h_targets),
                   static_cast<void*>(S->h_vblocks), static_cast<void*>(S->h_fblocks), static_cast<void*>(S->h_hout),
-                  static_cast<void*>(S->h_ycoef)})
+                  static_cast<void*>(S->h_ctl), static_cast<void*>(S->h_act)})
     if (p) (void)hipHostFree(p);
 }
 

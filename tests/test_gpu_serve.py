@@ -52,10 +52,10 @@ def test_result_same_alone_and_in_mix():
 
 
 def test_graph_replay_matches_eager_launches():
-    """Each tick replays its captured GMRES iteration graph many times (one
-    replay per lockstep iteration, host work and synchronisation between
-    replays); the results must be bitwise those of launching the same kernels
-    one by one."""
+    """Each tick replays its captured graph of a round of GMRES iterations
+    (kernels and device-side control steps) several times, with a device to
+    host copy and a synchronisation between replays; the results must be
+    bitwise those of launching the same kernels one by one."""
     from rigidfibers_amd.serve import Server
 
     reqs = _requests([23, 64, 17, 120, 45], steps=4, seed=5)
