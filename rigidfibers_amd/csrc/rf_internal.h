@@ -123,8 +123,10 @@ struct SusCtl {
   double* g;   // m + 1 rotated right-hand side
   double* y;   // m coefficients of x += V y
 };
-hipError_t launch_gmres_control_batch(int ns, int m, int kdim, double tol, int maxit, const SusDev* sus, SusIter* it,
-                                      SusCtl* ctl, int* active, hipStream_t s);
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, const SusDev* sus,
+                                      SusIter* it, SusCtl* ctl, const double* part, int* active, hipStream_t s);
+hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
+                                  const int2* vblocks, double* part, hipStream_t s);
 
 hipError_t launch_assemble_theta_batch(int N, int NQI, bool analytical, int ntargets, const SusDev* sus,
                                        const int2* targets, const PhysConsts& pc, hipStream_t s);
@@ -141,8 +143,8 @@ hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it,
 hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, const SusDev* sus,
                              const SusIter* it, const int2* vblocks, double* part, hipStream_t s);
 // w -= V[0..K) hout[phase (kdim+1) ..]
-hipError_t launch_orth_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
-                             const int2* vblocks, hipStream_t s);
+hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const SusIter* it, const int2* vblocks,
+                                    double* part, hipStream_t s);
 
 // Dense linear algebra (rf_linalg.hip).
 struct GemvPlan {

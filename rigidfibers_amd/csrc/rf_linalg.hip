@@ -430,42 +430,108 @@ __global__ __launch_bounds__(64) void dots_final_batch_kernel(int phase, int nvb
   if (threadIdx.x == 0) c.hout[phase * (kdim + 1) + j] = v;
 }
 
-// w -= V[0..K) h, h = hout[phase (kdim + 1) ..]
-__global__ __launch_bounds__(kVecBlock) void orth_batch_kernel(int phase, int kdim, const SusDev* __restrict__ sus,
-                                                               const SusIter* __restrict__ it,
-                                                               const int2* __restrict__ vblocks) {
+
+// Sum of a suspension's block partials of row j, in block order (the same
+// order wherever it is formed, so every consumer sees the same value).
+__device__ __forceinline__ double sum_block_parts(const double* __restrict__ part, int nvb, int j, int first,
+                                                  int count) {
+  const double* p = part + static_cast<int64_t>(j) * nvb + first;
+  double v = 0.0;
+  for (int b = 0; b < count; ++b) v += p[b];
+  return v;
+}
+
+// Gram-Schmidt pass `phase` (0 or 1) on one 256-row block of one suspension:
+// h_j (j < K) from the pass's block partials, w -= sum_j h_j V_j on the
+// block's rows, then the block partials of the next reduction on the same
+// rows: <V_j, w> after pass 0, <w, w> after pass 1.  Partials of reduction p
+// live in part + p kdim nvb, row j at j nvb + block.
+__global__ __launch_bounds__(kVecBlock) void orth_dots_batch_kernel(int phase, int nvb, int kdim,
+                                                                    const SusDev* __restrict__ sus,
+                                                                    const SusIter* __restrict__ it,
+                                                                    const int2* __restrict__ vblocks,
+                                                                    double* __restrict__ part) {
+  extern __shared__ double dsh[];  // h[kdim], then the waves' partials [kVecBlock / 64][kdim]
+  double* hsh = dsh;
+  double* wsh = dsh + kdim;
   const int2 e = vblocks[blockIdx.x];
   const SusDev& d = sus[e.x];
   const SusIter& c = it[e.x];
-  if (c.w == nullptr || c.K == 0) return;
+  if (c.w == nullptr) return;  // the whole block belongs to one suspension
+  const int K = c.K;
+  const int64_t region = static_cast<int64_t>(kdim) * nvb;
+  for (int j = threadIdx.x; j < K; j += kVecBlock)
+    hsh[j] = sum_block_parts(part + phase * region, nvb, j, d.vb_first, d.vb_count);
+  __syncthreads();
   const int r = e.y * kVecBlock + threadIdx.x;
-  if (r >= d.n) return;
-  const double* h = c.hout + phase * (kdim + 1);
-  double s = 0.0;
-  for (int k = 0; k < c.K; ++k) s = fma(d.V[static_cast<int64_t>(k) * d.n + r], h[k], s);
-  c.w[r] -= s;
+  const bool in = r < d.n;
+  double wr = 0.0;
+  if (in) {
+    double sacc = 0.0;
+    for (int k = 0; k < K; ++k) sacc = fma(d.V[static_cast<int64_t>(k) * d.n + r], hsh[k], sacc);
+    wr = c.w[r] - sacc;
+    c.w[r] = wr;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int J = phase == 0 ? K : 1;
+  for (int j = 0; j < J; ++j) {
+    double v = in ? (phase == 0 ? d.V[static_cast<int64_t>(j) * d.n + r] * wr : wr * wr) : 0.0;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) wsh[wv * kdim + j] = v;
+  }
+  __syncthreads();
+  double* pout = part + (phase + 1) * region;
+  for (int j = threadIdx.x; j < J; j += kVecBlock) {
+    double t = 0.0;
+    for (int q = 0; q < kVecBlock / 64; ++q) t += wsh[q * kdim + j];
+    pout[static_cast<int64_t>(j) * nvb + blockIdx.x] = t;
+  }
 }
 
-// Control step of the serving lockstep (one lane per suspension), the same
-// GMRES(m) as the single-simulation solver: true residual at every restart,
-// classical Gram-Schmidt twice (the two passes' coefficients summed), Givens
-// rotations, stop on ||r|| <= tol ||b||.  Writes the next iteration's
-// operations into it[q] and whether it has any into active[q].
-__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int ns, int m, int kdim, double tol, int maxit,
+hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
+                                  const int2* vblocks, double* part, hipStream_t s) {
+  if (nvb <= 0) return hipSuccess;
+  const size_t shmem = static_cast<size_t>(kdim) * (1 + kVecBlock / 64) * sizeof(double);
+  hipLaunchKernelGGL(orth_dots_batch_kernel, dim3(nvb), dim3(kVecBlock), shmem, s, phase, nvb, kdim, sus, it,
+                     vblocks, part);
+  return hipGetLastError();
+}
+
+// Control step of the serving lockstep (one workgroup per suspension), the
+// same GMRES(m) as the single-simulation solver: true residual at every
+// restart, classical Gram-Schmidt twice (the two passes' coefficients summed),
+// Givens rotations, stop on ||r|| <= tol ||b||.  The lanes form the
+// iteration's reductions from the block partials, lane 0 advances the state
+// and writes the next iteration's operations into it[q] and whether it has
+// any into active[q].
+__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit,
                                                                  const SusDev* __restrict__ sus,
                                                                  SusIter* __restrict__ it, SusCtl* __restrict__ ctl,
+                                                                 const double* __restrict__ part,
                                                                  int* __restrict__ active) {
-  const int q = blockIdx.x * 64 + threadIdx.x;
-  if (q >= ns) return;
+  extern __shared__ double hsh[];  // h0[K], h1[K], ||w||^2
+  const int q = blockIdx.x;
   const SusDev& d = sus[q];
   SusIter& c = it[q];
   SusCtl& C = ctl[q];
-  const double* h = c.hout;
+  const int K = c.K;
+  const int64_t region = static_cast<int64_t>(kdim) * nvb;
+  if (c.w != nullptr)
+    for (int t = threadIdx.x; t <= 2 * K; t += 64) {
+      const int p = t < K ? 0 : (t < 2 * K ? 1 : 2);
+      const int j = t < K ? t : (t < 2 * K ? t - K : 0);
+      hsh[t] = sum_block_parts(part + p * region, nvb, j, d.vb_first, d.vb_count);
+    }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* h0 = hsh;
+  const double* h1 = hsh + K;
+  const double w2 = hsh[2 * K];
   int ycount = 0;
   double* svec = nullptr;
   double sval = 1.0;
-  if (C.phase == 0) {  // h holds ||b - A x||^2
-    const double beta = sqrt(h[2 * (kdim + 1)]);
+  if (C.phase == 0) {  // w = b - A x
+    const double beta = sqrt(w2);
     C.res = beta;
     if (!isfinite(beta)) {
       C.nonfinite = 1;
@@ -483,16 +549,16 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int ns, int m, 
       C.k = 0;
       C.phase = 1;
     }
-  } else if (C.phase == 1) {  // h holds <V_j, w> of both passes and ||w||^2
+  } else if (C.phase == 1) {  // w = A V_k orthogonalised twice against V_0..V_k
     const int k = C.k;
     ++C.it;
-    const double hn = sqrt(h[2 * (kdim + 1)]);
+    const double hn = sqrt(w2);
     if (!isfinite(hn)) {
       C.nonfinite = 1;
       C.phase = 2;
     } else {
       double* Hk = C.H + static_cast<int64_t>(k) * (m + 1);
-      for (int i = 0; i <= k; ++i) Hk[i] = h[i] + h[(kdim + 1) + i];
+      for (int i = 0; i <= k; ++i) Hk[i] = h0[i] + h1[i];
       Hk[k + 1] = hn;
       for (int i = 0; i < k; ++i) {
         const double a0 = Hk[i], a1 = Hk[i + 1];
@@ -545,11 +611,11 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int ns, int m, 
   active[q] = (C.phase != 2 || ycount > 0 || svec != nullptr) ? 1 : 0;
 }
 
-hipError_t launch_gmres_control_batch(int ns, int m, int kdim, double tol, int maxit, const SusDev* sus, SusIter* it,
-                                      SusCtl* ctl, int* active, hipStream_t s) {
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, const SusDev* sus,
+                                      SusIter* it, SusCtl* ctl, const double* part, int* active, hipStream_t s) {
   if (ns <= 0) return hipSuccess;
-  hipLaunchKernelGGL(gmres_control_batch_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, ns, m, kdim, tol, maxit, sus,
-                     it, ctl, active);
+  hipLaunchKernelGGL(gmres_control_batch_kernel, dim3(ns), dim3(64), (2 * kdim + 1) * sizeof(double), s, m, kdim,
+                     nvb, tol, maxit, sus, it, ctl, part, active);
   return hipGetLastError();
 }
 
@@ -570,10 +636,10 @@ hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, c
   return hipGetLastError();
 }
 
-hipError_t launch_orth_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
-                             const int2* vblocks, hipStream_t s) {
-  if (nvb <= 0) return hipSuccess;
-  hipLaunchKernelGGL(orth_batch_kernel, dim3(nvb), dim3(kVecBlock), 0, s, phase, kdim, sus, it, vblocks);
+hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const SusIter* it, const int2* vblocks,
+                                    double* part, hipStream_t s) {
+  if (nvb <= 0 || kmax <= 0) return hipSuccess;
+  hipLaunchKernelGGL(dots_batch_kernel, dim3(nvb, kmax), dim3(kVecBlock), 0, s, 0, nvb, sus, it, vblocks, part);
   return hipGetLastError();
 }
 

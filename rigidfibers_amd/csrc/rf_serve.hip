@@ -240,7 +240,7 @@ bool build_tables(rf_server* S) {
       !grow_host(&S->h_targets, &S->c_ht, nt) || !grow_device(&S->d_targets, &S->c_dt, nt) ||
       !grow_host(&S->h_vblocks, &S->c_hv, nv) || !grow_device(&S->d_vblocks, &S->c_dv, nv) ||
       !grow_host(&S->h_fblocks, &S->c_hf, nf) || !grow_device(&S->d_fblocks, &S->c_df, nf) ||
-      !grow_device(&S->d_part, &S->c_part, static_cast<size_t>(nv) * S->kdim) ||
+      !grow_device(&S->d_part, &S->c_part, 3 * static_cast<size_t>(nv) * S->kdim) ||
       !grow_host(&S->h_hout, &S->c_hh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
       !grow_device(&S->d_hout, &S->c_dh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
       !grow_host(&S->h_ctl, &S->c_hctl, ns) || !grow_device(&S->d_ctl, &S->c_dctl, ns) ||
@@ -301,17 +301,16 @@ bool iteration_launches(rf_server* S) {
   if (rf::launch_iter_prep_batch(S->nvb, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
   if (rf::launch_gemv_theta_batch(S->N, S->ntargets, S->d_sus, S->d_it, S->d_targets, S->pc, st) != hipSuccess)
     return false;
-  for (int ph = 0; ph < 2; ++ph) {
-    if (rf::launch_dots_batch(S->nvb, ns, S->m, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
+  // <V_j, w> partials, then two fused passes (orthogonalise, next partials)
+  if (rf::launch_dots_partial_batch(S->nvb, S->m, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) != hipSuccess)
+    return false;
+  for (int ph = 0; ph < 2; ++ph)
+    if (rf::launch_orth_dots_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
         hipSuccess)
       return false;
-    if (rf::launch_orth_batch(S->nvb, S->kdim, ph, S->d_sus, S->d_it, S->d_vblocks, st) != hipSuccess) return false;
-  }
-  if (rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) != hipSuccess)
-    return false;
-  return rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->cfg.base.gmres_tolerance,
-                                        S->cfg.base.gmres_max_iterations, S->d_sus, S->d_it, S->d_ctl, S->d_act,
-                                        st) == hipSuccess;
+  return rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->nvb, S->cfg.base.gmres_tolerance,
+                                        S->cfg.base.gmres_max_iterations, S->d_sus, S->d_it, S->d_ctl, S->d_part,
+                                        S->d_act, st) == hipSuccess;
 }
 
 bool round_launches(rf_server* S) {
