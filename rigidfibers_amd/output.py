@@ -9,6 +9,9 @@
                                         orientations_t .out (src/simulation.cu:606-822)
                                         and the Fortran t_AMat / t_BVec / ... names
                                         (fortran/fibers.f95:205-303)
+  current.map                           per entry of A the simulation variables it
+                                        couples (src/simulation.cu:711-740,
+                                        src/kernels/assemble_system_2D.cu:71-92,162-225)
   performance.out                       name,time rows (src/performance.cc:110-134)
 """
 from __future__ import annotations
@@ -70,6 +73,38 @@ def write_performance(path, rows):
         fh.write("name,time\n")
         for name, t in rows:
             fh.write(f"{name},{t:.9f}\n")
+
+
+def mapping_fields(rows, cols, N):
+    """(i, j, k_j, k_i, a, b) of the entries (rows, cols) of A: row 3N i + 3 k_i + a,
+    column 3N j + 3 k_j + b.  Entries of a fiber's own block other than its unit
+    diagonal are never assembled and carry -1 in every field."""
+    rows = np.asarray(rows, dtype=np.int64)
+    cols = np.asarray(cols, dtype=np.int64)
+    w = 3 * N
+    i, ri = np.divmod(rows, w)
+    j, rj = np.divmod(cols, w)
+    ki, a = np.divmod(ri, 3)
+    kj, b = np.divmod(rj, 3)
+    f = np.stack([i, j, kj, ki, a, b])
+    unset = (i == j) & ~((ki == kj) & (a == b))
+    f[:, unset] = -1
+    return f
+
+
+def write_mapping(path, M, N, rows_per_chunk=256):
+    """current.map: one line "[V]|i|j|k_j|k_i|a|b|row|col" per entry of the
+    n x n matrix, rows outer, columns inner."""
+    n = 3 * N * M
+    cols = np.arange(n, dtype=np.int64)
+    with open(path, "w") as fh:
+        for r0 in range(0, n, rows_per_chunk):
+            r1 = min(n, r0 + rows_per_chunk)
+            rows = np.repeat(np.arange(r0, r1, dtype=np.int64), n)
+            cc = np.tile(cols, r1 - r0)
+            f = mapping_fields(rows, cc, N)
+            fh.write("".join(f"[V]|{a}|{b}|{c}|{d}|{e}|{g}|{r}|{q}\n"
+                             for a, b, c, d, e, g, r, q in zip(*f.tolist(), rows.tolist(), cc.tolist())))
 
 
 def dump_step(out_dir, platform, step, b, x, tvel, rvel, positions, orientations, A=None):

@@ -185,6 +185,9 @@ def validate(args):
             p, o = sim.next_state()
             sim.advance()
             names = output.dump_step(args.output_dir, args.platform, t, b, x, tv, rv, p, o, A)
+            if t == 0 and args.platform == "cuda":  # written once: it does not change
+                output.write_mapping(os.path.join(args.output_dir, "current.map"), M,
+                                     params.NUMBER_OF_TERMS_IN_FORCE_EXPANSION)
             for key, field in (("b", "BVec"), ("x", "XVec"), ("t", "TRANSVel"), ("r", "ROTVel"), ("p", "POS"),
                                ("o", "ORIENT")):
                 ref = os.path.join(ref_dir, f"{t}_{field}.ref")
