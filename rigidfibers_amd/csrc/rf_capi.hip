@@ -860,6 +860,7 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
   s->have_history = history != 0;
   s->have_next = false;
   s->vvalid = false;
+  s->have_previous = false;  // extrapolation needs two solutions of consecutive steps
   return RF_OK;
 }
 

@@ -563,6 +563,11 @@ void worker_main(rf_server* S) {
     const auto t_start = Clock::now();
     const bool ok = tick(S);
     const double ms = std::chrono::duration<double, std::milli>(Clock::now() - t_start).count();
+    if (!ok) {  // let queued work finish before the suspensions' memory is freed
+      (void)hipStreamSynchronize(S->stream);
+      (void)hipGetLastError();
+      release_iteration(S);
+    }
     std::vector<Suspension*> keep;
     int64_t fsteps = 0;
     for (auto* s : S->active) {
