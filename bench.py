@@ -69,7 +69,7 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
     matrix_format = args.matrix_format if solver in ITERATIVE else "auto"
     sim = Simulation(params, pos, ori, solver=solver, integration=args.integration, matrix_format=matrix_format,
                      velocity_operator=args.velocity_operator, refinement_tolerance=1e-10,
-                     initial_guess=args.initial_guess)
+                     initial_guess=args.initial_guess, lu_gemm=args.lu_gemm)
     try:
         for t in range(warmup):
             sim.step(t)
@@ -78,10 +78,12 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
         phases = {"assemble_system": 0.0, "solve_system": 0.0, "update_velocities": 0.0, "update_fibers": 0.0}
         acc = {"gemv": 0.0, "gemv_launches": 0, "gemm": 0.0, "gemm_flop": 0.0, "gemm_launches": 0}
         iters = []
+        library = False
         t0 = time.perf_counter()
         for t in range(warmup, warmup + steps):
             sim.step(t)
             tm = sim.timings()
+            library = library or bool(tm["gemm_library"])
             for k in phases:
                 phases[k] += tm[k]
             for k in acc:
@@ -95,7 +97,7 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
     finally:
         sim.close()
     return {"elapsed": elapsed, "phases": phases, "acc": acc, "iters": iters, "stats": stats, "n": n,
-            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps}
+            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps, "library": library}
 
 
 def measured_traffic(kernel, M, N):
@@ -126,7 +128,8 @@ def roofline(args, solver, g):
                 "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4)}
     peak = MFMA_PEAK_TFLOPS[solver]
     achieved = acc["gemm_flop"] / (acc["gemm"] * 1e-3) / 1e12 if acc["gemm"] > 0 else 0.0
-    return {"bound": "mfma", "kernel": GEMM_KERNEL[solver], "achieved": round(achieved, 1), "peak": peak,
+    kernel = "hipBLASLt TN GEMM (trailing updates)" if g.get("library") else GEMM_KERNEL[solver]
+    return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 1), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
             "flop_per_step": acc["gemm_flop"] / g["steps"], "launches_per_step": acc["gemm_launches"] / g["steps"],
             "avg_launch_ms": round(acc["gemm"] / max(acc["gemm_launches"], 1), 4)}
@@ -305,6 +308,8 @@ def main():
     ap.add_argument("--initial-guess", default="previous", choices=["previous", "zero", "extrapolated"],
                     help="iterative solvers' start: the previous solution or 2 x_n - x_{n-1}")
     ap.add_argument("--lu-steps", type=int, default=2, help="timed steps per solver of the precision study")
+    ap.add_argument("--lu-gemm", default="library", choices=["library", "builtin"],
+                    help="mixed LUs' trailing updates: hipBLASLt when it has an algorithm, or the built-in kernel")
     ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")

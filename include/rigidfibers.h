@@ -124,7 +124,10 @@ typedef struct rf_config {
   int32_t matrix_format;                              /* rf_matrix_format */
   int32_t velocity_operator;                          /* rf_velocity_operator */
   double inner_tolerance;                             /* GMRES-IR: relative residual of each inner solve (0: 1e-4) */
-  int32_t reserved[4];
+  int32_t lu_gemm;                                    /* mixed LUs' trailing updates: 0 hipBLASLt when it has an
+                                                         algorithm (else the built-in kernel), 1 the built-in
+                                                         MFMA kernel always */
+  int32_t reserved[3];
 } rf_config;
 
 typedef struct rf_solver_stats {
@@ -148,7 +151,7 @@ typedef struct rf_timings {
   double gemm_flop;        /* their algorithmic flops, 2 m n K each */
   double gemv_half;        /* GMRES-IR: summed duration of the FP16 matrix-vector kernels */
   int32_t gemv_half_launches;
-  int32_t reserved;
+  int32_t gemm_library;    /* 1: the trailing updates ran through hipBLASLt (rf_config.lu_gemm) */
 } rf_timings;
 
 typedef struct rf_simulation rf_simulation;

@@ -61,7 +61,8 @@ class RfConfig(C.Structure):
         ("matrix_format", C.c_int32),
         ("velocity_operator", C.c_int32),
         ("inner_tolerance", C.c_double),
-        ("reserved", C.c_int32 * 4),
+        ("lu_gemm", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 
@@ -83,7 +84,7 @@ class RfTimings(C.Structure):
         ("gemm_flop", C.c_double),
         ("gemv_half", C.c_double),
         ("gemv_half_launches", C.c_int32),
-        ("reserved", C.c_int32),
+        ("gemm_library", C.c_int32),
     ]
 
 

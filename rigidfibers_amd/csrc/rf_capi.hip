@@ -157,6 +157,7 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
   if (c->matrix_format < RF_MATRIX_AUTO || c->matrix_format > RF_MATRIX_THETA)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown matrix format");
+  if (c->lu_gemm != 0 && c->lu_gemm != 1) return fail(RF_ERR_INVALID_ARGUMENT, "lu_gemm must be 0 or 1");
   if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS &&
       c->initial_guess != RF_GUESS_EXTRAPOLATED)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
@@ -669,7 +670,7 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     const rf::LuMode mode = cfg->solver == RF_SOLVER_DIRECT ? rf::LuMode::kFP32
                             : cfg->solver == RF_SOLVER_DIRECT_MIXED_BF16 ? rf::LuMode::kBF16
                                                                           : rf::LuMode::kFP16;
-    hipError_t e = rf::lu_workspace_create(mode, n, s->lda, &s->lu);
+    hipError_t e = rf::lu_workspace_create(mode, n, s->lda, cfg->lu_gemm == 0, &s->lu);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return cleanup(fail(e == hipErrorOutOfMemory ? RF_ERR_OUT_OF_MEMORY : RF_ERR_DEVICE,
@@ -805,6 +806,7 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.gemm_launches = s->lu_last.gemm_launches;
   s->timings.gemm = s->lu_last.gemm_ms;
   s->timings.gemm_flop = s->lu_last.gemm_flop;
+  s->timings.gemm_library = s->lu_last.library ? 1 : 0;
   RF_TRY(rf_advance(s));
   if (!s->stats.converged) {
     g_last_error = "solver did not reach its tolerance (relative residual " +

@@ -25,6 +25,7 @@ struct LuWorkspace {
   void* At = nullptr;      // packed L panel: n rows x kLuPanel, K-contiguous, mode's element type
   void* Bt = nullptr;      // packed U panel: n columns x kLuPanel
   double* Dinv = nullptr;  // per diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 FP64 each, [j][c]
+  void* blt = nullptr;     // hipBLASLt state for the mixed modes' trailing updates (none: built-in kernel)
   static constexpr int kEvents = 256;  // event pairs around the trailing updates
   hipEvent_t ev[2 * kEvents] = {};
   int64_t bytes = 0;
@@ -39,6 +40,7 @@ struct LuResult {
   double gemm_ms = 0.0;           // summed duration of the trailing-update GEMM launches
   double gemm_flop = 0.0;         // their algorithmic flops (2 m n K each)
   int gemm_launches = 0;
+  bool library = false;           // the trailing updates ran through hipBLASLt
 };
 
 // C -= A B with K-contiguous packs At (m rows, row stride ldak elements) and Bt
@@ -48,7 +50,18 @@ int gemm_stage_k(LuMode mode);
 hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
                            const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s);
 
-hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws);
+// The same update through hipBLASLt (mixed modes): blt_create makes the
+// library state (handle, descriptor, workspace of *bytes), blt_gemm_sub runs
+// it and returns hipErrorNotSupported when the library has no algorithm.
+hipError_t blt_create(LuMode mode, void** out, int64_t* bytes);
+void blt_destroy(void* p);
+hipError_t blt_gemm_sub(void* p, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak, const void* Bt,
+                        int64_t ldbk, float* C, int64_t ldc, hipStream_t s);
+
+// library: run the mixed modes' trailing updates through hipBLASLt when it is
+// available (otherwise, and for FP32 and the in-panel updates, the built-in
+// kernel).
+hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, bool library, LuWorkspace* ws);
 int64_t lu_workspace_bytes(const LuWorkspace& ws);
 void lu_workspace_destroy(LuWorkspace* ws);
 

@@ -417,9 +417,18 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
       // trailing update, K = W
       const bool timed = nev < LuWorkspace::kEvents;
       if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-      if ((e = gemm_packed_sub(mode, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
-                               at(KE, KE), lda, s)) != hipSuccess)
+      e = ws.blt ? blt_gemm_sub(ws.blt, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
+                                at(KE, KE), lda, s)
+                 : hipErrorNotSupported;
+      if (e == hipSuccess) {
+        res->library = true;
+      } else if (e == hipErrorNotSupported) {
+        if ((e = gemm_packed_sub(mode, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
+                                 at(KE, KE), lda, s)) != hipSuccess)
+          return e;
+      } else {
         return e;
+      }
       if (timed) {
         (void)hipEventRecord(ws.ev[2 * nev + 1], s);
         ++nev;
@@ -465,7 +474,7 @@ static hipError_t lu_apply_inverse(LuWorkspace& ws, double* y, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws) {
+hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, bool library, LuWorkspace* ws) {
   ws->mode = mode;
   ws->n = n;
   ws->lda = lda;
@@ -484,6 +493,13 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
                                    sizeof(double) * 3 * static_cast<size_t>(n) + 2 * pack + dinv);
+  if (library && mode != LuMode::kFP32) {
+    int64_t wb = 0;
+    if (blt_create(mode, &ws->blt, &wb) == hipSuccess)
+      ws->bytes += wb;
+    else
+      (void)hipGetLastError();  // no library state: the built-in kernel runs every update
+  }
   return hipSuccess;
 }
 
@@ -499,6 +515,7 @@ void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->Dinv) (void)hipFree(ws->Dinv);
   for (auto& ev : ws->ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (ws->blt) blt_destroy(ws->blt);
   *ws = LuWorkspace{};
 }
 

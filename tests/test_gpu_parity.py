@@ -269,7 +269,7 @@ def test_ring_symmetry():
                                       ("mixed-fp16", 230), ("gmres-ir", 60), ("gmres-ir", 230)])
 def test_solvers_agree_with_oracle(oracle64, solver, M):
     """x of each solver against the oracle's LU solution of the oracle's system
-    (n = 900 and 3450: one and several 256-column outer LU panels with ragged
+    (n = 900 and 3450: one and several 1024-column outer LU panels with ragged
     edges)."""
     from oracle.pyoracle import make_params
 
@@ -288,6 +288,30 @@ def test_solvers_agree_with_oracle(oracle64, solver, M):
     print(solver, st, rel_err(x, x_ref))
     assert st["converged"]
     assert rel_err(x, x_ref) < 1e-6
+
+
+@pytest.mark.parametrize("solver", ["mixed-bf16", "mixed-fp16"])
+def test_lu_trailing_updates_library_and_builtin(solver):
+    """The mixed LUs' trailing updates through hipBLASLt and through the
+    built-in MFMA kernel give the same refined solution (n = 4500: four
+    trailing updates), and the timings say which one ran."""
+    M = 300
+    pos, ori = random_scene(M, side=5.0 * (M / 60) ** (1 / 3), seed=8, min_dist=0.4)
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    out = {}
+    for lu_gemm in ("library", "builtin"):
+        with _sim(params, pos, ori, solver=solver, lu_gemm=lu_gemm, refinement_tolerance=1e-11) as sim:
+            sim.step(0)
+            _, _, x = sim.system()
+            st, tm = sim.solver_stats(), sim.timings()
+        assert st["converged"] and st["relative_residual"] <= 1e-11
+        out[lu_gemm] = (x, tm["gemm_library"], tm["gemm_launches"])
+    assert out["library"][1] == 1 and out["builtin"][1] == 0
+    assert out["library"][2] == out["builtin"][2] == 4
+    e = rel_err(out["library"][0], out["builtin"][0])
+    print(f"{solver}: library vs built-in trailing updates, x {e:.1e}")
+    assert e < 1e-9
 
 
 def test_gmres_ir_uses_the_fp16_copy():
