@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 MFMA_PEAK_TFLOPS = {"direct": 157.3, "mixed-bf16": 2500.0, "mixed-fp16": 2500.0}
 GEMM_KERNEL = {"direct": "gemm_packed_sub_kernel<float>", "mixed-bf16": "gemm_packed_sub_kernel<__bf16>",
                "mixed-fp16": "gemm_packed_sub_kernel<_Float16>"}
-ITERATIVE = ("gmres", "bicgstab")
+ITERATIVE = ("gmres", "bicgstab", "gmres-ir")
 
 
 def build_scene(M, radius, seed):
@@ -141,8 +141,11 @@ def rel_err(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(np.asarray(b))))
 
 
-def precision_study(args, pos, ori):
-    """Plain FP32 LU vs the BF16/FP16-MFMA LUs with FP64 refinement, same scene and steps."""
+def precision_study(args, pos, ori, plain=None):
+    """Plain FP32 LU vs the BF16/FP16-MFMA LUs with FP64 refinement, same scene
+    and steps; and, beside the headline's plain FP32 GMRES run (`plain`),
+    GMRES-IR with its inner solves on the FP16 copy of the theta-form system
+    (same steps and tolerance)."""
     out, ref = {}, None
     for solver in ("direct", "mixed-bf16", "mixed-fp16"):
         g = run_gpu(args, pos, ori, solver, args.lu_steps, 1, tag="precision ")
@@ -160,6 +163,18 @@ def precision_study(args, pos, ori):
         out["plain_fp32_lu" if solver == "direct" else solver.replace("-", "_")] = entry
     out["steps"] = args.lu_steps
     out["warmup"] = 1
+    if plain is not None:
+        g = run_gpu(args, pos, ori, "gmres-ir", args.steps, args.warmup, tag="precision ")
+        rate = args.fibers * g["steps"] / g["elapsed"]
+        plain_rate = args.fibers * plain["steps"] / plain["elapsed"]
+        out["gmres_ir_fp16"] = {
+            "fiber_steps_per_s": round(rate, 2), "ms_per_step": round(1e3 * g["elapsed"] / g["steps"], 2),
+            "solve_ms_per_step": round(g["phases"]["solve_system"] / g["steps"], 2),
+            "plain_solve_ms_per_step": round(plain["phases"]["solve_system"] / plain["steps"], 2),
+            "iterations_per_step": g["iters"], "steps": g["steps"],
+            "speedup_vs_plain_fp32_gmres": round(rate / plain_rate, 3),
+            "pos_rel_err_vs_plain_fp32_gmres": rel_err(g["pos"], plain["pos"]),
+            "ori_rel_err_vs_plain_fp32_gmres": rel_err(g["ori"], plain["ori"])}
     return out
 
 
@@ -359,7 +374,7 @@ def main():
     }
     line["kernels"]["gemv_ms_per_step"] = round(g["acc"]["gemv"] / K, 3)
     if not args.no_precision_study:
-        line["precision"] = precision_study(args, pos, ori)
+        line["precision"] = precision_study(args, pos, ori, plain=g if args.solver == "gmres" else None)
     if not args.no_serving:
         line["serving"] = serving(args)
     if not args.no_large:
