@@ -100,6 +100,32 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
             "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps, "library": library}
 
 
+# VALU issue of the analytical theta assembly (N = 5, 8 x 3 nodes): lane
+# instructions per ordered fiber pair, from its counters at M = 5000
+# (profiles/assemble_theta_M5000_counters.md: 1.074e10 wave instructions per
+# launch x 64 lanes / 5000 x 4999 pairs), against one wave instruction per 4
+# cycles on each of 1024 SIMDs at 2.4 GHz.
+ASSEMBLY_LANE_INSTR_PER_PAIR = 27_500
+VALU_LANE_INSTR_PER_S = 1024 * 64 / 4 * 2.4e9
+
+
+def step_bound(args, g):
+    """The step at its bounds: assembly at full VALU issue of its instruction
+    count, every matvec of the step and the stored velocity pass at 8 TB/s."""
+    if not (args.integration == "analytical" and args.solver == "gmres"):
+        return None
+    M, N, K = args.fibers, 5, g["steps"]
+    asm_ms = 1e3 * ASSEMBLY_LANE_INSTR_PER_PAIR * M * (M - 1) / VALU_LANE_INSTR_PER_S
+    mv_ms = 1e3 * M * (3 + 6 * (N - 1)) * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
+    vel_ms = 1e3 * M * 12 * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
+    matvecs = g["acc"]["gemv_launches"] / K
+    total = asm_ms + matvecs * mv_ms + vel_ms
+    return {"assembly_ms": round(asm_ms, 2), "matvec_ms": round(mv_ms, 3), "matvecs_per_step": round(matvecs, 2),
+            "velocity_ms": round(vel_ms, 3), "ms_per_step": round(total, 2),
+            "fiber_steps_per_s": round(M / (total * 1e-3), 1),
+            "achieved_frac": round((M * K / g["elapsed"]) / (M / (total * 1e-3)), 3)}
+
+
 def measured_traffic(kernel, M, N):
     """HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE counter
     passes (scaled by the copy-kernel calibration), or None for other shapes."""
@@ -372,6 +398,9 @@ def main():
         "solver": {"iterations_per_step": g["iters"], "last": g["stats"]},
         "device_bytes": g["device_bytes"],
     }
+    bound = step_bound(args, g)
+    if bound is not None:
+        line["step_bound"] = bound
     line["kernels"]["gemv_ms_per_step"] = round(g["acc"]["gemv"] / K, 3)
     if not args.no_precision_study:
         line["precision"] = precision_study(args, pos, ori, plain=g if args.solver == "gmres" else None)
