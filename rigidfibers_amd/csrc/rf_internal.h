@@ -104,18 +104,18 @@ struct SusIter {
   int gmode;
   double* w;                       // orthogonalised against V[0..K), then ||w||^2
   int K;
-  double* hout;                    // 3 (kdim + 1): <V_j, w> twice, ||w||^2
 };
 
 // Device-side GMRES(m) state of one suspension in the serving lockstep: the
-// control step after each iteration reads the iteration's Hessenberg column
-// (SusIter::hout), advances this state and writes the next SusIter.
+// control step after each iteration forms the iteration's reductions from
+// the block partials, advances this state and writes the next SusIter.
 struct SusCtl {
-  int phase;   // 0: true residual next, 1: Arnoldi step next, 2: done
+  int phase;   // 3: ||b|| next (w = b), 0: true residual next, 1: Arnoldi step next, 2: done
   int k;       // Arnoldi column of the current cycle
   int it;      // Arnoldi steps so far
   int converged;
   int nonfinite;
+  int cold;    // start from x = 0 (no usable previous solution)
   double bnorm, res;
   double* H;   // (m + 1) x m Hessenberg matrix, column-major
   double* cs;  // m Givens cosines
@@ -138,11 +138,9 @@ hipError_t launch_update_batch(int nfblocks, const SusDev* sus, const int2* fblo
 // x = 0 / x += V y / svec *= sval, per suspension (before the iteration's GEMV)
 hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it, const int2* vblocks,
                                   hipStream_t s);
-// phase 0/1: hout[phase (kdim+1) + j] = <V_j, w>, j < K; phase 2: hout[2 (kdim+1)] = <w, w>.
-// part: nvb * kdim doubles.
-hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, const SusDev* sus,
-                             const SusIter* it, const int2* vblocks, double* part, hipStream_t s);
-// w -= V[0..K) hout[phase (kdim+1) ..]
+// Block partials of <V_j, w>, j < K, into part[j nvb + block] (the first
+// Gram-Schmidt reduction of an iteration; launch_orth_dots_batch forms the
+// later ones).
 hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const SusIter* it, const int2* vblocks,
                                     double* part, hipStream_t s);
 

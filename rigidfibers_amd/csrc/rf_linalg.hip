@@ -414,21 +414,6 @@ __global__ __launch_bounds__(kVecBlock) void dots_batch_kernel(int phase, int nv
   if (threadIdx.x == 0) part[static_cast<int64_t>(j) * nvb + blockIdx.x] = t;
 }
 
-// Sum of a suspension's block partials in table order -> hout.
-__global__ __launch_bounds__(64) void dots_final_batch_kernel(int phase, int nvb, int kdim,
-                                                              const SusDev* __restrict__ sus,
-                                                              const SusIter* __restrict__ it,
-                                                              const double* __restrict__ part) {
-  const int s = blockIdx.x, j = blockIdx.y;
-  const SusDev& d = sus[s];
-  const SusIter& c = it[s];
-  if (c.w == nullptr || (phase < 2 && j >= c.K) || (phase == 2 && j > 0)) return;
-  double v = 0.0;
-  for (int b = threadIdx.x; b < d.vb_count; b += 64) v += part[static_cast<int64_t>(j) * nvb + d.vb_first + b];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  if (threadIdx.x == 0) c.hout[phase * (kdim + 1) + j] = v;
-}
 
 
 // Sum of a suspension's block partials of row j, in block order (the same
@@ -527,10 +512,23 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
   const double* h0 = hsh;
   const double* h1 = hsh + K;
   const double w2 = hsh[2 * K];
-  int ycount = 0;
+  int zero_x = 0, ycount = 0;
   double* svec = nullptr;
   double sval = 1.0;
-  if (C.phase == 0) {  // w = b - A x
+  if (C.phase == 3) {  // w = b
+    C.bnorm = sqrt(w2);
+    if (!isfinite(C.bnorm)) {
+      C.nonfinite = 1;
+      C.phase = 2;
+    } else if (C.bnorm == 0.0) {  // x = 0 is the solution
+      zero_x = 1;
+      C.converged = 1;
+      C.phase = 2;
+    } else {
+      zero_x = C.cold;
+      C.phase = 0;
+    }
+  } else if (C.phase == 0) {  // w = b - A x
     const double beta = sqrt(w2);
     C.res = beta;
     if (!isfinite(beta)) {
@@ -591,7 +589,7 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
     }
   }
   SusIter nx{};
-  nx.hout = c.hout;
+  nx.zero_x = zero_x;
   nx.ycount = ycount;
   nx.ycoef = C.y;
   nx.svec = svec;
@@ -608,7 +606,7 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
     nx.K = C.k + 1;
   }
   c = nx;
-  active[q] = (C.phase != 2 || ycount > 0 || svec != nullptr) ? 1 : 0;
+  active[q] = (C.phase != 2 || zero_x || ycount > 0 || svec != nullptr) ? 1 : 0;
 }
 
 hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, const SusDev* sus,
@@ -626,15 +624,6 @@ hipError_t launch_iter_prep_batch(int nvb, const SusDev* sus, const SusIter* it,
   return hipGetLastError();
 }
 
-hipError_t launch_dots_batch(int nvb, int nsus, int kmax, int kdim, int phase, const SusDev* sus,
-                             const SusIter* it, const int2* vblocks, double* part, hipStream_t s) {
-  if (nvb <= 0 || nsus <= 0) return hipSuccess;
-  const int jy = (phase == 2) ? 1 : kmax;
-  if (jy <= 0) return hipSuccess;
-  hipLaunchKernelGGL(dots_batch_kernel, dim3(nvb, jy), dim3(kVecBlock), 0, s, phase, nvb, sus, it, vblocks, part);
-  hipLaunchKernelGGL(dots_final_batch_kernel, dim3(nsus, jy), dim3(64), 0, s, phase, nvb, kdim, sus, it, part);
-  return hipGetLastError();
-}
 
 hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const SusIter* it, const int2* vblocks,
                                     double* part, hipStream_t s) {

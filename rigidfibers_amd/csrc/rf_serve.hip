@@ -145,10 +145,8 @@ struct rf_server {
   int2* d_fblocks = nullptr;
   int2* h_fblocks = nullptr;
   double* d_part = nullptr;
-  double* d_hout = nullptr;
-  double* h_hout = nullptr;
   size_t c_dsus = 0, c_hsus = 0, c_dit = 0, c_hit = 0, c_dt = 0, c_ht = 0, c_dv = 0, c_hv = 0, c_df = 0, c_hf = 0,
-         c_part = 0, c_dh = 0, c_hh = 0;
+         c_part = 0;
   int ntargets = 0, nvb = 0, nfb = 0;
   rf::SusCtl* d_ctl = nullptr;
   rf::SusCtl* h_ctl = nullptr;
@@ -241,8 +239,6 @@ bool build_tables(rf_server* S) {
       !grow_host(&S->h_vblocks, &S->c_hv, nv) || !grow_device(&S->d_vblocks, &S->c_dv, nv) ||
       !grow_host(&S->h_fblocks, &S->c_hf, nf) || !grow_device(&S->d_fblocks, &S->c_df, nf) ||
       !grow_device(&S->d_part, &S->c_part, 3 * static_cast<size_t>(nv) * S->kdim) ||
-      !grow_host(&S->h_hout, &S->c_hh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
-      !grow_device(&S->d_hout, &S->c_dh, static_cast<size_t>(ns) * 3 * (S->kdim + 1)) ||
       !grow_host(&S->h_ctl, &S->c_hctl, ns) || !grow_device(&S->d_ctl, &S->c_dctl, ns) ||
       !grow_device(&S->d_ctlmem, &S->c_cmem,
                    static_cast<size_t>(ns) * ((S->m + 1) * static_cast<size_t>(S->m) + 4 * S->m + 1)) ||
@@ -360,8 +356,9 @@ bool ensure_iteration_graph(rf_server* S) {
   return true;
 }
 
-// One round of lockstep iterations (graph replay or eager); *any = whether a
-// suspension still has work afterwards.
+// One round of lockstep iterations (graph replay or eager), then one copy of
+// the control states and work flags; *any = whether a suspension still has
+// work afterwards.
 bool lockstep_round(rf_server* S, bool* any) {
   const int ns = static_cast<int>(S->active.size());
   hipStream_t st = S->stream;
@@ -371,6 +368,7 @@ bool lockstep_round(rf_server* S, bool* any) {
     return false;
   }
   if (hipMemcpyAsync(S->h_act, S->d_act, ns * sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(S->h_ctl, S->d_ctl, ns * sizeof(rf::SusCtl), hipMemcpyDeviceToHost, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess)
     return false;
   *any = false;
@@ -380,36 +378,15 @@ bool lockstep_round(rf_server* S, bool* any) {
 
 // GMRES(m) for every admitted suspension (the same algorithm as the
 // single-simulation solver: true residual at each restart, classical
-// Gram-Schmidt twice, Givens rotations), its control on the device: the host
-// sets up ||b|| and the first operations, then replays rounds of iterations
-// until no suspension has work left.
+// Gram-Schmidt twice, Givens rotations), its control on the device: the first
+// iteration measures ||b|| (w = b), the control step then sets x = 0 unless a
+// previous solution is the start and asks for the true residual; the host
+// replays rounds of iterations until no suspension has work left.
 bool solve_lockstep(rf_server* S) {
   const int ns = static_cast<int>(S->active.size());
-  const int hs = 3 * (S->kdim + 1);
   const int m = S->m;
   const int maxit = S->cfg.base.gmres_max_iterations;
   const bool warm_ok = S->cfg.base.initial_guess != RF_GUESS_ZERO;
-  // ||b|| of every suspension
-  for (int q = 0; q < ns; ++q) {
-    Suspension* s = S->active[q];
-    rf::SusIter& c = S->h_it[q];
-    c = rf::SusIter{};
-    c.w = s->b;
-    c.K = 0;
-    c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
-  }
-  {
-    hipStream_t st = S->stream;
-    if (hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
-        rf::launch_dots_batch(S->nvb, ns, 1, S->kdim, 2, S->d_sus, S->d_it, S->d_vblocks, S->d_part, st) !=
-            hipSuccess ||
-        hipMemcpyAsync(S->h_hout, S->d_hout, static_cast<size_t>(ns) * hs * sizeof(double), hipMemcpyDeviceToHost,
-                       st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return false;
-  }
-  // initial control state and first operations: x = 0 unless warm, then the
-  // true residual
   const int64_t per = static_cast<int64_t>(m + 1) * m + 4 * m + 1;
   for (int q = 0; q < ns; ++q) {
     Suspension* s = S->active[q];
@@ -421,26 +398,13 @@ bool solve_lockstep(rf_server* S) {
     C.sn = C.cs + m;
     C.g = C.sn + m;
     C.y = C.g + m + 1;
-    C.bnorm = std::sqrt(S->h_hout[static_cast<int64_t>(q) * hs + 2 * (S->kdim + 1)]);
+    C.phase = 3;
+    C.cold = !(warm_ok && s->have_solution);
     rf::SusIter& c = S->h_it[q];
     c = rf::SusIter{};
-    c.hout = S->d_hout + static_cast<int64_t>(q) * hs;
     c.ycoef = C.y;
-    c.zero_x = !(warm_ok && s->have_solution);
-    if (!std::isfinite(C.bnorm)) {
-      C.nonfinite = 1;
-      C.phase = 2;
-    } else if (C.bnorm == 0.0) {
-      c.zero_x = 1;
-      C.converged = 1;
-      C.phase = 2;
-    } else {
-      C.phase = 0;
-      c.gx = s->x;
-      c.gy = s->V;
-      c.gmode = 1;
-      c.w = s->V;
-    }
+    c.w = s->b;
+    c.K = 0;
   }
   hipStream_t st = S->stream;
   if (hipMemcpyAsync(S->d_ctl, S->h_ctl, ns * sizeof(rf::SusCtl), hipMemcpyHostToDevice, st) != hipSuccess ||
@@ -455,10 +419,7 @@ bool solve_lockstep(rf_server* S) {
     if (r >= max_rounds) return false;
     if (!lockstep_round(S, &any)) return false;
   }
-  if (hipMemcpyAsync(S->h_ctl, S->d_ctl, ns * sizeof(rf::SusCtl), hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess)
-    return false;
-  for (int q = 0; q < ns; ++q) {
+  for (int q = 0; q < ns; ++q) {  // h_ctl holds the last round's control states
     Suspension* s = S->active[q];
     const rf::SusCtl& C = S->h_ctl[q];
     s->have_solution = true;
@@ -599,11 +560,11 @@ void worker_main(rf_server* S) {
 void free_tables(rf_server* S) {
   for (void* p : {static_cast<void*>(S->d_sus), static_cast<void*>(S->d_it), static_cast<void*>(S->d_targets),
                   static_cast<void*>(S->d_vblocks), static_cast<void*>(S->d_fblocks), static_cast<void*>(S->d_part),
-                  static_cast<void*>(S->d_hout), static_cast<void*>(S->d_ctl), static_cast<void*>(S->d_ctlmem),
+                  static_cast<void*>(S->d_ctl), static_cast<void*>(S->d_ctlmem),
                   static_cast<void*>(S->d_act)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(S->h_sus), static_cast<void*>(S->h_it), static_cast<void*>(S->h_targets),
-                  static_cast<void*>(S->h_vblocks), static_cast<void*>(S->h_fblocks), static_cast<void*>(S->h_hout),
+                  static_cast<void*>(S->h_vblocks), static_cast<void*>(S->h_fblocks),
                   static_cast<void*>(S->h_ctl), static_cast<void*>(S->h_act)})
     if (p) (void)hipHostFree(p);
 }
