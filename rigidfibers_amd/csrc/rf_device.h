@@ -124,6 +124,50 @@ __device__ __forceinline__ void stokes_kernel(float rx, float ry, float rz, floa
   K[5] = cy * rz;
 }
 
+// log(P / Q) for normal P, Q > 0 without the division P / Q and without a
+// general logarithm: P = 2^eP mP, Q = 2^eQ mQ (exact), the mantissa ratio
+// brought into [sqrt(1/2), sqrt(2)) by one exact halving or doubling, then
+// log(P / Q) = e ln 2 + 2 atanh(y), y = (mP - mQ) / (mP + mQ), from the odd
+// Taylor series 2 sum_{k <= 11} y^(2k+1) / (2k+1): |y| <= 3 - 2 sqrt(2), so the
+// first omitted term is below 1e-18 of the sum.  mP - mQ is exact when the
+// ratio is near 1, so the result keeps its relative precision there (where
+// log(P / Q) of the rounded ratio would not).  Other inputs (zero, subnormal,
+// infinite, NaN) take the library logarithm of the ratio.
+__device__ __forceinline__ double log_ratio(double P, double Q) {
+  constexpr double kMin = 2.2250738585072014e-308;  // smallest normal double
+  constexpr double kMax = 1.7976931348623157e308;
+  if (!(P >= kMin && Q >= kMin && P <= kMax && Q <= kMax)) return log(P / Q);
+  int eP, eQ;
+  double mP = frexp(P, &eP), mQ = frexp(Q, &eQ);
+  const int adj = (mP > 1.4142135623730951 * mQ) - (mP < 0.70710678118654752 * mQ);
+  mP = ldexp(mP, -adj);
+  const int e = eP - eQ + adj;
+  const double num = mP - mQ, den = mP + mQ;
+  // y = num / den: reciprocal refined twice, then one residual correction
+  double r = __builtin_amdgcn_rcp(den);
+  r = fma(fma(-den, r, 1.0), r, r);
+  r = fma(fma(-den, r, 1.0), r, r);
+  double y = num * r;
+  y = fma(fma(-den, y, num), r, y);
+  const double z = y * y;
+  double p = 1.0 / 23.0;
+  p = fma(p, z, 1.0 / 21.0);
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const double t = 2.0 * y;
+  const double tail = fma(t * z, p, t);  // 2 y (1 + z p)
+  // ln 2 as a double plus the remainder, so e ln 2 loses nothing for e != 0
+  const double de = static_cast<double>(e);
+  return fma(de, 0.69314718055994529, fma(de, 2.3190468138462996e-17, tail));
+}
+
 // Moments I1_n, I3_n, I5_n (n = 0..N+2) of int_{-1}^{1} s^n (s^2 + b s + l)^{-m/2} ds
 // as the reference computes them: closed forms for n = 0, 1 (collinear branch
 // when l - b^2/4 < 1e-7), forward recurrence when l < 10, else a backward
@@ -131,15 +175,15 @@ __device__ __forceinline__ void stokes_kernel(float rx, float ry, float rz, floa
 template <int N>
 __device__ __forceinline__ void moments(double b, double l, double I1[N + 3], double I3[N + 3], double I5[N + 3]) {
   // Divisions by the recurrence's integer constants are multiplications by
-  // reciprocals folded at compile time; the remaining divisions are 1/m, 1/uu,
-  // 1/ul, 1/l and the ratio inside the single logarithm.
+  // reciprocals folded at compile time; the remaining divisions are 1/m and
+  // 1/l (1/pu, 1/pl on the collinear branch), the logarithm is log_ratio.
   const double m = l - 0.25 * b * b;
   const double qu = 1.0 + b + l, ql = 1.0 - b + l;  // |X(+-1) - x_j|^2 > 0 for distinct fibers
   const double iuu = rsqrt(qu), iul = rsqrt(ql);
   const double uu = qu * iuu, ul = ql * iul;
   const double pu = 2.0 + b, pl = -2.0 + b;
   // log|pu + 2uu| - log|pl + 2ul|
-  I1[0] = log(fabs((pu + 2.0 * uu) / (pl + 2.0 * ul)));
+  I1[0] = log_ratio(fabs(pu + 2.0 * uu), fabs(pl + 2.0 * ul));
   I1[1] = uu - ul - 0.5 * b * I1[0];
   if (m < 1e-7) {
     const double ipu = 1.0 / pu, ipl = 1.0 / pl;
@@ -176,30 +220,28 @@ __device__ __forceinline__ void moments(double b, double l, double I1[N + 3], do
     // its largest term (the reference's own start at 27 still moves them by
     // ~2e-7 for l in [10, 20], so small l keeps 27).
     const int start = l < 100.0 ? 27 : l < 200.0 ? 24 : l < 400.0 ? 22 : l < 1000.0 ? 20 : l < 4000.0 ? 18 : 16;
-    double i1n1 = 0.0, i1n2 = 0.0, i3n1 = 0.0, i3n2 = 0.0, i5n1 = 0.0, i5n2 = 0.0;
+    // I_n of each family in slot n % 3: a step writes its own slot and reads
+    // the other two, so a skipped step (a prefix of the sweep) leaves a zero
+    // and the sweep needs no register copies.
+    double r1[3] = {0.0, 0.0, 0.0}, r3[3] = {0.0, 0.0, 0.0}, r5[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 27; n >= 2; --n) {
       if (n > 16 && n > start) continue;  // still all zeros
+      const int k0 = n % 3, k1 = (n + 1) % 3, k2 = (n + 2) % 3;
       const double rn = n;
       // s^(n+1) at s = -1 is +1 for odd n
       const double d = (n & 1) ? dm : dp;
       const double cA = 1.0 / (rn + 1.0);                                                   // folded
       const double cB = (rn + 2.0) / (rn + 1.0) * ((1.0 - 2.0 * (rn + 2.0)) / (2.0 * (rn + 2.0)));  // folded
       const double cC = (rn + 2.0) / (rn + 1.0);                                             // folded
-      const double i1n0 = il * (cA * d + cB * b * i1n1 - cC * i1n2);
-      const double i3n0 = il * (i1n0 - b * i3n1 - i3n2);
-      const double i5n0 = il * (i3n0 - b * i5n1 - i5n2);
+      r1[k0] = il * (cA * d + cB * b * r1[k1] - cC * r1[k2]);
+      r3[k0] = il * (r1[k0] - b * r3[k1] - r3[k2]);
+      r5[k0] = il * (r3[k0] - b * r5[k1] - r5[k2]);
       if (n < N + 3) {
-        I1[n] = i1n0;
-        I3[n] = i3n0;
-        I5[n] = i5n0;
+        I1[n] = r1[k0];
+        I3[n] = r3[k0];
+        I5[n] = r5[k0];
       }
-      i1n2 = i1n1;
-      i1n1 = i1n0;
-      i3n2 = i3n1;
-      i3n1 = i3n0;
-      i5n2 = i5n1;
-      i5n1 = i5n0;
     }
   }
 }
