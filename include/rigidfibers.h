@@ -127,7 +127,10 @@ typedef struct rf_config {
   int32_t lu_gemm;                                    /* mixed LUs' trailing updates: 0 hipBLASLt when it has an
                                                          algorithm (else the built-in kernel), 1 the built-in
                                                          MFMA kernel always */
-  int32_t reserved[3];
+  int32_t separate_residual;                          /* iterative solvers on the theta form: 0 the first residual
+                                                         b - A x0 of a warm start comes from the assembly kernel
+                                                         (one matvec fewer per solve), 1 from a matvec */
+  int32_t reserved[2];
 } rf_config;
 
 typedef struct rf_solver_stats {

@@ -62,7 +62,8 @@ class RfConfig(C.Structure):
         ("velocity_operator", C.c_int32),
         ("inner_tolerance", C.c_double),
         ("lu_gemm", C.c_int32),
-        ("reserved", C.c_int32 * 3),
+        ("separate_residual", C.c_int32),
+        ("reserved", C.c_int32 * 2),
     ]
 
 

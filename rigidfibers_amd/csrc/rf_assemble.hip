@@ -276,6 +276,7 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
 // arrays s = xx yy zz xy xz yz at stride Lp: b = 0 writes xx xy xz, b = 1 yy yz,
 // b = 2 zz, so the three column jobs together write each component once.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
 
 // The work of one workgroup: target fiber i, source group gy.
 //   phase 1  lane (p, q_i) writes G_m(q_i) of source slot p into LDS as row
@@ -289,11 +290,17 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 //   phase 3  each lane writes 4 consecutive (source, mode) columns of one
 //            component array (16-byte stores); the k_i = 0 components
 //            (Theta_0 t_i) and the RHS of each source slot are formed here.
+// With x0, the workgroup then reads back the rows it stored and forms its
+// share of the off-diagonal product, sum_{j in group} A_ij x0_j, into
+// apart[gy n + row] (the theta GEMV's arithmetic: FP32 values times FP64
+// x0, FP64 sums).  The read is HBM traffic while the other workgroups of the
+// CU keep its VALUs busy, which is what makes it cheaper than a matvec.
 template <int N, int NQI, bool ANALYTIC>
 __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64_t Lp, const double* __restrict__ pos,
                                                      const double* __restrict__ ori, float* __restrict__ T,
                                                      double* __restrict__ bpart, int jgroup, const PhysConsts& pc,
-                                                     float* __restrict__ Tv, double* __restrict__ cpart) {
+                                                     float* __restrict__ Tv, double* __restrict__ cpart,
+                                                     const double* __restrict__ x0, double* __restrict__ apart) {
   constexpr Quad<N, NQI> quad{};
   constexpr int Q = 3 * NQI;
   constexpr int PAIRS = kPairThreads / Q;
@@ -457,6 +464,97 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     for (int q = 0; q < PAIRS; ++q) s += bred[q][tid];
     cpart[(static_cast<int64_t>(gy) * M + i) * 6 + (tid - ROWS)] = s;
   }
+
+  if (x0 != nullptr) {  // the stores above are visible to the workgroup after the barrier
+    constexpr int NACC = 1 + 3 * (N - 1);  // s0, then U_k (k >= 1) as in gemv_theta_block
+    double acc[NACC];
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+    // column (i-row block, e): s0 += Q0 . x_e, U_k += Theta_k x_e
+    auto add_column = [&](const float* q, double xa, double xb, double xc) {
+      acc[0] = fma(static_cast<double>(q[0]), xa,
+                   fma(static_cast<double>(q[Lp]), xb, fma(static_cast<double>(q[2 * Lp]), xc, acc[0])));
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        const float* qk = q + (3 + 6 * (k - 1)) * Lp;
+        const double txx = qk[0], tyy = qk[Lp], tzz = qk[2 * Lp], txy = qk[3 * Lp], txz = qk[4 * Lp],
+                     tyz = qk[5 * Lp];
+        double* u = acc + 1 + 3 * (k - 1);
+        u[0] = fma(txx, xa, fma(txy, xb, fma(txz, xc, u[0])));
+        u[1] = fma(txy, xa, fma(tyy, xb, fma(tyz, xc, u[1])));
+        u[2] = fma(txz, xa, fma(tyz, xb, fma(tzz, xc, u[2])));
+      }
+    };
+    // Pairs of consecutive columns per lane with 8-byte loads (the group's
+    // first column jb N is even -- jgroup is a multiple of the sources per
+    // sweep, a multiple of 4 -- and Lp a multiple of 64, so the loads are
+    // aligned), then the remaining column one per lane.
+    const int ebeg = jb * N, eend = je * N;
+    const int efull = ebeg + (eend - ebeg) / 2 * 2;
+    for (int e0 = ebeg + 2 * tid; e0 < efull; e0 += 2 * kPairThreads) {
+      double xv[2][3];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) xv[c][a] = x0[3 * static_cast<int64_t>(e0 + c) + a];
+      const float* q = Ti + e0;
+      auto ld = [&](int comp) { return *reinterpret_cast<const floatx2*>(q + static_cast<int64_t>(comp) * Lp); };
+      {
+        const floatx2 q0 = ld(0), q1 = ld(1), q2 = ld(2);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+          acc[0] = fma(static_cast<double>(q0[c]), xv[c][0],
+                       fma(static_cast<double>(q1[c]), xv[c][1], fma(static_cast<double>(q2[c]), xv[c][2], acc[0])));
+      }
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        const int base = 3 + 6 * (k - 1);
+        const floatx2 vxx = ld(base), vyy = ld(base + 1), vzz = ld(base + 2), vxy = ld(base + 3),
+                      vxz = ld(base + 4), vyz = ld(base + 5);
+        double* u = acc + 1 + 3 * (k - 1);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const double txx = vxx[c], tyy = vyy[c], tzz = vzz[c], txy = vxy[c], txz = vxz[c], tyz = vyz[c];
+          const double xa = xv[c][0], xb = xv[c][1], xc = xv[c][2];
+          u[0] = fma(txx, xa, fma(txy, xb, fma(txz, xc, u[0])));
+          u[1] = fma(txy, xa, fma(tyy, xb, fma(tyz, xc, u[1])));
+          u[2] = fma(txz, xa, fma(tyz, xb, fma(tzz, xc, u[2])));
+        }
+      }
+    }
+    for (int e = efull + tid; e < eend; e += kPairThreads)
+      add_column(Ti + e, x0[3 * static_cast<int64_t>(e)], x0[3 * static_cast<int64_t>(e) + 1],
+                 x0[3 * static_cast<int64_t>(e) + 2]);
+    constexpr int WAVES = kPairThreads / 64;
+    static_assert(WAVES * NACC * sizeof(double) <= RP * Q * sizeof(float), "partials fit the freed G");
+    double* red = reinterpret_cast<double*>(G);  // G is free after the last sweep
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) {
+      double v = acc[a];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[wave * NACC + a] = v;
+    }
+    __syncthreads();
+    if (tid < ROWS) {
+      const int k = tid / 3, a = tid - 3 * k;
+      auto sum = [&](int idx) {
+        double t = 0.0;
+        for (int w = 0; w < WAVES; ++w) t += red[w * NACC + idx];
+        return t;
+      };
+      double out;
+      if (k == 0) {
+        out = static_cast<double>(pc.D1) * tid_[a] * sum(0);
+      } else {
+        const double U[3] = {sum(1 + 3 * (k - 1)), sum(2 + 3 * (k - 1)), sum(3 + 3 * (k - 1))};
+        const double tU = tid_[0] * U[0] + tid_[1] * U[1] + tid_[2] * U[2];
+        out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * tid_[a] * tU);
+      }
+      apart[static_cast<int64_t>(gy) * n + static_cast<int64_t>(ROWS) * i + tid] = out;
+    }
+  }
 }
 
 // Waves per SIMD the theta kernel asks the register allocator for: as many
@@ -475,8 +573,10 @@ constexpr int theta_min_waves() {
 template <int N, int NQI, bool ANALYTIC>
 __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI, ANALYTIC>(), 8))) void assemble_theta_kernel(
     int M, int64_t Lp, const double* __restrict__ pos, const double* __restrict__ ori, float* __restrict__ T,
-    double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart) {
-  assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart);
+    double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart,
+    const double* __restrict__ x0, double* __restrict__ apart) {
+  assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart, x0,
+                                         apart);
 }
 
 // Serving: workgroup = (suspension, target fiber), all sources in one group,
@@ -486,7 +586,8 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(th
     const SusDev* __restrict__ sus, const int2* __restrict__ targets, PhysConsts pc) {
   const int2 e = targets[blockIdx.x];
   const SusDev& d = sus[e.x];
-  assemble_theta_block<N, NQI, ANALYTIC>(e.y, 0, d.M, d.Lp, d.pos, d.ori, d.T, d.b, d.M, pc, d.Tv, d.cpart);
+  assemble_theta_block<N, NQI, ANALYTIC>(e.y, 0, d.M, d.Lp, d.pos, d.ori, d.T, d.b, d.M, pc, d.Tv, d.cpart, nullptr,
+                                         nullptr);
 }
 
 // Dense A (column-major, leading dimension n) from the theta form; identity
@@ -536,6 +637,16 @@ __global__ void sum_groups_kernel(int groups, int64_t len, const double* __restr
   double s = 0.0;
   for (int g = 0; g < groups; ++g) s += part[static_cast<int64_t>(g) * len + r];
   out[r] = s;
+}
+
+__global__ void residual_groups_kernel(int groups, int64_t len, const double* __restrict__ apart,
+                                       const double* __restrict__ b, const double* __restrict__ x0,
+                                       double* __restrict__ r) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= len) return;
+  double s = 0.0;
+  for (int g = 0; g < groups; ++g) s += apart[static_cast<int64_t>(g) * len + k];
+  r[k] = b[k] - (s + x0[k]);  // identity diagonal blocks
 }
 
 PairGrid pair_grid(int M, int pairs_per_tile) {
@@ -624,33 +735,34 @@ hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, 
 
 template <int N, int NQI, bool AN>
 static hipError_t launch_theta_t(int M, int64_t Lp, const double* pos, const double* ori, float* T, double* bpart,
-                                 float* Tv, double* cpart, const PhysConsts& pc, hipStream_t s, int* groups) {
+                                 float* Tv, double* cpart, const double* x0, double* apart, const PhysConsts& pc,
+                                 hipStream_t s, int* groups) {
   constexpr int PAIRS = kPairThreads / (3 * NQI);
   const PairGrid g = theta_grid(M, PAIRS);
   *groups = g.groups;
   hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M, Lp, pos,
-                     ori, T, bpart, g.jgroup, pc, Tv, cpart);
+                     ori, T, bpart, g.jgroup, pc, Tv, cpart, x0, apart);
   return hipGetLastError();
 }
 
 template <int N, int NQI>
 static hipError_t launch_theta_n(bool an, int M, int64_t Lp, const double* pos, const double* ori, float* T,
-                                 double* bpart, float* Tv, double* cpart, const PhysConsts& pc, hipStream_t s,
-                                 int* groups) {
-  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups)
-            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups);
+                                 double* bpart, float* Tv, double* cpart, const double* x0, double* apart,
+                                 const PhysConsts& pc, hipStream_t s, int* groups) {
+  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups)
+            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups);
 }
 
 #define RF_TH_CASE(NN)                                                                             \
   case NN:                                                                                         \
-    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
-    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
-    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, pc, s, groups); \
+    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
+    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
+    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
     return hipErrorInvalidValue;
 
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
                                  const double* ori, float* T, double* bpart, float* Tv, double* cpart,
-                                 const PhysConsts& pc, hipStream_t s, int* groups) {
+                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups) {
   switch (N) {
     RF_TH_CASE(1)
     RF_TH_CASE(2)
@@ -724,6 +836,15 @@ hipError_t launch_assemble_theta_batch(int N, int NQI, bool analytical, int ntar
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t launch_residual_groups(int groups, int64_t len, const double* apart, const double* b, const double* x0,
+                                  double* r, hipStream_t s) {
+  const int threads = 256;
+  const int64_t blocks = (len + threads - 1) / threads;
+  hipLaunchKernelGGL(residual_groups_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, s, groups, len,
+                     apart, b, x0, r);
+  return hipGetLastError();
 }
 
 hipError_t launch_sum_groups(int groups, int64_t len, const double* part, double* out, hipStream_t s) {

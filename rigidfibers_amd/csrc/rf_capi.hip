@@ -87,6 +87,13 @@ struct rf_simulation {
   double* b = nullptr;
   double* x = nullptr;
   double* bpart = nullptr;
+  // first residual of a warm-started iterative solve, formed with the
+  // assembly (rf_config.separate_residual == 0 on the theta form)
+  double* apart = nullptr;   // off-diagonal products A x0, one slot per source group
+  double* r0 = nullptr;      // b - A x0
+  bool guess_taken = false;  // the assembly took this solve's initial guess
+  bool guess_warm = false;
+  bool r0_ready = false;     // r0 belongs to the current system and guess
   float* wf = nullptr;
   double* fm = nullptr;
   double* vpart = nullptr;
@@ -158,6 +165,8 @@ rf_status validate_config(const rf_config* c) {
   if (c->matrix_format < RF_MATRIX_AUTO || c->matrix_format > RF_MATRIX_THETA)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown matrix format");
   if (c->lu_gemm != 0 && c->lu_gemm != 1) return fail(RF_ERR_INVALID_ARGUMENT, "lu_gemm must be 0 or 1");
+  if (c->separate_residual != 0 && c->separate_residual != 1)
+    return fail(RF_ERR_INVALID_ARGUMENT, "separate_residual must be 0 or 1");
   if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS &&
       c->initial_guess != RF_GUESS_EXTRAPOLATED)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
@@ -267,9 +276,10 @@ rf_status device_dot(rf_simulation* s, const double* u, const double* v, double*
 // Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
 // GMRES(m) for A x = b: A the FP32 system, or its FP16 copy (half).  Starts
 // from x when warm, else from 0; stops at ||b - A x|| <= tol ||b|| or after
-// maxit iterations.  *out receives the iterations, convergence and residual.
+// maxit iterations.  r0, when given, is b - A x of the warm start.  *out
+// receives the iterations, convergence and residual.
 rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, double tol, int maxit, bool half,
-                     rf_solver_stats* out) {
+                     rf_solver_stats* out, const double* r0 = nullptr) {
   const int64_t n = s->n;
   const int m = s->cfg.gmres_restart;
   double* V = s->V;
@@ -296,6 +306,10 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
       RF_HIP(rf::axpby(n, 1.0, b, 0.0, Vk(0), s->stream));
       beta = bnorm;
       first = false;
+    } else if (first && r0) {
+      first = false;
+      RF_HIP(rf::axpby(n, 1.0, r0, 0.0, Vk(0), s->stream));
+      RF_TRY(device_norm(s, Vk(0), &beta));
     } else {
       first = false;
       RF_HIP(sim_gemv(s, x, s->gpart, Vk(0), b, 1, half));
@@ -385,10 +399,27 @@ rf_status initial_guess(rf_simulation* s, bool* warm) {
   return RF_OK;
 }
 
+// The initial guess of an iterative solve: taken by the assembly when it
+// formed the first residual with the system (*r0 then points at it), else
+// here.
+rf_status take_guess(rf_simulation* s, bool* warm, const double** r0) {
+  *r0 = nullptr;
+  if (s->guess_taken) {
+    *warm = s->guess_warm;
+    if (s->r0_ready && s->guess_warm) *r0 = s->r0;
+    s->guess_taken = false;
+    s->r0_ready = false;
+    return RF_OK;
+  }
+  return initial_guess(s, warm);
+}
+
 rf_status solve_gmres(rf_simulation* s) {
   bool warm = false;
-  RF_TRY(initial_guess(s, &warm));
-  return gmres_core(s, s->b, s->x, warm, s->cfg.gmres_tolerance, s->cfg.gmres_max_iterations, false, &s->stats);
+  const double* r0 = nullptr;
+  RF_TRY(take_guess(s, &warm, &r0));
+  return gmres_core(s, s->b, s->x, warm, s->cfg.gmres_tolerance, s->cfg.gmres_max_iterations, false, &s->stats,
+                    r0);
 }
 
 // GMRES-IR: x <- x + d with d from GMRES on the FP16 copy, A16 d = r, each
@@ -405,7 +436,8 @@ rf_status solve_gmres_ir(rf_simulation* s) {
   double bnorm = 0.0;
   RF_TRY(device_norm(s, s->b, &bnorm));
   bool warm = false;
-  RF_TRY(initial_guess(s, &warm));
+  const double* r0 = nullptr;
+  RF_TRY(take_guess(s, &warm, &r0));
   if (!warm || bnorm == 0.0) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
   s->stats = rf_solver_stats{0, 0, 0.0};
   if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
@@ -415,7 +447,11 @@ rf_status solve_gmres_ir(rf_simulation* s) {
   }
   int it = 0;
   while (true) {
-    RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1, false));
+    if (r0)
+      RF_HIP(rf::axpby(n, 1.0, r0, 0.0, r, s->stream));
+    else
+      RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1, false));
+    r0 = nullptr;
     double res = 0.0;
     RF_TRY(device_norm(s, r, &res));
     if (!std::isfinite(res)) return fail(RF_ERR_NONFINITE, "non-finite residual in GMRES-IR");
@@ -449,7 +485,8 @@ rf_status solve_bicgstab(rf_simulation* s) {
   double bnorm = 0.0;
   RF_TRY(device_norm(s, s->b, &bnorm));
   bool warm = false;
-  RF_TRY(initial_guess(s, &warm));
+  const double* r0x = nullptr;
+  RF_TRY(take_guess(s, &warm, &r0x));
   s->stats = rf_solver_stats{0, 0, 0.0};
   if (!std::isfinite(bnorm)) return fail(RF_ERR_NONFINITE, "non-finite right-hand side");
   if (bnorm == 0.0 || !warm) RF_HIP(rf::fill(n, 0.0, s->x, s->stream));
@@ -459,7 +496,10 @@ rf_status solve_bicgstab(rf_simulation* s) {
   }
   double rho, res;
   if (warm) {
-    RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
+    if (r0x)
+      RF_HIP(rf::axpby(n, 1.0, r0x, 0.0, r, s->stream));
+    else
+      RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
     RF_TRY(device_norm(s, r, &res));
     rho = res * res;
   } else {
@@ -652,6 +692,10 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
   if (st == RF_OK) st = s->alloc(&s->x, static_cast<size_t>(n));
   if (st == RF_OK && cfg->initial_guess == RF_GUESS_EXTRAPOLATED) st = s->alloc(&s->xprev, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->bpart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
+  if (s->theta && !cfg->separate_residual) {
+    if (st == RF_OK) st = s->alloc(&s->apart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
+    if (st == RF_OK) st = s->alloc(&s->r0, static_cast<size_t>(n));
+  }
   if (st == RF_OK) st = s->alloc(&s->vpart, static_cast<size_t>(rf::max_pair_groups(M)) * M * 6);
   if (st == RF_OK) st = s->alloc(&s->wf, static_cast<size_t>(M) * 3 * 3 * s->NQI);
   if (st == RF_OK) st = s->alloc(&s->fm, static_cast<size_t>(M) * 3 * (s->N + 1));
@@ -705,9 +749,21 @@ void rf_destroy(rf_simulation* s) {
 
 rf_status rf_assemble_system(rf_simulation* s) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  const double* x0 = nullptr;
+  s->r0_ready = false;
   if (s->theta) {
+    if (s->apart) {  // the solve's initial guess now, its residual with the system
+      if (!s->guess_taken) {
+        bool warm = false;
+        RF_TRY(initial_guess(s, &warm));
+        s->guess_taken = true;
+        s->guess_warm = warm;
+      }
+      if (s->guess_warm) x0 = s->x;
+    }
     RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
-                                     s->bpart, s->Tv, s->cpart, s->pc, s->stream, &s->bgroups));
+                                     s->bpart, s->Tv, s->cpart, x0, x0 ? s->apart : nullptr, s->pc, s->stream,
+                                     &s->bgroups));
     s->vvalid = s->vstored;
     if (s->T16) RF_HIP(rf::theta_to_fp16(s->N, s->M, s->Lp, s->T, s->T16, s->S16, s->stream));
   } else {
@@ -716,6 +772,10 @@ rf_status rf_assemble_system(rf_simulation* s) {
     s->bgroups = rf::pair_grid(s->M, 192 / (3 * s->NQI)).groups;
   }
   RF_HIP(rf::launch_sum_groups(s->bgroups, s->n, s->bpart, s->b, s->stream));
+  if (x0) {
+    RF_HIP(rf::launch_residual_groups(s->bgroups, s->n, s->apart, s->b, x0, s->r0, s->stream));
+    s->r0_ready = true;
+  }
   return RF_OK;
 }
 
@@ -863,6 +923,8 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
   s->have_next = false;
   s->vvalid = false;
   s->have_previous = false;  // extrapolation needs two solutions of consecutive steps
+  s->guess_taken = false;    // a residual formed with the old state is void
+  s->r0_ready = false;
   return RF_OK;
 }
 

@@ -36,7 +36,12 @@ PairGrid theta_grid(int M, int sources_per_iteration);
 int64_t theta_stride(int M, int N);
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
                                  const double* ori, float* T, double* bpart, float* Tv, double* cpart,
-                                 const PhysConsts& pc, hipStream_t s, int* groups);
+                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups);
+// With x0, the theta assembly also writes each group's off-diagonal product
+// rows sum_{j in group} A_ij x0_j into apart[group len + row].
+// r = b - (x0 + sum_g apart[g]): the residual of x0 from those partials.
+hipError_t launch_residual_groups(int groups, int64_t len, const double* apart, const double* b, const double* x0,
+                                  double* r, hipStream_t s);
 // Velocities from the stored operator of the last theta assembly (one pass
 // over Tv): tvel, rvel [3*M] FP64.
 hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, const double* cpart, int groups,

@@ -427,6 +427,49 @@ def test_extrapolated_guess(oracle64, solver):
     assert ep < 1e-5 and eo < 1e-5
 
 
+@pytest.mark.parametrize("integration,solver,N,nqi,M", [("numerical", "gmres", 5, 8, 120),
+                                                         ("analytical", "gmres", 5, 8, 120),
+                                                         ("numerical", "bicgstab", 5, 8, 120),
+                                                         ("analytical", "gmres-ir", 5, 8, 120),
+                                                         ("numerical", "gmres", 6, 4, 120),
+                                                         ("analytical", "gmres", 5, 8, 4500)])
+def test_residual_from_assembly(integration, solver, N, nqi, M):
+    """The first residual b - A x0 of a warm-started solve, formed by the
+    theta assembly kernel (each workgroup reads back the rows it stored and
+    multiplies them with x0), gives the solve that the separate matvec gives.
+    The iterations are capped, so a wrong residual could not be repaired by a
+    later restart: one warm step must agree to rounding in solution,
+    velocities and reported residual, with one matvec fewer.  M = 120 splits
+    the sources over 15 workgroups per target (partial products summed),
+    M = 4500 gives one workgroup per target as at the large sizes."""
+    from rigidfibers_amd.scenes import cube
+
+    pos, ori = cube(M, 6.0 * (M / 120) ** (1 / 3), min_dist=0.4, seed=5)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.NUMBER_OF_TERMS_IN_FORCE_EXPANSION = N
+    params.NUMBER_OF_QUADRATURE_INTERVALS = nqi
+    params.GMRES_RESTART = 10
+    params.GMRES_MAX_ITERATIONS = 3
+    params.BICGSTAB_MAX_ITERATIONS = 2
+    out = {}
+    for separate in (False, True):
+        with _sim(params, pos, ori, solver=solver, integration=integration, separate_residual=separate) as sim:
+            sim.step(0)  # cold start: no residual product
+            sim.step(1)
+            _, _, x = sim.system()
+            tv, rv = sim.velocities()
+            out[separate] = (x.copy(), tv.copy(), rv.copy(), sim.solver_stats(), sim.timings()["gemv_launches"])
+    (x0, t0, r0, s0, g0), (x1, t1, r1, s1, g1) = out[False], out[True]
+    ex, ev, ew = rel_err(x0, x1), rel_err(t0, t1), rel_err(r0, r1)
+    print(f"{integration} {solver} N={N} nqi={nqi} M={M}: x {ex:.1e} v {ev:.1e} w {ew:.1e}, residuals "
+          f"{s0['relative_residual']:.3e} / {s1['relative_residual']:.3e}, matvecs {g0} / {g1}")
+    assert ex < 1e-9 and ev < 1e-9 and ew < 1e-9
+    assert s0["iterations"] == s1["iterations"]
+    assert abs(s0["relative_residual"] - s1["relative_residual"]) <= 1e-6 * s1["relative_residual"]
+    assert g0 == g1 - 1
+
+
 @pytest.mark.parametrize("analytical", [False, True])
 def test_theta_form_matches_dense(analytical):
     """The theta-form system (the iterative solvers' storage) expands to the
