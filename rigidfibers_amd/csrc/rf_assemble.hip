@@ -317,6 +317,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ __align__(16) float G[RP * Q];
   __shared__ __align__(16) float Pj[NM * RP];
   __shared__ double bred[PAIRS][NR];
+  __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -333,8 +334,15 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
   float* Vi = Tv ? Tv + static_cast<int64_t>(i) * 12 * Lp : nullptr;
 
-  // RHS (and velocity-constant) sums of each source slot, kept in LDS
+  // RHS (and velocity-constant) sums of each source slot, kept in LDS; the
+  // mode factors gamma_k, eigen_k as a table the RHS lanes index by k
   for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
+#pragma unroll
+  for (int t = 0; t < kMaxTerms; ++t)
+    if (tid == t) {
+      modes[t] = pc.gamma[t];
+      modes[kMaxTerms + t] = pc.eigen[t];
+    }
   __syncthreads();
 
   const int nu = NC + (Vi ? 12 : 0);  // component arrays written per sweep
@@ -370,49 +378,66 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     }
     __syncthreads();
     const int npm = min(PAIRS, je - j0) * N;  // valid columns of this sweep
-    const int sself = (i - j0) * N;           // the self pair's columns, if in this sweep
-    const int64_t ecol = static_cast<int64_t>(j0) * N;
-    for (int job = tid; job < nu * NV + PAIRS * (N + 1); job += kPairThreads) {
-      if (job < nu * NV) {
-        const int u = job / NV;
-        const int pm0 = 4 * (job - u * NV);
-        floatx4 val;
-        if (u < 3) {  // Q0_b = (Theta_0 t_i)_b, Theta_0 = the P_1 projection
-          const float* b1 = Pj + RP + PAIRS + pm0;
-          const floatx4 x0 = *reinterpret_cast<const floatx4*>(b1 + sym_index(0, u) * CB);
-          const floatx4 x1 = *reinterpret_cast<const floatx4*>(b1 + sym_index(1, u) * CB);
-          const floatx4 x2 = *reinterpret_cast<const floatx4*>(b1 + sym_index(2, u) * CB);
-#pragma unroll
-          for (int e = 0; e < 4; ++e) val[e] = x0[e] * ti[0] + x1[e] * ti[1] + x2[e] * ti[2];
-        } else {
-          int kk, c;
-          if (u < NC) {  // Theta_{k_i}, k_i = 1..N-1: projection kk = k_i + 1
-            kk = 2 + (u - 3) / 6;
-            c = (u - 3) - 6 * (kk - 2);
-          } else {  // velocity operator: Theta_0 (kk = 1), then Phi_0 (kk = 0)
-            const int w = u - NC;
-            kk = w < 6 ? 1 : 0;
-            c = w < 6 ? w : w - 6;
-          }
-          val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
-        }
+    const bool full = npm == PN;                // every sweep but possibly the last
+    const bool self_here = i >= j0 && i < j0 + PAIRS;  // the self pair's columns are in this sweep
+    const int sself = (i - j0) * N;
+    float* const Tcol = Ti + static_cast<int64_t>(j0) * N;  // this sweep's first column, component 0
+    // 4 consecutive columns of one component array (offset `off` from the
+    // sweep's first column of component 0): self-pair columns zeroed, columns
+    // past the sweep's last source not written.  The conditions are uniform
+    // over the workgroup except in the sweep that holds them.
+    auto put = [&](float* base, uint32_t off, int pm0, floatx4 val) {
+      if (self_here) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if (static_cast<unsigned>(pm0 + e - sself) < static_cast<unsigned>(N)) val[e] = 0.0f;
-        float* dst = (u < NC ? Ti + u * Lp : Vi + (u - NC) * Lp) + ecol + pm0;
-        if (pm0 + 3 < npm) {
-          *reinterpret_cast<floatx4*>(dst) = val;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (pm0 + e < npm) dst[e] = val[e];
-        }
-        continue;
       }
-      // RHS job (source slot p2, row block k): k < N forms rows 3k..3k+2 of
-      // b from the projection of G_0 F/2 onto w P_{k+1}; k = N the velocity
-      // constants sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2.
-      const int rj = job - nu * NV;
+      float* dst = base + off;
+      if (full || pm0 + 3 < npm) {
+        *reinterpret_cast<floatx4*>(dst) = val;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (pm0 + e < npm) dst[e] = val[e];
+      }
+    };
+    const uint32_t lp32 = static_cast<uint32_t>(Lp);
+    // system components: u < 3 Q0_u = (Theta_0 t_i)_u (Theta_0 = the P_1
+    // projection), then Theta_{k_i} for k_i = 1..N-1 (projection k_i + 1)
+    for (int job = tid; job < NC * NV; job += kPairThreads) {
+      const int u = job / NV;
+      const int pm0 = 4 * (job - u * NV);
+      floatx4 val;
+      if (u < 3) {
+        const float* b1 = Pj + RP + PAIRS + pm0;
+        const floatx4 x0 = *reinterpret_cast<const floatx4*>(b1 + sym_index(0, u) * CB);
+        const floatx4 x1 = *reinterpret_cast<const floatx4*>(b1 + sym_index(1, u) * CB);
+        const floatx4 x2 = *reinterpret_cast<const floatx4*>(b1 + sym_index(2, u) * CB);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) val[e] = x0[e] * ti[0] + x1[e] * ti[1] + x2[e] * ti[2];
+      } else {
+        const int kk = 2 + (u - 3) / 6;
+        const int c = (u - 3) - 6 * (kk - 2);
+        val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
+      }
+      put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
+    }
+    // velocity operator: Theta_0 (projection 1), then Phi_0 (projection 0)
+    if (Vi) {
+      float* const Vcol = Vi + static_cast<int64_t>(j0) * N;
+      for (int job = tid; job < 12 * NV; job += kPairThreads) {
+        const int w = job / NV;
+        const int pm0 = 4 * (job - w * NV);
+        const int kk = w < 6 ? 1 : 0;
+        const int c = w < 6 ? w : w - 6;
+        const floatx4 val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
+        put(Vcol, static_cast<uint32_t>(w) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
+      }
+    }
+    // RHS (source slot p2, row block k): k < N forms rows 3k..3k+2 of b from
+    // the projection of G_0 F/2 onto w P_{k+1}; k = N the velocity constants
+    // sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2.
+    for (int rj = tid; rj < PAIRS * (N + 1); rj += kPairThreads) {
       const int p2 = rj / (N + 1), k = rj - p2 * (N + 1), j2 = j0 + p2;
       if (j2 >= je || j2 == i) continue;
       auto proj = [&](int kk, float f[3]) {  // sum_b Theta^(0)_kk[a][b] F_b / 2
@@ -427,13 +452,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       if (k < N) {
         float TF[3];
         proj(k + 1, TF);
-        float gam = 0.0f, eig = 0.0f;
-#pragma unroll
-        for (int kk = 1; kk < N; ++kk)
-          if (k == kk) {
-            gam = pc.gamma[kk];
-            eig = pc.eigen[kk];
-          }
+        const float gam = modes[k], eig = modes[kMaxTerms + k];
         const float QF = TF[0] * ti[0] + TF[1] * ti[1] + TF[2] * ti[2];
         double v[3];
 #pragma unroll
@@ -565,7 +584,7 @@ template <int N, int NQI, bool ANALYTIC>
 constexpr int theta_min_waves() {
   constexpr int Q = 3 * NQI, PAIRS = kPairThreads / Q, NM = N + 1;
   constexpr int RP = 6 * PAIRS * NM;
-  constexpr int bytes = 4 * (RP * Q + NM * RP) + 8 * PAIRS * (3 * N + 6);
+  constexpr int bytes = 4 * (RP * Q + NM * RP) + 8 * PAIRS * (3 * N + 6) + 8 * kMaxTerms;
   constexpr int blocks = (160 * 1024) / bytes;
   return (blocks >= 5 && !ANALYTIC) ? 4 : (blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1));
 }
