@@ -216,17 +216,24 @@ __device__ __forceinline__ void moments(double b, double l, double I1[N + 3], do
     const double dm = uu - ul, dp = uu + ul;  // uu - s^(n+1) ul for even / odd n+1
     // Zero start at n = 27 as the reference has it, later where l is large:
     // the start values' error shrinks ~l^(-1/2) per step, and over random
-    // (b, l) these starts change every L combination by < 1e-15 relative to
-    // its largest term (the reference's own start at 27 still moves them by
-    // ~2e-7 for l in [10, 20], so small l keeps 27).
-    const int start = l < 100.0 ? 27 : l < 200.0 ? 24 : l < 400.0 ? 22 : l < 1000.0 ? 20 : l < 4000.0 ? 18 : 16;
+    // (b, l), |b| up to 2 sqrt(l), these starts change every Legendre
+    // combination L by < 3e-13 of its largest term, five orders below the
+    // FP32 rounding of the G they form (the reference's own start at 27
+    // still moves them by ~2e-7 for l in [10, 20], so l < 100 keeps 27).
+    const int start = l < 100.0    ? 27
+                      : l < 125.0  ? 22
+                      : l < 244.0  ? 20
+                      : l < 745.0  ? 18
+                      : l < 2842.0 ? 16
+                      : l < 8673.0 ? 14
+                                   : 12;
     // I_n of each family in slot n % 3: a step writes its own slot and reads
     // the other two, so a skipped step (a prefix of the sweep) leaves a zero
     // and the sweep needs no register copies.
     double r1[3] = {0.0, 0.0, 0.0}, r3[3] = {0.0, 0.0, 0.0}, r5[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int n = 27; n >= 2; --n) {
-      if (n > 16 && n > start) continue;  // still all zeros
+      if (n > 12 && n > start) continue;  // still all zeros
       const int k0 = n % 3, k1 = (n + 1) % 3, k2 = (n + 2) % 3;
       const double rn = n;
       // s^(n+1) at s = -1 is +1 for odd n
