@@ -26,16 +26,17 @@ namespace rf {
 
 constexpr int kPairThreads = 192;
 
+// Modes from the highest down: mode m is the last reader of the moments
+// I3_{m+2}, I5_{m+2}, so they die as the sweep goes.
 template <int N, int NQI, int M_>
-__device__ __forceinline__ void analytic_modes(const double R0[3], const double tj[3], double eps2,
-                                               const double* I1, const double* I3, const double* I5,
-                                               float acc[N + 1][6]) {
-  if constexpr (M_ <= N) {
+__device__ __forceinline__ void analytic_modes(const NodeProducts& q, double eps2, const double* I1, const double* I3,
+                                               const double* I5, float acc[N + 1][6]) {
+  if constexpr (M_ >= 0) {
     double G[6];
-    analytic_g<M_>(R0, tj, eps2, I1, I3, I5, G);
+    analytic_g<M_>(q, eps2, I1, I3, I5, G);
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[M_][c] = static_cast<float>(G[c]);
-    analytic_modes<N, NQI, M_ + 1>(R0, tj, eps2, I1, I3, I5, acc);
+    analytic_modes<N, NQI, M_ - 1>(q, eps2, I1, I3, I5, acc);
   }
 }
 
@@ -81,7 +82,7 @@ __device__ __forceinline__ void inner_integrals(const double xi[3], const double
     const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
     double I1[N + 3], I3[N + 3], I5[N + 3];
     moments<N>(bb, ll, I1, I3, I5);
-    analytic_modes<N, NQI, 0>(R0, tjd, pc.eps2_d, I1, I3, I5, acc);
+    analytic_modes<N, NQI, N>(node_products(R0, tjd), pc.eps2_d, I1, I3, I5, acc);
   }
 }
 

@@ -255,10 +255,29 @@ __device__ __forceinline__ void moments(double b, double l, double I1[N + 3], do
 
 // G_f = L01 I + L03 R0R0^T - L13 (t R0^T + R0 t^T) + L23 t t^T
 //     + 2 eps^2 (L03 I - 3 L05 R0R0^T + 3 L15 (t R0^T + R0 t^T) - 3 L25 t t^T)
-// (fortran/compute_inner_integral_analytically.incl:207-260), FP64.
+// (fortran/compute_inner_integral_analytically.incl:207-260), FP64, from the
+// symmetric products of the node shared by every mode: RR = R0 R0^T,
+// TR = t R0^T + R0 t^T, TT = t t^T (components xx, yy, zz, xy, xz, yz).
+struct NodeProducts {
+  double RR[6], TR[6], TT[6];
+};
+
+__device__ __forceinline__ NodeProducts node_products(const double R0[3], const double t[3]) {
+  constexpr int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  NodeProducts q;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const int a = ia[c], bb = ib[c];
+    q.RR[c] = R0[a] * R0[bb];
+    q.TR[c] = fma(t[a], R0[bb], R0[a] * t[bb]);
+    q.TT[c] = t[a] * t[bb];
+  }
+  return q;
+}
+
 template <int F>
-__device__ __forceinline__ void analytic_g(const double R0[3], const double t[3], double eps2, const double* I1,
-                                           const double* I3, const double* I5, double G[6]) {
+__device__ __forceinline__ void analytic_g(const NodeProducts& q, double eps2, const double* I1, const double* I3,
+                                           const double* I5, double G[6]) {
   double L01 = 0, L03 = 0, L05 = 0, L13 = 0, L15 = 0, L23 = 0, L25 = 0;
 #pragma unroll
   for (int p = 0; p <= F; ++p) {
@@ -279,20 +298,10 @@ __device__ __forceinline__ void analytic_g(const double R0[3], const double t[3]
   const double crr = L03 - 3.0 * two_e2 * L05;     // of R0 R0^T
   const double ctr = -L13 + 3.0 * two_e2 * L15;    // of (t R0^T + R0 t^T)
   const double ctt = L23 - 3.0 * two_e2 * L25;     // of t t^T
-  // crr R R^T + ctr (t R^T + R t^T) + ctt t t^T = u R^T + v t^T with
-  // u = crr R + ctr t, v = ctr R + ctt t (symmetric, so only a <= b is formed).
-  double u[3], v[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    u[a] = fma(ctr, t[a], crr * R0[a]);
-    v[a] = fma(ctt, t[a], ctr * R0[a]);
-  }
-  constexpr int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    const int a = ia[c], bb = ib[c];
-    const double g = fma(u[a], R0[bb], v[a] * t[bb]);
-    G[c] = (a == bb) ? iso + g : g;
+    const double g0 = (c < 3) ? fma(crr, q.RR[c], iso) : crr * q.RR[c];
+    G[c] = fma(ctt, q.TT[c], fma(ctr, q.TR[c], g0));
   }
 }
 
