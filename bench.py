@@ -104,11 +104,11 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
 # VALU issue of the analytical theta assembly (N = 5, 8 x 3 nodes), with the
 # read-back of its rows for the first residual: lane instructions per ordered
 # fiber pair, from its counters at M = 5000
-# (profiles/assemble_theta_M5000_counters.md: 1.026e10 wave instructions per
+# (profiles/assemble_theta_M5000_counters.md: 8.83e9 wave instructions per
 # launch x 64 lanes / 5000 x 4999 pairs), against one wave instruction per 4
 # cycles on each of 1024 SIMDs at 2.4 GHz.  The read-back's 13.5 GB would take
 # 1.7 ms at 8 TB/s, under the VALU time, so the bound is the issue time.
-ASSEMBLY_LANE_INSTR_PER_PAIR = 26_300
+ASSEMBLY_LANE_INSTR_PER_PAIR = 22_600
 VALU_LANE_INSTR_PER_S = 1024 * 64 / 4 * 2.4e9
 
 
