@@ -423,10 +423,15 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       }
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
     }
-    // velocity operator: Theta_0 (projection 1), then Phi_0 (projection 0)
+    // velocity operator: Theta_0 (projection 1), then Phi_0 (projection 0).
+    // Analytical form: its jobs start on the lanes after the system
+    // components' last partial pass and the RHS jobs sit on the top lanes,
+    // so the three waves get the same number of passes (measured faster
+    // there, slower for the numerical form, which keeps the plain order).
     if (Vi) {
       float* const Vcol = Vi + static_cast<int64_t>(j0) * N;
-      for (int job = tid; job < 12 * NV; job += kPairThreads) {
+      constexpr int kVShift = ANALYTIC ? kPairThreads - (NC * NV) % kPairThreads : 0;
+      for (int job = (tid + kVShift) % kPairThreads; job < 12 * NV; job += kPairThreads) {
         const int w = job / NV;
         const int pm0 = 4 * (job - w * NV);
         const int kk = w < 6 ? 1 : 0;
@@ -438,7 +443,8 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     // RHS (source slot p2, row block k): k < N forms rows 3k..3k+2 of b from
     // the projection of G_0 F/2 onto w P_{k+1}; k = N the velocity constants
     // sum_q w G_0 F/2 and sum_q w P_1 G_0 F/2.
-    for (int rj = tid; rj < PAIRS * (N + 1); rj += kPairThreads) {
+    constexpr int kRShift = ANALYTIC ? PAIRS * (N + 1) % kPairThreads : 0;
+    for (int rj = (tid + kRShift) % kPairThreads; rj < PAIRS * (N + 1); rj += kPairThreads) {
       const int p2 = rj / (N + 1), k = rj - p2 * (N + 1), j2 = j0 + p2;
       if (j2 >= je || j2 == i) continue;
       auto proj = [&](int kk, float f[3]) {  // sum_b Theta^(0)_kk[a][b] F_b / 2
