@@ -301,7 +301,8 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
                                                      const double* __restrict__ ori, float* __restrict__ T,
                                                      double* __restrict__ bpart, int jgroup, const PhysConsts& pc,
                                                      float* __restrict__ Tv, double* __restrict__ cpart,
-                                                     const double* __restrict__ x0, double* __restrict__ apart) {
+                                                     const double* __restrict__ x0, double* __restrict__ apart,
+                                                     const SusDev* __restrict__ serving = nullptr) {
   constexpr Quad<N, NQI> quad{};
   constexpr int Q = 3 * NQI;
   constexpr int PAIRS = kPairThreads / Q;
@@ -491,6 +492,16 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     cpart[(static_cast<int64_t>(gy) * M + i) * 6 + (tid - ROWS)] = s;
   }
 
+  // Serving: the suspension's fields are read only here, after the sweeps,
+  // so that they hold no registers during them.
+  double* r0v = nullptr;
+  double* npart = nullptr;
+  if (serving != nullptr) {
+    x0 = serving->x0;
+    r0v = serving->V;
+    npart = serving->npart;
+  }
+  double out = 0.0;     // lane tid < ROWS: row tid of the off-diagonal product A x0
   if (x0 != nullptr) {  // the stores above are visible to the workgroup after the barrier
     constexpr int NACC = 1 + 3 * (N - 1);  // s0, then U_k (k >= 1) as in gemv_theta_block
     double acc[NACC];
@@ -570,7 +581,6 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         for (int w = 0; w < WAVES; ++w) t += red[w * NACC + idx];
         return t;
       };
-      double out;
       if (k == 0) {
         out = static_cast<double>(pc.D1) * tid_[a] * sum(0);
       } else {
@@ -578,7 +588,33 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         const double tU = tid_[0] * U[0] + tid_[1] * U[1] + tid_[2] * U[2];
         out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * tid_[a] * tU);
       }
-      apart[static_cast<int64_t>(gy) * n + static_cast<int64_t>(ROWS) * i + tid] = out;
+      if (apart) apart[static_cast<int64_t>(gy) * n + static_cast<int64_t>(ROWS) * i + tid] = out;
+    }
+  }
+  // Serving (all sources in this workgroup, b written directly): the first
+  // residual r0 = b - (x0 + A_offdiag x0) (r0 = b without x0) into r0v and the
+  // target's squared norms of b and r0 into npart[2 i], npart[2 i + 1].
+  if (npart != nullptr) {
+    static_assert(2 * ROWS * sizeof(double) <= NM * RP * sizeof(float), "norm partials fit the freed Pj");
+    double* nred = reinterpret_cast<double*>(Pj);  // Pj is free after the last sweep
+    if (tid < ROWS) {
+      double sb = 0.0;
+      for (int q = 0; q < PAIRS; ++q) sb += bred[q][tid];  // this row of b, as stored above
+      const int64_t row = static_cast<int64_t>(ROWS) * i + tid;
+      const double r = (x0 != nullptr) ? sb - (out + x0[row]) : sb;
+      r0v[row] = r;
+      nred[tid] = sb * sb;
+      nred[ROWS + tid] = r * r;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double b2 = 0.0, r2 = 0.0;
+      for (int t = 0; t < ROWS; ++t) {
+        b2 += nred[t];
+        r2 += nred[ROWS + t];
+      }
+      npart[2 * i] = b2;
+      npart[2 * i + 1] = r2;
     }
   }
 }
@@ -606,14 +642,15 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(th
 }
 
 // Serving: workgroup = (suspension, target fiber), all sources in one group,
-// so b and the velocity constants are written directly.
+// so b and the velocity constants are written directly, and with them the
+// first GMRES residual (into the basis' first vector) and its norms.
 template <int N, int NQI, bool ANALYTIC>
 __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI, ANALYTIC>(), 8))) void assemble_theta_batch_kernel(
     const SusDev* __restrict__ sus, const int2* __restrict__ targets, PhysConsts pc) {
   const int2 e = targets[blockIdx.x];
   const SusDev& d = sus[e.x];
   assemble_theta_block<N, NQI, ANALYTIC>(e.y, 0, d.M, d.Lp, d.pos, d.ori, d.T, d.b, d.M, pc, d.Tv, d.cpart, nullptr,
-                                         nullptr);
+                                         nullptr, &d);
 }
 
 // Dense A (column-major, leading dimension n) from the theta form; identity

@@ -85,7 +85,9 @@ struct SusDev {
   double* cpart;                   // velocity constants [M][6]
   double* b;
   double* x;
-  double* V;                       // Krylov basis, kdim x n
+  const double* x0;                // the start of this tick's solve: x when warm, else null (x = 0)
+  double* npart;                   // [M][2]: per target |b|^2, |b - A x0|^2 (the assembly writes them)
+  double* V;                       // Krylov basis, kdim x n; the assembly writes b - A x0 to V_0
   double* tv;                      // velocities of this step
   double* rv;
   const double* pos_prev;
@@ -115,7 +117,8 @@ struct SusIter {
 // control step after each iteration forms the iteration's reductions from
 // the block partials, advances this state and writes the next SusIter.
 struct SusCtl {
-  int phase;   // 3: ||b|| next (w = b), 0: true residual next, 1: Arnoldi step next, 2: done
+  int phase;   // 3: start (V_0 = b - A x0 and the norms from the assembly), 0: true residual next,
+               // 1: Arnoldi step next, 2: done
   int k;       // Arnoldi column of the current cycle
   int it;      // Arnoldi steps so far
   int converged;

@@ -486,38 +486,57 @@ hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* su
 // same GMRES(m) as the single-simulation solver: true residual at every
 // restart, classical Gram-Schmidt twice (the two passes' coefficients summed),
 // Givens rotations, stop on ||r|| <= tol ||b||.  The lanes form the
-// iteration's reductions from the block partials, lane 0 advances the state
-// and writes the next iteration's operations into it[q] and whether it has
-// any into active[q].
+// iteration's reductions from the block partials (at the start step, the
+// norms of b and of the first residual from the assembly's per-target
+// partials), lane 0 advances the state and writes the next iteration's
+// operations into it[q] and whether it has any into active[q].
 __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit,
                                                                  const SusDev* __restrict__ sus,
                                                                  SusIter* __restrict__ it, SusCtl* __restrict__ ctl,
                                                                  const double* __restrict__ part,
                                                                  int* __restrict__ active) {
   extern __shared__ double hsh[];  // h0[K], h1[K], ||w||^2
+  __shared__ double nsh[2][64];     // start: the lanes' sums of the per-target norm partials
   const int q = blockIdx.x;
   const SusDev& d = sus[q];
   SusIter& c = it[q];
   SusCtl& C = ctl[q];
   const int K = c.K;
   const int64_t region = static_cast<int64_t>(kdim) * nvb;
+  const bool start = C.phase == 3;
   if (c.w != nullptr)
     for (int t = threadIdx.x; t <= 2 * K; t += 64) {
       const int p = t < K ? 0 : (t < 2 * K ? 1 : 2);
       const int j = t < K ? t : (t < 2 * K ? t - K : 0);
       hsh[t] = sum_block_parts(part + p * region, nvb, j, d.vb_first, d.vb_count);
     }
+  if (start) {  // lane l sums targets l, l + 64, ... (a fixed order)
+    double b2 = 0.0, r2 = 0.0;
+    for (int i = threadIdx.x; i < d.M; i += 64) {
+      b2 += d.npart[2 * i];
+      r2 += d.npart[2 * i + 1];
+    }
+    nsh[0][threadIdx.x] = b2;
+    nsh[1][threadIdx.x] = r2;
+  }
   __syncthreads();
   if (threadIdx.x != 0) return;
   const double* h0 = hsh;
   const double* h1 = hsh + K;
-  const double w2 = hsh[2 * K];
+  const double w2 = (c.w != nullptr) ? hsh[2 * K] : 0.0;
   int zero_x = 0, ycount = 0;
   double* svec = nullptr;
   double sval = 1.0;
-  if (C.phase == 3) {  // w = b
-    C.bnorm = sqrt(w2);
-    if (!isfinite(C.bnorm)) {
+  if (start) {  // V_0 = b - A x0 (b when cold) from the assembly
+    double b2 = 0.0, r2 = 0.0;
+    for (int l = 0; l < 64; ++l) {
+      b2 += nsh[0][l];
+      r2 += nsh[1][l];
+    }
+    C.bnorm = sqrt(b2);
+    const double beta = sqrt(r2);
+    C.res = beta;
+    if (!isfinite(C.bnorm) || !isfinite(beta)) {
       C.nonfinite = 1;
       C.phase = 2;
     } else if (C.bnorm == 0.0) {  // x = 0 is the solution
@@ -526,7 +545,19 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
       C.phase = 2;
     } else {
       zero_x = C.cold;
-      C.phase = 0;
+      if (beta / C.bnorm <= tol) {
+        C.converged = 1;
+        C.phase = 2;
+      } else if (C.it >= maxit) {
+        C.phase = 2;
+      } else {
+        svec = d.V;
+        sval = 1.0 / beta;
+        for (int i = 0; i <= m; ++i) C.g[i] = 0.0;
+        C.g[0] = beta;
+        C.k = 0;
+        C.phase = 1;
+      }
     }
   } else if (C.phase == 0) {  // w = b - A x
     const double beta = sqrt(w2);

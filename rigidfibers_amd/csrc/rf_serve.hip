@@ -72,6 +72,7 @@ struct Suspension {
   double* cpart = nullptr;
   double* b = nullptr;
   double* x = nullptr;
+  double* npart = nullptr;  // [M][2] norms of b and the first residual, per target
   double* V = nullptr;
   int steps_done = 0;
   rf_status status = RF_OK;
@@ -172,6 +173,7 @@ int64_t footprint(const rf_server* S, int M, int64_t* Lp_out) {
   b += align256(static_cast<int64_t>(M) * 12 * Lp * 4);  // velocity operator
   b += align256(6LL * M * 8);                      // velocity constants
   b += 2 * align256(n * 8);                        // b, x
+  b += align256(2LL * M * 8);                      // norm partials
   b += align256(static_cast<int64_t>(S->kdim) * n * 8);  // Krylov basis
   if (Lp_out) *Lp_out = Lp;
   return b;
@@ -207,6 +209,7 @@ rf_status admit(rf_server* S, Request* r, Suspension** out) {
   s->cpart = reinterpret_cast<double*>(take(6LL * s->M * 8));
   s->b = reinterpret_cast<double*>(take(static_cast<int64_t>(s->n) * 8));
   s->x = reinterpret_cast<double*>(take(static_cast<int64_t>(s->n) * 8));
+  s->npart = reinterpret_cast<double*>(take(2LL * s->M * 8));
   s->V = reinterpret_cast<double*>(take(static_cast<int64_t>(S->kdim) * s->n * 8));
   if (hipMemcpyAsync(s->pos[0], r->pos0.data(), f3, hipMemcpyHostToDevice, S->stream) != hipSuccess ||
       hipMemcpyAsync(s->ori[0], r->ori0.data(), f3, hipMemcpyHostToDevice, S->stream) != hipSuccess ||
@@ -260,6 +263,9 @@ bool build_tables(rf_server* S) {
     d.cpart = s->cpart;
     d.b = s->b;
     d.x = s->x;
+    // the solve starts from the previous solution when there is one
+    d.x0 = (S->cfg.base.initial_guess != RF_GUESS_ZERO && s->have_solution) ? s->x : nullptr;
+    d.npart = s->npart;
     d.V = s->V;
     d.tv = s->tv[s->vcur];
     d.rv = s->rv[s->vcur];
@@ -378,10 +384,11 @@ bool lockstep_round(rf_server* S, bool* any) {
 
 // GMRES(m) for every admitted suspension (the same algorithm as the
 // single-simulation solver: true residual at each restart, classical
-// Gram-Schmidt twice, Givens rotations), its control on the device: the first
-// iteration measures ||b|| (w = b), the control step then sets x = 0 unless a
-// previous solution is the start and asks for the true residual; the host
-// replays rounds of iterations until no suspension has work left.
+// Gram-Schmidt twice, Givens rotations), its control on the device: the
+// assembly has written the first residual b - A x0 (b when cold) into V_0 and
+// the norms' per-target partials; a start step of the control takes them
+// (x = 0 when cold) and sets up the first Arnoldi step; the host replays
+// rounds of iterations until no suspension has work left.
 bool solve_lockstep(rf_server* S) {
   const int ns = static_cast<int>(S->active.size());
   const int m = S->m;
@@ -403,12 +410,14 @@ bool solve_lockstep(rf_server* S) {
     rf::SusIter& c = S->h_it[q];
     c = rf::SusIter{};
     c.ycoef = C.y;
-    c.w = s->b;
-    c.K = 0;
   }
   hipStream_t st = S->stream;
+  // the start step: the control reads the norms of b and of the first
+  // residual, which the assembly left in V_0, and sets up the first Arnoldi step
   if (hipMemcpyAsync(S->d_ctl, S->h_ctl, ns * sizeof(rf::SusCtl), hipMemcpyHostToDevice, st) != hipSuccess ||
-      hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess)
+      hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
+      rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->nvb, S->cfg.base.gmres_tolerance, maxit, S->d_sus,
+                                     S->d_it, S->d_ctl, S->d_part, S->d_act, st) != hipSuccess)
     return false;
   if (!S->cfg.eager && !ensure_iteration_graph(S)) return false;
   // every suspension stops by itself after at most maxit Arnoldi steps and
