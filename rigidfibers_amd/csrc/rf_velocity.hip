@@ -207,7 +207,7 @@ __device__ __forceinline__ void velocity_stored_block(int i, int M, int N, int64
   const float* Vi = Tv + static_cast<int64_t>(i) * 12 * Lp;
   const int L = M * N;
   double sw[3] = {0.0, 0.0, 0.0}, sv[3] = {0.0, 0.0, 0.0};
-  for (int e = threadIdx.x; e < L; e += kVelStoredThreads) {
+  auto column = [&](int e) {
     const double x0 = x[3 * static_cast<int64_t>(e)], x1 = x[3 * static_cast<int64_t>(e) + 1],
                  x2 = x[3 * static_cast<int64_t>(e) + 2];
     const float* p = Vi + e;
@@ -223,7 +223,14 @@ __device__ __forceinline__ void velocity_stored_block(int i, int M, int N, int64
     sv[0] = fma(vxx, x0, fma(vxy, x1, fma(vxz, x2, sv[0])));
     sv[1] = fma(vxy, x0, fma(vyy, x1, fma(vyz, x2, sv[1])));
     sv[2] = fma(vxz, x0, fma(vyz, x1, fma(vzz, x2, sv[2])));
+  };
+  // two columns per pass (24 loads in flight per lane), in the same order
+  int e = threadIdx.x;
+  for (; e + kVelStoredThreads < L; e += 2 * kVelStoredThreads) {
+    column(e);
+    column(e + kVelStoredThreads);
   }
+  if (e < L) column(e);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const double acc[6] = {sv[0], sv[1], sv[2], sw[0], sw[1], sw[2]};
 #pragma unroll
