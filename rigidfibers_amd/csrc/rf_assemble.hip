@@ -347,7 +347,6 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     }
   __syncthreads();
 
-  const int nu = NC + (Vi ? 12 : 0);  // component arrays written per sweep
 
   for (int j0 = jb; j0 < je; j0 += PAIRS) {
     const int j = j0 + p;

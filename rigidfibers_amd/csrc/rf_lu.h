@@ -24,6 +24,10 @@ struct LuWorkspace {
   double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
   void* At = nullptr;      // packed L panel: n rows x kLuPanel, K-contiguous, mode's element type
   void* Bt = nullptr;      // packed U panel: n columns x kLuPanel
+  void* At2 = nullptr;     // the same for the next panel, factored while the current one's
+  void* Bt2 = nullptr;     // trailing update runs (look-ahead)
+  hipStream_t aux = nullptr;  // the look-ahead's stream (default priority)
+  hipEvent_t next_cols = nullptr, next_done = nullptr;
   double* Dinv = nullptr;  // per diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 FP64 each, [j][c]
   void* blt = nullptr;     // hipBLASLt state for the mixed modes' trailing updates (none: built-in kernel)
   static constexpr int kEvents = 256;  // event pairs around the trailing updates

@@ -377,64 +377,91 @@ __global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const f
 
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
 
+// Blocked right-looking LU with pivoting inside the 64-wide diagonal blocks,
+// panels of kLuPanel columns, and a one-panel look-ahead: the trailing update
+// of panel p first updates the next panel's columns, then the aux stream
+// factors that panel (diagonal blocks, L21, the in-panel U12 and K = 64
+// updates: latency-bound small kernels) while the main stream updates the
+// remaining columns.  The two streams touch disjoint columns of F and
+// alternate between two pairs of pack buffers.
 template <typename T>
 static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
   const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
   float* F = ws.F;
-  T* At = static_cast<T*>(ws.At);
-  T* Bt = static_cast<T*>(ws.Bt);
+  T* Ab[2] = {static_cast<T*>(ws.At), static_cast<T*>(ws.At2)};
+  T* Bb[2] = {static_cast<T*>(ws.Bt), static_cast<T*>(ws.Bt2)};
   auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
   hipError_t e = hipSuccess;
   int nev = 0;
-  for (int64_t K0 = 0; K0 < n; K0 += W) {
+  // the panel [K0, KE) itself, on stream st, packs into At / Bt
+  auto factor_panel = [&](int64_t K0, T* At, T* Bt, hipStream_t st) -> hipError_t {
     const int64_t KE = std::min<int64_t>(n, K0 + W);
     for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
       const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
       const int64_t k1 = k0 + kb;
-      hipLaunchKernelGGL(lu_diag_kernel, dim3(1), dim3(64), 0, s, lda, F, k0, kb, ws.perm, ws.info);
-      if (k1 < n) {  // a full block (only the last block of the matrix is short)
-        hipLaunchKernelGGL((lu_l21_kernel<T>), dim3(grid_of(n - k1, 256)), dim3(256), 0, s, k1, n, lda, F, k0, K0,
+      hipLaunchKernelGGL(lu_diag_kernel, dim3(1), dim3(64), 0, st, lda, F, k0, kb, ws.perm, ws.info);
+      if (k1 < n)  // a full block (only the last block of the matrix is short)
+        hipLaunchKernelGGL((lu_l21_kernel<T>), dim3(grid_of(n - k1, 256)), dim3(256), 0, st, k1, n, lda, F, k0, K0,
                            At);
-      }
       if (k1 < KE) {
-        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, s, k1, KE, lda, F, k0,
+        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, st, k1, KE, lda, F, k0,
                            K0, ws.perm, Bt);
-        if ((e = gemm_packed_sub(mode, n - k1, KE - k1, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
-                                 Bt + (k1 - K0) * W + (k0 - K0), W, at(k1, k1), lda, s)) != hipSuccess)
-          return e;
+        hipError_t ee = gemm_packed_sub(mode, n - k1, KE - k1, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
+                                        Bt + (k1 - K0) * W + (k0 - K0), W, at(k1, k1), lda, st);
+        if (ee != hipSuccess) return ee;
       }
     }
-    if (KE < n) {
-      // the panel's rows of U for the trailing columns
-      for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
-        const int64_t k1 = k0 + kLuNb;
-        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(n - KE, kLuNb)), dim3(64), 0, s, KE, n, lda, F, k0, K0,
-                           ws.perm, Bt);
-        if (k1 < KE && (e = gemm_packed_sub(mode, KE - k1, n - KE, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
-                                            Bt + (KE - K0) * W + (k0 - K0), W, at(k1, KE), lda, s)) != hipSuccess)
-          return e;
-      }
-      // trailing update, K = W
-      const bool timed = nev < LuWorkspace::kEvents;
-      if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-      e = ws.blt ? blt_gemm_sub(ws.blt, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
-                                at(KE, KE), lda, s)
-                 : hipErrorNotSupported;
-      if (e == hipSuccess) {
-        res->library = true;
-      } else if (e == hipErrorNotSupported) {
-        if ((e = gemm_packed_sub(mode, n - KE, n - KE, KE - K0, At + (KE - K0) * W, W, Bt + (KE - K0) * W, W,
-                                 at(KE, KE), lda, s)) != hipSuccess)
-          return e;
-      } else {
+    return hipGetLastError();
+  };
+  // C[KE:n, c0:c1] -= L[KE:n, K0:KE] U[K0:KE, c0:c1], K = KE - K0 (timed)
+  auto trailing = [&](int64_t K0, int64_t KE, int64_t c0, int64_t c1, T* At, T* Bt) -> hipError_t {
+    const bool timed = nev < LuWorkspace::kEvents;
+    if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
+    hipError_t ee = ws.blt ? blt_gemm_sub(ws.blt, n - KE, c1 - c0, KE - K0, At + (KE - K0) * W, W,
+                                          Bt + (c0 - K0) * W, W, at(KE, c0), lda, s)
+                           : hipErrorNotSupported;
+    if (ee == hipSuccess) {
+      res->library = true;
+    } else if (ee == hipErrorNotSupported) {
+      if ((ee = gemm_packed_sub(mode, n - KE, c1 - c0, KE - K0, At + (KE - K0) * W, W, Bt + (c0 - K0) * W, W,
+                                at(KE, c0), lda, s)) != hipSuccess)
+        return ee;
+    } else {
+      return ee;
+    }
+    if (timed) {
+      (void)hipEventRecord(ws.ev[2 * nev + 1], s);
+      ++nev;
+      res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(c1 - c0) *
+                        static_cast<double>(KE - K0);
+    }
+    return hipSuccess;
+  };
+  if ((e = factor_panel(0, Ab[0], Bb[0], s)) != hipSuccess) return e;
+  for (int64_t K0 = 0, p = 0; K0 < n; K0 += W, ++p) {
+    const int64_t KE = std::min<int64_t>(n, K0 + W);
+    if (KE >= n) break;
+    T* At = Ab[p & 1];
+    T* Bt = Bb[p & 1];
+    // the panel's rows of U for the trailing columns
+    for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
+      const int64_t k1 = k0 + kLuNb;
+      hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(n - KE, kLuNb)), dim3(64), 0, s, KE, n, lda, F, k0, K0,
+                         ws.perm, Bt);
+      if (k1 < KE && (e = gemm_packed_sub(mode, KE - k1, n - KE, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
+                                          Bt + (KE - K0) * W + (k0 - K0), W, at(k1, KE), lda, s)) != hipSuccess)
         return e;
-      }
-      if (timed) {
-        (void)hipEventRecord(ws.ev[2 * nev + 1], s);
-        ++nev;
-        res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(n - KE) * static_cast<double>(KE - K0);
-      }
     }
+    // trailing update: the next panel's columns, then (look-ahead) that panel
+    // on the aux stream while the main stream updates the rest
+    const int64_t NE = std::min<int64_t>(n, KE + W);
+    if ((e = trailing(K0, KE, KE, NE, At, Bt)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ws.next_cols, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(ws.aux, ws.next_cols, 0)) != hipSuccess) return e;
+    if ((e = factor_panel(KE, Ab[(p + 1) & 1], Bb[(p + 1) & 1], ws.aux)) != hipSuccess) return e;
+    if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
+    if (NE < n && (e = trailing(K0, KE, NE, n, At, Bt)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
   res->gemm_launches = nev;
   hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(n, kLuNb)), dim3(64), 0, s, n, lda, F, ws.Dinv);
@@ -487,12 +514,17 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, bool library
   if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->At, pack)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->Bt, pack)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->At2, pack)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Bt2, pack)) != hipSuccess) return e;
+  if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&ws->next_cols, hipEventDisableTiming)) != hipSuccess) return e;
+  if ((e = hipEventCreateWithFlags(&ws->next_done, hipEventDisableTiming)) != hipSuccess) return e;
   const size_t dinv = sizeof(double) * 2 * kLuNb * kLuNb * static_cast<size_t>((n + kLuNb - 1) / kLuNb);
   if ((e = hipMalloc(&ws->Dinv, dinv)) != hipSuccess) return e;
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n) + 2 * pack + dinv);
+                                   sizeof(double) * 3 * static_cast<size_t>(n) + 4 * pack + dinv);
   if (library && mode != LuMode::kFP32) {
     int64_t wb = 0;
     if (blt_create(mode, &ws->blt, &wb) == hipSuccess)
@@ -510,8 +542,14 @@ void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->perm) (void)hipFree(ws->perm);
   if (ws->info) (void)hipFree(ws->info);
   if (ws->vec) (void)hipFree(ws->vec);
+  if (ws->aux) (void)hipStreamSynchronize(ws->aux);
   if (ws->At) (void)hipFree(ws->At);
   if (ws->Bt) (void)hipFree(ws->Bt);
+  if (ws->At2) (void)hipFree(ws->At2);
+  if (ws->Bt2) (void)hipFree(ws->Bt2);
+  if (ws->next_cols) (void)hipEventDestroy(ws->next_cols);
+  if (ws->next_done) (void)hipEventDestroy(ws->next_done);
+  if (ws->aux) (void)hipStreamDestroy(ws->aux);
   if (ws->Dinv) (void)hipFree(ws->Dinv);
   for (auto& ev : ws->ev)
     if (ev) (void)hipEventDestroy(ev);
