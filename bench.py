@@ -70,7 +70,7 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
     sim = Simulation(params, pos, ori, solver=solver, integration=args.integration, matrix_format=matrix_format,
                      velocity_operator=args.velocity_operator, refinement_tolerance=1e-10,
                      initial_guess=args.initial_guess, lu_gemm=args.lu_gemm,
-                     separate_residual=args.separate_residual)
+                     separate_residual=args.separate_residual, confirm_residual=args.confirm_residual)
     try:
         for t in range(warmup):
             sim.step(t)
@@ -356,6 +356,8 @@ def main():
                     help="mixed LUs' trailing updates: hipBLASLt when it has an algorithm, or the built-in kernel")
     ap.add_argument("--separate-residual", action="store_true",
                     help="form the first residual of a warm start with a matvec, not in the assembly kernel")
+    ap.add_argument("--confirm-residual", action="store_true",
+                    help="GMRES: confirm a converged restart cycle with the true residual (one matvec more)")
     ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")

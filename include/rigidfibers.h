@@ -130,13 +130,20 @@ typedef struct rf_config {
   int32_t separate_residual;                          /* iterative solvers on the theta form: 0 the first residual
                                                          b - A x0 of a warm start comes from the assembly kernel
                                                          (one matvec fewer per solve), 1 from a matvec */
-  int32_t reserved[2];
+  int32_t confirm_residual;                           /* GMRES: 0 stops on the restart cycle's own residual
+                                                         estimate, as lib/viennacl/linalg/gmres.hpp:330-335
+                                                         does; 1 confirms it with the true residual b - A x
+                                                         (one matvec more per solve), as fortran/sgmres.f does
+                                                         with icntl(8) = 1 */
+  int32_t reserved[1];
 } rf_config;
 
 typedef struct rf_solver_stats {
   int32_t iterations;          /* Krylov iterations (matvecs) or refinement sweeps of the last solve */
   int32_t converged;           /* 1 if the tolerance was met */
-  double relative_residual;    /* ||b - A x|| / ||b|| of the returned x (FP64) */
+  double relative_residual;    /* ||b - A x|| / ||b|| of the returned x (FP64); GMRES without
+                                  confirm_residual: the restart cycle's estimate of it when that
+                                  cycle converged, as BiCGStab's is its recurrence's */
 } rf_solver_stats;
 
 typedef struct rf_timings {

@@ -63,7 +63,8 @@ class RfConfig(C.Structure):
         ("inner_tolerance", C.c_double),
         ("lu_gemm", C.c_int32),
         ("separate_residual", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("confirm_residual", C.c_int32),
+        ("reserved", C.c_int32 * 1),
     ]
 
 

@@ -167,6 +167,8 @@ rf_status validate_config(const rf_config* c) {
   if (c->lu_gemm != 0 && c->lu_gemm != 1) return fail(RF_ERR_INVALID_ARGUMENT, "lu_gemm must be 0 or 1");
   if (c->separate_residual != 0 && c->separate_residual != 1)
     return fail(RF_ERR_INVALID_ARGUMENT, "separate_residual must be 0 or 1");
+  if (c->confirm_residual != 0 && c->confirm_residual != 1)
+    return fail(RF_ERR_INVALID_ARGUMENT, "confirm_residual must be 0 or 1");
   if (c->initial_guess != RF_GUESS_ZERO && c->initial_guess != RF_GUESS_PREVIOUS &&
       c->initial_guess != RF_GUESS_EXTRAPOLATED)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown initial guess");
@@ -276,12 +278,18 @@ rf_status device_dot(rf_simulation* s, const double* u, const double* v, double*
 // Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
 // GMRES(m) for A x = b: A the FP32 system, or its FP16 copy (half).  Starts
 // from x when warm, else from 0; stops at ||b - A x|| <= tol ||b|| or after
-// maxit iterations.  r0, when given, is b - A x of the warm start.  *out
+// maxit iterations.  Each restart cycle begins with the true residual; a
+// cycle whose own estimate |g_k| meets the tolerance ends the solve after
+// its update of x (lib/viennacl/linalg/gmres.hpp:330-335), unless confirm
+// asks for the true residual of that x first (fortran/sgmres.f with
+// icntl(8) = 1).  r0, when given, is b - A x of the warm start.  *out
 // receives the iterations, convergence and residual.
 rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, double tol, int maxit, bool half,
                      rf_solver_stats* out, const double* r0 = nullptr) {
   const int64_t n = s->n;
   const int m = s->cfg.gmres_restart;
+  // GMRES-IR's inner solves leave the true residual to the outer loop
+  const bool confirm = !half && s->cfg.confirm_residual;
   double* V = s->V;
   auto Vk = [&](int k) { return V + static_cast<int64_t>(k) * n; };
 
@@ -371,6 +379,10 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
     }
     RF_HIP(hipMemcpyAsync(s->hdev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s->stream));
     RF_HIP(rf::axpy_multi(n, k_used, V, n, s->hdev, 1.0, x, s->stream));
+    if (!confirm && res / bnorm <= tol) {
+      converged = true;
+      break;
+    }
     // the loop head recomputes the true residual b - A x
   }
   out->iterations = it;

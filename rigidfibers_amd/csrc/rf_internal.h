@@ -131,8 +131,9 @@ struct SusCtl {
   double* g;   // m + 1 rotated right-hand side
   double* y;   // m coefficients of x += V y
 };
-hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, const SusDev* sus,
-                                      SusIter* it, SusCtl* ctl, const double* part, int* active, hipStream_t s);
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, int confirm,
+                                      const SusDev* sus, SusIter* it, SusCtl* ctl, const double* part, int* active,
+                                      hipStream_t s);
 hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* sus, const SusIter* it,
                                   const int2* vblocks, double* part, hipStream_t s);
 

@@ -485,12 +485,14 @@ hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* su
 // Control step of the serving lockstep (one workgroup per suspension), the
 // same GMRES(m) as the single-simulation solver: true residual at every
 // restart, classical Gram-Schmidt twice (the two passes' coefficients summed),
-// Givens rotations, stop on ||r|| <= tol ||b||.  The lanes form the
+// Givens rotations, stop on ||r|| <= tol ||b||, a cycle whose estimate meets
+// it ending the solve with its update of x (confirm: with the true residual
+// of that x).  The lanes form the
 // iteration's reductions from the block partials (at the start step, the
 // norms of b and of the first residual from the assembly's per-target
 // partials), lane 0 advances the state and writes the next iteration's
 // operations into it[q] and whether it has any into active[q].
-__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit,
+__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit, int confirm,
                                                                  const SusDev* __restrict__ sus,
                                                                  SusIter* __restrict__ it, SusCtl* __restrict__ ctl,
                                                                  const double* __restrict__ part,
@@ -611,7 +613,12 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
           C.y[i] = acc / C.H[i + static_cast<int64_t>(i) * (m + 1)];
         }
         ycount = ku;
-        C.phase = 0;
+        if (!confirm && C.res / C.bnorm <= tol) {  // the update of x is the last operation
+          C.converged = 1;
+          C.phase = 2;
+        } else {
+          C.phase = 0;
+        }
       } else {
         svec = d.V + static_cast<int64_t>(k + 1) * d.n;
         sval = 1.0 / hn;
@@ -640,11 +647,12 @@ __global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim
   active[q] = (C.phase != 2 || zero_x || ycount > 0 || svec != nullptr) ? 1 : 0;
 }
 
-hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, const SusDev* sus,
-                                      SusIter* it, SusCtl* ctl, const double* part, int* active, hipStream_t s) {
+hipError_t launch_gmres_control_batch(int ns, int m, int kdim, int nvb, double tol, int maxit, int confirm,
+                                      const SusDev* sus, SusIter* it, SusCtl* ctl, const double* part, int* active,
+                                      hipStream_t s) {
   if (ns <= 0) return hipSuccess;
   hipLaunchKernelGGL(gmres_control_batch_kernel, dim3(ns), dim3(64), (2 * kdim + 1) * sizeof(double), s, m, kdim,
-                     nvb, tol, maxit, sus, it, ctl, part, active);
+                     nvb, tol, maxit, confirm, sus, it, ctl, part, active);
   return hipGetLastError();
 }
 

@@ -311,8 +311,8 @@ bool iteration_launches(rf_server* S) {
         hipSuccess)
       return false;
   return rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->nvb, S->cfg.base.gmres_tolerance,
-                                        S->cfg.base.gmres_max_iterations, S->d_sus, S->d_it, S->d_ctl, S->d_part,
-                                        S->d_act, st) == hipSuccess;
+                                        S->cfg.base.gmres_max_iterations, S->cfg.base.confirm_residual, S->d_sus,
+                                        S->d_it, S->d_ctl, S->d_part, S->d_act, st) == hipSuccess;
 }
 
 bool round_launches(rf_server* S) {
@@ -384,7 +384,8 @@ bool lockstep_round(rf_server* S, bool* any) {
 
 // GMRES(m) for every admitted suspension (the same algorithm as the
 // single-simulation solver: true residual at each restart, classical
-// Gram-Schmidt twice, Givens rotations), its control on the device: the
+// Gram-Schmidt twice, Givens rotations, the solve ending on a converged
+// cycle's estimate unless confirm_residual), its control on the device: the
 // assembly has written the first residual b - A x0 (b when cold) into V_0 and
 // the norms' per-target partials; a start step of the control takes them
 // (x = 0 when cold) and sets up the first Arnoldi step; the host replays
@@ -416,8 +417,9 @@ bool solve_lockstep(rf_server* S) {
   // residual, which the assembly left in V_0, and sets up the first Arnoldi step
   if (hipMemcpyAsync(S->d_ctl, S->h_ctl, ns * sizeof(rf::SusCtl), hipMemcpyHostToDevice, st) != hipSuccess ||
       hipMemcpyAsync(S->d_it, S->h_it, ns * sizeof(rf::SusIter), hipMemcpyHostToDevice, st) != hipSuccess ||
-      rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->nvb, S->cfg.base.gmres_tolerance, maxit, S->d_sus,
-                                     S->d_it, S->d_ctl, S->d_part, S->d_act, st) != hipSuccess)
+      rf::launch_gmres_control_batch(ns, S->m, S->kdim, S->nvb, S->cfg.base.gmres_tolerance, maxit,
+                                     S->cfg.base.confirm_residual, S->d_sus, S->d_it, S->d_ctl, S->d_part, S->d_act,
+                                     st) != hipSuccess)
     return false;
   if (!S->cfg.eager && !ensure_iteration_graph(S)) return false;
   // every suspension stops by itself after at most maxit Arnoldi steps and

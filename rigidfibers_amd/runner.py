@@ -73,6 +73,8 @@ def _add_common(p, platform, command):
         dims.add_argument("--D3", action="store_true")
     p.add_argument("--velocity-integration", choices=["follow", "numerical", "analytical"], default="follow")
     p.add_argument("--initial-guess", choices=["previous", "zero", "extrapolated"], default="previous")
+    p.add_argument("--confirm-residual", action="store_true",
+                   help="GMRES: confirm a converged restart cycle with the true residual, as the Fortran solver does")
     p.add_argument("--output-dir", default=".")
     if command == "validate":
         p.add_argument("--reference-dir", default=None,
@@ -109,7 +111,8 @@ def _make_sim(args, params, pos, ori):
     from .simulation import Simulation
 
     return Simulation(params, pos, ori, solver=_solver(args), integration=_integration(args),
-                      velocity_integration=args.velocity_integration, initial_guess=args.initial_guess)
+                      velocity_integration=args.velocity_integration, initial_guess=args.initial_guess,
+                      confirm_residual=args.confirm_residual)
 
 
 def _report_solver(sim, verbose):

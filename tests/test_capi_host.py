@@ -43,6 +43,7 @@ def test_config_struct_layout_and_defaults():
     assert (cfg.gmres_restart, cfg.gmres_max_iterations, cfg.gmres_tolerance) == (10, 1000, 1e-5)
     assert (cfg.bicgstab_max_iterations, cfg.bicgstab_tolerance) == (1000, 1e-5)
     assert cfg.initial_guess == 1
+    assert (cfg.separate_residual, cfg.confirm_residual) == (0, 0)
     assert C.sizeof(_lib.RfConfig) == 152  # sizeof(rf_config) in C (gcc, x86-64)
     assert lib.rf_version().decode().startswith("rigidfibers_amd")
     assert lib.rf_config_init(None) == _lib.RF_ERR_INVALID_ARGUMENT

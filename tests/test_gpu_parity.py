@@ -320,6 +320,43 @@ def test_lu_trailing_updates_library_and_builtin(solver):
     assert e < 1e-9
 
 
+@pytest.mark.parametrize("matrix_format", ["theta", "dense"])
+def test_gmres_stops_on_the_cycle_estimate(matrix_format):
+    """A restart cycle whose own residual estimate meets the tolerance ends
+    the solve after its update of x (lib/viennacl/linalg/gmres.hpp:330-335);
+    confirm_residual computes the true residual of that x first, as
+    fortran/sgmres.f does with icntl(8) = 1.  The same x either way, one
+    matvec more per solve with confirmation, and the estimate is the true
+    residual to far below the tolerance."""
+    M = 120
+    pos, ori = random_scene(M, side=6.0, seed=23, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_TOLERANCE = 1e-8
+    steps = 3
+    out = {}
+    for confirm in (False, True):
+        with _sim(params, pos, ori, solver="gmres", integration="analytical", matrix_format=matrix_format,
+                  confirm_residual=confirm) as sim:
+            res, matvecs = [], 0
+            for t in range(steps):
+                sim.step(t)
+                st = sim.solver_stats()
+                assert st["converged"] and st["iterations"] > 0
+                res.append((st["iterations"], st["relative_residual"]))
+                matvecs += sim.timings()["gemv_launches"]  # the step's own count
+            out[confirm] = (sim.state(), res, matvecs)
+    (s0, r0, g0), (s1, r1, g1) = out[False], out[True]
+    print(f"{matrix_format}: matvecs {g0} / {g1}, residuals (estimate / true) "
+          + ", ".join(f"{a[1]:.6e} / {b[1]:.6e}" for a, b in zip(r0, r1)))
+    assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
+    assert [a[0] for a in r0] == [b[0] for b in r1]
+    assert g1 - g0 == steps
+    for (_, est), (_, true) in zip(r0, r1):
+        assert est <= 1e-8 and true <= 1e-8
+        assert abs(est - true) <= 1e-3 * 1e-8
+
+
 def test_gmres_ir_uses_the_fp16_copy():
     """GMRES-IR runs its inner iterations on the FP16 copy and reaches the
     outer tolerance on the FP32 system; its states follow plain GMRES."""

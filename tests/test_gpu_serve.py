@@ -91,6 +91,23 @@ def test_served_matches_simulation(integration):
         assert ep < 1e-8 and eo < 1e-8
 
 
+def test_served_confirm_residual_same_result():
+    """With confirm_residual the device-side control computes the true
+    residual after a converged cycle before it stops; the result and the
+    Arnoldi step counts are those of stopping on the cycle's estimate."""
+    from rigidfibers_amd.serve import Server
+
+    reqs = _requests([23, 64, 45], steps=3, seed=9)
+    out = {}
+    for confirm in (False, True):
+        with Server(_params(), max_active=4, confirm_residual=confirm) as srv:
+            out[confirm] = _serve(srv, reqs)
+    for a, b in zip(out[False], out[True]):
+        assert a["info"]["status"] == 0 and b["info"]["status"] == 0
+        assert a["info"]["gmres_iterations"] == b["info"]["gmres_iterations"] > 3
+        assert np.array_equal(a["positions"], b["positions"]) and np.array_equal(a["orientations"], b["orientations"])
+
+
 def test_memory_budget_and_slots():
     from rigidfibers_amd.serve import Server
 
