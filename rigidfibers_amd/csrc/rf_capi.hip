@@ -4,13 +4,15 @@
 //
 // Solvers.  GMRES(m) follows the reference semantics (x0 = 0, no
 // preconditioner, restart m, stop on ||b - A x|| / ||b|| <= tol with the true
-// residual recomputed at every restart: fortran/solve_system.incl:18-57,
-// fortran/sgmres.f:1240-1358; lib/viennacl/linalg/gmres.hpp:169-336).  The
-// Krylov basis, all vectors and the matvec accumulation are FP64 on the device
-// (A stays FP32); the basis is orthogonalised with classical Gram-Schmidt
-// applied twice, which spans the same space as the reference's MGS/Householder
-// variants, and the (m+1) x m Hessenberg least-squares problem is solved on
-// the host, one device->host copy per iteration.  BiCGStab follows
+// residual recomputed at every restart and a converged cycle's estimate
+// ending the solve: lib/viennacl/linalg/gmres.hpp:169-336;
+// fortran/solve_system.incl:18-57, fortran/sgmres.f:1240-1358).  The Krylov
+// basis, all vectors and the matvec accumulation are FP64 on the device (A
+// stays FP32); the basis is orthogonalised with classical Gram-Schmidt
+// applied twice, which spans the same space as the reference's
+// MGS/Householder variants, and the (m+1) x m Hessenberg least-squares
+// problem is solved on the device as well: the host enqueues a whole restart
+// cycle and waits once per cycle.  BiCGStab follows
 // lib/viennacl/linalg/bicgstab.hpp:92-160.
 #include <hip/hip_runtime.h>
 
@@ -108,6 +110,7 @@ struct rf_simulation {
   double* dpart = nullptr;
   double* hdev = nullptr;
   int* flag = nullptr;
+  rf::CycleCtl cyc{};  // GMRES: the device control of a restart cycle
   rf::GemvPlan plan{};
   rf::LuWorkspace lu{};
   rf::LuResult lu_last{};  // factorisation statistics of the last direct solve
@@ -227,7 +230,7 @@ int solver_kdim(const rf_config& c) {
 // y = A x or y = b - A x on the simulation's stream, timed with an event pair
 // while the pool lasts.
 hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, const double* b, int mode,
-                    bool half = false) {
+                    bool half = false, const int* skip = nullptr) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (half) {
     if (s->nhalf < rf_simulation::kGemvEvents) {
@@ -236,7 +239,7 @@ hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, 
       ++s->nhalf;
     }
     return rf::gemv_theta16(s->N, s->M, s->Lp, s->T16, s->S16, s->ori[s->cur], x, y, b, mode, s->pc, s->stream, e0,
-                            e1);
+                            e1, skip);
   }
   if (s->ngemv < rf_simulation::kGemvEvents) {
     e0 = s->gev[2 * s->ngemv];
@@ -244,8 +247,8 @@ hipError_t sim_gemv(rf_simulation* s, const double* x, double* part, double* y, 
     ++s->ngemv;
   }
   if (s->theta)
-    return rf::gemv_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], x, y, b, mode, s->pc, s->stream, e0, e1);
-  return rf::gemv(s->plan, s->A, x, part, y, b, mode, s->stream, e0, e1);
+    return rf::gemv_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], x, y, b, mode, s->pc, s->stream, e0, e1, skip);
+  return rf::gemv(s->plan, s->A, x, part, y, b, mode, s->stream, e0, e1, skip);
 }
 
 rf_status check_nonfinite(rf_simulation* s, const double* v, int64_t len, const char* what) {
@@ -303,8 +306,6 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
     out->converged = 1;
     return RF_OK;
   }
-  std::vector<double> H(static_cast<size_t>(m + 1) * m, 0.0), cs(m), sn(m), g(m + 1), y(m);
-  std::vector<double> hbuf(2 * (m + 1) + 2);
   int it = 0;
   bool first = true, converged = false;
   double res = bnorm;
@@ -331,54 +332,26 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
     }
     if (it >= maxit) break;
     RF_HIP(rf::axpby(n, 1.0 / beta, Vk(0), 0.0, Vk(0), s->stream));
-    std::fill(g.begin(), g.end(), 0.0);
-    g[0] = beta;
-    int k_used = 0;
-    for (int k = 0; k < m && it < maxit; ++k) {
-      ++it;
-      double* w = Vk(k + 1);
-      RF_HIP(sim_gemv(s, Vk(k), s->gpart, w, nullptr, 0, half));
-      // h1 = V^T w, w -= V h1, h2 = V^T w, w -= V h2, ||w||^2
-      double* h1 = s->hdev;
-      double* h2 = s->hdev + (m + 1);
-      double* ww = s->hdev + 2 * (m + 1);
-      RF_HIP(rf::mdot(n, k + 1, V, n, w, s->dpart, h1, s->stream));
-      RF_HIP(rf::axpy_multi(n, k + 1, V, n, h1, -1.0, w, s->stream));
-      RF_HIP(rf::mdot(n, k + 1, V, n, w, s->dpart, h2, s->stream));
-      RF_HIP(rf::axpy_multi(n, k + 1, V, n, h2, -1.0, w, s->stream));
-      RF_HIP(rf::mdot(n, 1, w, n, w, s->dpart, ww, s->stream));
-      RF_HIP(hipMemcpyAsync(hbuf.data(), s->hdev, sizeof(double) * (2 * (m + 1) + 1), hipMemcpyDeviceToHost,
-                            s->stream));
-      RF_HIP(hipStreamSynchronize(s->stream));
-      const double hn = std::sqrt(hbuf[2 * (m + 1)]);
-      if (!std::isfinite(hn)) return fail(RF_ERR_NONFINITE, "non-finite Arnoldi vector in GMRES");
-      for (int i = 0; i <= k; ++i) H[i + static_cast<size_t>(k) * (m + 1)] = hbuf[i] + hbuf[(m + 1) + i];
-      H[(k + 1) + static_cast<size_t>(k) * (m + 1)] = hn;
-      if (hn > 0.0) RF_HIP(rf::axpby(n, 1.0 / hn, w, 0.0, w, s->stream));
-      for (int i = 0; i < k; ++i) {
-        const double a0 = H[i + static_cast<size_t>(k) * (m + 1)], a1 = H[i + 1 + static_cast<size_t>(k) * (m + 1)];
-        H[i + static_cast<size_t>(k) * (m + 1)] = cs[i] * a0 + sn[i] * a1;
-        H[i + 1 + static_cast<size_t>(k) * (m + 1)] = -sn[i] * a0 + cs[i] * a1;
-      }
-      const double a0 = H[k + static_cast<size_t>(k) * (m + 1)], a1 = H[k + 1 + static_cast<size_t>(k) * (m + 1)];
-      const double rr = std::hypot(a0, a1);
-      cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
-      sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
-      H[k + static_cast<size_t>(k) * (m + 1)] = rr;
-      H[k + 1 + static_cast<size_t>(k) * (m + 1)] = 0.0;
-      g[k + 1] = -sn[k] * g[k];
-      g[k] = cs[k] * g[k];
-      k_used = k + 1;
-      res = std::fabs(g[k + 1]);
-      if (res / bnorm <= tol || hn == 0.0) break;
+    // The cycle's steps are enqueued at once and the host waits once: the
+    // control runs on the device, and the products and vector kernels of the
+    // steps after the one that ends the cycle return at once.
+    const int steps = std::min(m, maxit - it);
+    int& launched = half ? s->nhalf : s->ngemv;
+    const int launched0 = launched;
+    RF_HIP(rf::cycle_start(s->cyc, m, beta, s->stream));
+    for (int k = 0; k < steps; ++k) {
+      RF_HIP(sim_gemv(s, Vk(k), s->gpart, Vk(k + 1), nullptr, 0, half, &s->cyc.head->done));
+      RF_HIP(rf::arnoldi_step(n, k, steps - 1, m, V, Vk(k + 1), s->dpart, tol, bnorm, s->cyc, s->stream));
     }
-    for (int i = k_used - 1; i >= 0; --i) {
-      double acc = g[i];
-      for (int j = i + 1; j < k_used; ++j) acc -= H[i + static_cast<size_t>(j) * (m + 1)] * y[j];
-      y[i] = acc / H[i + static_cast<size_t>(i) * (m + 1)];
-    }
-    RF_HIP(hipMemcpyAsync(s->hdev, y.data(), sizeof(double) * k_used, hipMemcpyHostToDevice, s->stream));
-    RF_HIP(rf::axpy_multi(n, k_used, V, n, s->hdev, 1.0, x, s->stream));
+    RF_HIP(rf::cycle_update(n, V, s->cyc, x, s->stream));
+    rf::CycleHead hd{};
+    RF_HIP(hipMemcpyAsync(&hd, s->cyc.head, sizeof(hd), hipMemcpyDeviceToHost, s->stream));
+    RF_HIP(hipStreamSynchronize(s->stream));
+    // products that returned at once are not counted (nor timed)
+    launched = std::min(launched, launched0 + hd.k_used + hd.nonfinite);
+    it += hd.k_used + hd.nonfinite;
+    if (hd.nonfinite) return fail(RF_ERR_NONFINITE, "non-finite Arnoldi vector in GMRES");
+    res = hd.res;
     if (!confirm && res / bnorm <= tol) {
       converged = true;
       break;
@@ -717,7 +690,15 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
   if (st == RF_OK && s->kdim > 0) st = s->alloc(&s->V, static_cast<size_t>(s->kdim) * n);
   if (st == RF_OK) st = s->alloc(&s->work, 6 * static_cast<size_t>(n));
   const int kmax = (s->kdim > 2 ? s->kdim : 2) + 1;
-  if (st == RF_OK) st = s->alloc(&s->dpart, static_cast<size_t>(rf::mdot_blocks(n)) * kmax);
+  size_t dpart_len = static_cast<size_t>(rf::mdot_blocks(n)) * kmax;
+  if (s->kdim > 0) {
+    const int m = cfg->gmres_restart;
+    dpart_len = std::max(dpart_len, rf::cycle_part_doubles(n, m));
+    double* mem = nullptr;
+    if (st == RF_OK) st = s->alloc(&mem, rf::cycle_ctl_doubles(m));
+    if (st == RF_OK) s->cyc = rf::cycle_ctl(mem, m);
+  }
+  if (st == RF_OK) st = s->alloc(&s->dpart, dpart_len);
   if (st == RF_OK) st = s->alloc(&s->hdev, static_cast<size_t>(2 * kmax + 4));
   if (st == RF_OK) st = s->alloc(&s->flag, 1);
   if (st != RF_OK) return cleanup(st);

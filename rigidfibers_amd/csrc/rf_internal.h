@@ -54,11 +54,12 @@ hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const d
 hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s);
 hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
                         const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,
-                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
-// y = A x (mode 0) or y = b - A x (mode 1) with A in theta form.
+                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* skip = nullptr);
+// y = A x (mode 0) or y = b - A x (mode 1) with A in theta form.  The
+// products return at once when *skip (device memory) is set.
 hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
                       const double* b, int mode, const PhysConsts& pc, hipStream_t s, hipEvent_t ev_begin = nullptr,
-                      hipEvent_t ev_end = nullptr);
+                      hipEvent_t ev_end = nullptr, const int* skip = nullptr);
 hipError_t launch_velocities(int N, int NQI, bool analytical, int M, const double* pos, const double* ori,
                              const double* x, float* wf, double* fmono, double* vpart, double* tvel, double* rvel,
                              const PhysConsts& pc, hipStream_t s);
@@ -163,7 +164,8 @@ size_t gemv_partial_elems(const GemvPlan& p);
 // y = A x  (mode 0) or y = b - A x (mode 1); A FP32 column-major.  Optional
 // events are recorded around the streaming kernel (not the chunk reduction).
 hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part, double* y, const double* b,
-                int mode, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                int mode, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr,
+                const int* skip = nullptr);
 // out[k] = <V_k, w> for k < K (V column stride ldv), partial buffer of
 // mdot_blocks(n) * K doubles.
 int mdot_blocks(int64_t n);
@@ -176,5 +178,41 @@ hipError_t axpy_multi(int64_t n, int K, const double* V, int64_t ldv, const doub
 hipError_t axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s);
 hipError_t fill(int64_t n, double value, double* y, hipStream_t s);
 hipError_t nonfinite_count(int64_t n, const double* x, int* flag_dev, hipStream_t s);
+
+// One restart cycle of single-simulation GMRES(m) with its control on the
+// device (rf_linalg.hip): the host enqueues every step of the cycle and waits
+// once; the step that ends the cycle sets head.done, and the products and
+// vector kernels of the later steps then return at once.
+struct CycleHead {
+  int done;       // the cycle has ended
+  int k_used;     // its Arnoldi steps
+  int nonfinite;  // a non-finite ||w|| ended it
+  int pad;
+  double scale;   // 1 / ||w|| of the last step: normalises V_{k+1}
+  double res;     // |g_k|, the cycle's residual estimate
+};
+struct CycleCtl {
+  CycleHead* head;
+  double* H;  // (m + 1) x m Hessenberg matrix, column k at H + k (m + 1)
+  double* cs;
+  double* sn;
+  double* g;  // m + 1
+  double* y;  // m
+};
+// Device doubles of a cycle's control (head included), and its layout in them.
+size_t cycle_ctl_doubles(int m);
+CycleCtl cycle_ctl(double* mem, int m);
+// Dot partials of the steps: 3 (m + 1) mdot_blocks(n) doubles.
+size_t cycle_part_doubles(int64_t n, int m);
+// g = (beta, 0, ..., 0), head cleared.
+hipError_t cycle_start(const CycleCtl& c, int m, double beta, hipStream_t s);
+// Step k on w = A V_k (= V_{k+1}, V column stride n): Gram-Schmidt twice
+// against V_0..V_k, the Hessenberg column and its Givens rotation, and the
+// end test ||r|| <= tol ||b||, a breakdown ||w|| = 0 or k = klast, where
+// y = H^-1 g is formed; else w /= ||w||.
+hipError_t arnoldi_step(int64_t n, int k, int klast, int m, const double* V, double* w, double* part, double tol,
+                        double bnorm, const CycleCtl& c, hipStream_t s);
+// x += sum_{k < k_used} V_k y_k
+hipError_t cycle_update(int64_t n, const double* V, const CycleCtl& c, double* x, hipStream_t s);
 
 }  // namespace rf

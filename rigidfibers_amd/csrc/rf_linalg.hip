@@ -22,7 +22,9 @@ constexpr int kGemvRows = 4 * kGemvThreads;
 
 __global__ __launch_bounds__(kGemvThreads) void gemv_f32_kernel(int64_t n, int64_t lda, const float* __restrict__ A,
                                                                 const double* __restrict__ x,
-                                                                double* __restrict__ part, int cols_per_chunk) {
+                                                                double* __restrict__ part, int cols_per_chunk,
+                                                                const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
   const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kGemvRows + 4 * threadIdx.x;
   const int64_t c0 = static_cast<int64_t>(blockIdx.y) * cols_per_chunk;
   int64_t c1 = c0 + cols_per_chunk;
@@ -81,7 +83,8 @@ __global__ __launch_bounds__(kGemvThreads) void gemv_f32_kernel(int64_t n, int64
 
 // y[r] = sum_k part[k][r]  (mode 0)   or   y[r] = b[r] - sum_k part[k][r]  (mode 1)
 __global__ void gemv_reduce_kernel(int64_t n, int chunks, const double* __restrict__ part, double* __restrict__ y,
-                                   const double* __restrict__ b, int mode) {
+                                   const double* __restrict__ b, int mode, const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (r >= n) return;
   double s = 0.0;
@@ -107,13 +110,13 @@ GemvPlan gemv_plan(int64_t n, int64_t lda) {
 size_t gemv_partial_elems(const GemvPlan& p) { return static_cast<size_t>(p.chunks) * static_cast<size_t>(p.n); }
 
 hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part, double* y, const double* b,
-                int mode, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                int mode, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end, const int* skip) {
   if (ev_begin) (void)hipEventRecord(ev_begin, s);
   hipLaunchKernelGGL(gemv_f32_kernel, dim3(p.row_tiles, p.chunks), dim3(kGemvThreads), 0, s, p.n, p.lda, A, x, part,
-                     p.cols_per_chunk);
+                     p.cols_per_chunk, skip);
   if (ev_end) (void)hipEventRecord(ev_end, s);
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3(static_cast<unsigned>((p.n + 255) / 256)), dim3(256), 0, s, p.n,
-                     p.chunks, part, y, b, mode);
+                     p.chunks, part, y, b, mode, skip);
   return hipGetLastError();
 }
 
@@ -196,7 +199,9 @@ template <int N>
 __global__ __launch_bounds__(256) void gemv_theta_kernel(int M, int64_t Lp, const float* __restrict__ T,
                                                          const double* __restrict__ ori,
                                                          const double* __restrict__ x, double* __restrict__ y,
-                                                         const double* __restrict__ b, int mode, PhysConsts pc) {
+                                                         const double* __restrict__ b, int mode, PhysConsts pc,
+                                                         const int* __restrict__ skip) {
+  if (skip != nullptr && *skip) return;
   gemv_theta_block<N>(blockIdx.x, M, Lp, T, ori, x, y, b, mode, pc);
 }
 
@@ -238,12 +243,12 @@ hipError_t launch_gemv_theta_batch(int N, int ntargets, const SusDev* sus, const
 
 hipError_t gemv_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const double* x, double* y,
                       const double* b, int mode, const PhysConsts& pc, hipStream_t s, hipEvent_t ev_begin,
-                      hipEvent_t ev_end) {
+                      hipEvent_t ev_end, const int* skip) {
   if (ev_begin) (void)hipEventRecord(ev_begin, s);
   switch (N) {
-#define RF_GT(NN)                                                                                                 \
-  case NN:                                                                                                        \
-    hipLaunchKernelGGL(gemv_theta_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T, ori, x, y, b, mode, pc); \
+#define RF_GT(NN)                                                                                               \
+  case NN:                                                                                                      \
+    hipLaunchKernelGGL(gemv_theta_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T, ori, x, y, b, mode, pc, skip); \
     break;
     RF_GT(1)
     RF_GT(2)
@@ -289,8 +294,10 @@ int mdot_blocks(int64_t n) {
 __global__ __launch_bounds__(kDotThreads) void mdot_partial_kernel(int64_t n, int64_t rpb,
                                                                    const double* __restrict__ V, int64_t ldv,
                                                                    const double* __restrict__ w,
-                                                                   double* __restrict__ part) {
+                                                                   double* __restrict__ part,
+                                                                   const int* __restrict__ skip) {
   __shared__ double sh[kDotThreads / 64];
+  if (skip != nullptr && *skip) return;
   const int k = blockIdx.y;
   const int64_t lo = static_cast<int64_t>(blockIdx.x) * rpb;
   int64_t hi = lo + rpb;
@@ -316,7 +323,8 @@ hipError_t mdot(int64_t n, int K, const double* V, int64_t ldv, const double* w,
   if (K <= 0) return hipSuccess;
   const int nb = mdot_blocks(n);
   const int64_t rpb = (n + nb - 1) / nb;
-  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, K), dim3(kDotThreads), 0, s, n, rpb, V, ldv, w, part);
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, K), dim3(kDotThreads), 0, s, n, rpb, V, ldv, w, part,
+                     static_cast<const int*>(nullptr));
   hipLaunchKernelGGL(mdot_final_kernel, dim3(K), dim3(64), 0, s, nb, part, out);
   return hipGetLastError();
 }
@@ -487,13 +495,13 @@ hipError_t launch_orth_dots_batch(int nvb, int kdim, int phase, const SusDev* su
 // restart, classical Gram-Schmidt twice (the two passes' coefficients summed),
 // Givens rotations, stop on ||r|| <= tol ||b||, a cycle whose estimate meets
 // it ending the solve with its update of x (confirm: with the true residual
-// of that x).  The lanes form the
-// iteration's reductions from the block partials (at the start step, the
-// norms of b and of the first residual from the assembly's per-target
-// partials), lane 0 advances the state and writes the next iteration's
-// operations into it[q] and whether it has any into active[q].
-__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit, int confirm,
-                                                                 const SusDev* __restrict__ sus,
+// of that x).  The lanes form the iteration's reductions from the block
+// partials (at the start step, the norms of b and of the first residual from
+// the assembly's per-target partials), lane 0 advances the state and writes
+// the next iteration's operations into it[q] and whether it has any into
+// active[q].
+__global__ __launch_bounds__(64) void gmres_control_batch_kernel(int m, int kdim, int nvb, double tol, int maxit,
+                                                                 int confirm, const SusDev* __restrict__ sus,
                                                                  SusIter* __restrict__ it, SusCtl* __restrict__ ctl,
                                                                  const double* __restrict__ part,
                                                                  int* __restrict__ active) {
@@ -751,8 +759,10 @@ __global__ __launch_bounds__(256) void gemv_theta16_kernel(int M, int64_t Lp, co
                                                            const float* __restrict__ S,
                                                            const double* __restrict__ ori,
                                                            const double* __restrict__ x, double* __restrict__ y,
-                                                           const double* __restrict__ b, int mode, PhysConsts pc) {
+                                                           const double* __restrict__ b, int mode, PhysConsts pc,
+                                                           const int* __restrict__ skip) {
   typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  if (skip != nullptr && *skip) return;
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int NACC = 1 + 3 * (N - 1);
   __shared__ double red[256 / 64][NACC];
@@ -829,13 +839,13 @@ __global__ __launch_bounds__(256) void gemv_theta16_kernel(int M, int64_t Lp, co
 
 hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
                         const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,
-                        hipEvent_t ev_begin, hipEvent_t ev_end) {
+                        hipEvent_t ev_begin, hipEvent_t ev_end, const int* skip) {
   if (ev_begin) (void)hipEventRecord(ev_begin, s);
   switch (N) {
 #define RF_G16(NN)                                                                                            \
   case NN:                                                                                                    \
     hipLaunchKernelGGL(gemv_theta16_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T16, S, ori, x, y, b, mode, \
-                       pc);                                                                                   \
+                       pc, skip);                                                                             \
     break;
     RF_G16(1)
     RF_G16(2)
@@ -850,6 +860,170 @@ hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const flo
       return hipErrorInvalidValue;
   }
   if (ev_end) (void)hipEventRecord(ev_end, s);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Single-simulation GMRES(m), one restart cycle enqueued at once with its
+// control on the device (the reference's solver reads and writes the
+// Hessenberg entries on the host at every step, lib/viennacl/linalg/
+// gmres.hpp:246,281-285).  A step is 6 launches after its product: the dots
+// and the update of each Gram-Schmidt pass (the update forms the
+// coefficients from the dots' block partials itself), ||w||^2, the control
+// step and the normalisation.  Each coefficient is the sum of its block
+// partials in block order, the same in the update and in the control.
+
+size_t cycle_ctl_doubles(int m) {
+  return sizeof(CycleHead) / sizeof(double) + static_cast<size_t>(m + 1) * m + 4 * static_cast<size_t>(m) + 1;
+}
+
+CycleCtl cycle_ctl(double* mem, int m) {
+  static_assert(sizeof(CycleHead) % sizeof(double) == 0, "CycleHead is a whole number of doubles");
+  CycleCtl c;
+  c.head = reinterpret_cast<CycleHead*>(mem);
+  c.H = mem + sizeof(CycleHead) / sizeof(double);
+  c.cs = c.H + static_cast<int64_t>(m + 1) * m;
+  c.sn = c.cs + m;
+  c.g = c.sn + m;
+  c.y = c.g + m + 1;
+  return c;
+}
+
+size_t cycle_part_doubles(int64_t n, int m) { return 3 * static_cast<size_t>(m + 1) * mdot_blocks(n); }
+
+__device__ __forceinline__ double sum_parts(const double* __restrict__ p, int nb) {
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += p[b];
+  return s;
+}
+
+__global__ void cycle_start_kernel(CycleCtl c, int m, double beta) {
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) c.g[i] = (i == 0) ? beta : 0.0;
+  if (threadIdx.x == 0) {
+    c.head->done = 0;
+    c.head->k_used = 0;
+    c.head->nonfinite = 0;
+    c.head->pad = 0;
+    c.head->scale = 1.0;
+    c.head->res = beta;
+  }
+}
+
+// w -= sum_{k < K} V_k h_k with h_k the sum of the dots' block partials of V_k.
+__global__ __launch_bounds__(256) void cgs_update_kernel(int64_t n, int K, const double* __restrict__ V, int64_t ldv,
+                                                         const double* __restrict__ part, int nb,
+                                                         double* __restrict__ w, const int* __restrict__ skip) {
+  extern __shared__ double h[];
+  if (*skip) return;
+  for (int k = threadIdx.x; k < K; k += blockDim.x) h[k] = sum_parts(part + static_cast<int64_t>(k) * nb, nb);
+  __syncthreads();
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < K; ++k) s = fma(V[static_cast<int64_t>(k) * ldv + r], h[k], s);
+  w[r] = fma(-1.0, s, w[r]);
+}
+
+// The control of step k: H(:, k) = h0 + h1 and ||w||, the earlier rotations
+// and a new one, g and the estimate |g_{k+1}|; at the cycle's end y = H^-1 g.
+__global__ __launch_bounds__(64) void arnoldi_step_kernel(int k, int klast, int m, int nb,
+                                                          const double* __restrict__ part, double tol, double bnorm,
+                                                          CycleCtl c) {
+  extern __shared__ double hs[];  // h0[K], h1[K], ||w||^2
+  CycleHead& hd = *c.head;
+  if (hd.done) return;
+  const int K = k + 1;
+  const int64_t region = static_cast<int64_t>(m + 1) * nb;
+  for (int t = threadIdx.x; t <= 2 * K; t += blockDim.x) {
+    const int p = t < K ? 0 : (t < 2 * K ? 1 : 2);
+    const int j = t < K ? t : (t < 2 * K ? t - K : 0);
+    hs[t] = sum_parts(part + p * region + static_cast<int64_t>(j) * nb, nb);
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double hn = sqrt(hs[2 * K]);
+  if (!isfinite(hn)) {
+    hd.nonfinite = 1;
+    hd.k_used = k;
+    hd.done = 1;
+    return;
+  }
+  double* Hk = c.H + static_cast<int64_t>(k) * (m + 1);
+  for (int i = 0; i <= k; ++i) Hk[i] = hs[i] + hs[K + i];
+  Hk[k + 1] = hn;
+  for (int i = 0; i < k; ++i) {
+    const double a0 = Hk[i], a1 = Hk[i + 1];
+    Hk[i] = c.cs[i] * a0 + c.sn[i] * a1;
+    Hk[i + 1] = -c.sn[i] * a0 + c.cs[i] * a1;
+  }
+  const double a0 = Hk[k], a1 = Hk[k + 1];
+  const double rr = hypot(a0, a1);
+  c.cs[k] = (rr == 0.0) ? 1.0 : a0 / rr;
+  c.sn[k] = (rr == 0.0) ? 0.0 : a1 / rr;
+  Hk[k] = rr;
+  Hk[k + 1] = 0.0;
+  c.g[k + 1] = -c.sn[k] * c.g[k];
+  c.g[k] = c.cs[k] * c.g[k];
+  const double res = fabs(c.g[k + 1]);
+  hd.res = res;
+  if (res / bnorm <= tol || hn == 0.0 || k == klast) {
+    for (int i = k; i >= 0; --i) {
+      double acc = c.g[i];
+      for (int j = i + 1; j <= k; ++j) acc -= c.H[i + static_cast<int64_t>(j) * (m + 1)] * c.y[j];
+      c.y[i] = acc / c.H[i + static_cast<int64_t>(i) * (m + 1)];
+    }
+    hd.k_used = K;
+    hd.done = 1;
+  } else {
+    hd.scale = 1.0 / hn;
+  }
+}
+
+__global__ void arnoldi_scale_kernel(int64_t n, double* __restrict__ w, const CycleHead* __restrict__ hd) {
+  if (hd->done) return;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r < n) w[r] = hd->scale * w[r];
+}
+
+__global__ void cycle_update_kernel(int64_t n, const double* __restrict__ V, CycleCtl c, double* __restrict__ x) {
+  const int K = c.head->k_used;
+  if (c.head->nonfinite) return;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < K; ++k) s = fma(V[static_cast<int64_t>(k) * n + r], c.y[k], s);
+  x[r] = fma(1.0, s, x[r]);
+}
+
+hipError_t cycle_start(const CycleCtl& c, int m, double beta, hipStream_t s) {
+  hipLaunchKernelGGL(cycle_start_kernel, dim3(1), dim3(64), 0, s, c, m, beta);
+  return hipGetLastError();
+}
+
+hipError_t arnoldi_step(int64_t n, int k, int klast, int m, const double* V, double* w, double* part, double tol,
+                        double bnorm, const CycleCtl& c, hipStream_t s) {
+  if (k < 0 || k > klast || klast >= m) return hipErrorInvalidValue;
+  const int nb = mdot_blocks(n);
+  const int64_t rpb = (n + nb - 1) / nb;
+  const int64_t region = static_cast<int64_t>(m + 1) * nb;
+  const int K = k + 1;
+  const unsigned vb = static_cast<unsigned>((n + 255) / 256);
+  const int* skip = &c.head->done;  // address arithmetic only
+  for (int pass = 0; pass < 2; ++pass) {
+    double* pp = part + pass * region;
+    hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, K), dim3(kDotThreads), 0, s, n, rpb, V, n, w, pp, skip);
+    hipLaunchKernelGGL(cgs_update_kernel, dim3(vb), dim3(256), K * sizeof(double), s, n, K, V, n, pp, nb, w, skip);
+  }
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 1), dim3(kDotThreads), 0, s, n, rpb, w, n, w, part + 2 * region,
+                     skip);
+  hipLaunchKernelGGL(arnoldi_step_kernel, dim3(1), dim3(64), (2 * K + 1) * sizeof(double), s, k, klast, m, nb, part,
+                     tol, bnorm, c);
+  hipLaunchKernelGGL(arnoldi_scale_kernel, dim3(vb), dim3(256), 0, s, n, w, c.head);
+  return hipGetLastError();
+}
+
+hipError_t cycle_update(int64_t n, const double* V, const CycleCtl& c, double* x, hipStream_t s) {
+  hipLaunchKernelGGL(cycle_update_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, V, c, x);
   return hipGetLastError();
 }
 

@@ -344,7 +344,13 @@ def test_gmres_stops_on_the_cycle_estimate(matrix_format):
                 st = sim.solver_stats()
                 assert st["converged"] and st["iterations"] > 0
                 res.append((st["iterations"], st["relative_residual"]))
-                matvecs += sim.timings()["gemv_launches"]  # the step's own count
+                mv = sim.timings()["gemv_launches"]  # the step's own count
+                # the steps' products, one true residual per restart (and per
+                # warm start on the dense form), one per solve to confirm;
+                # the products of steps after a cycle's end are not counted
+                its, m = st["iterations"], params.GMRES_RESTART
+                assert mv == its + (its - 1) // m + (t > 0 and matrix_format == "dense") + confirm
+                matvecs += mv
             out[confirm] = (sim.state(), res, matvecs)
     (s0, r0, g0), (s1, r1, g1) = out[False], out[True]
     print(f"{matrix_format}: matvecs {g0} / {g1}, residuals (estimate / true) "
@@ -355,6 +361,41 @@ def test_gmres_stops_on_the_cycle_estimate(matrix_format):
     for (_, est), (_, true) in zip(r0, r1):
         assert est <= 1e-8 and true <= 1e-8
         assert abs(est - true) <= 1e-3 * 1e-8
+
+
+@pytest.mark.parametrize("matrix_format", ["theta", "dense"])
+def test_gmres_restart_cycles_and_iteration_cap(matrix_format):
+    """Restart cycles of 3 steps, each enqueued at once with its control on
+    the device: the solve reaches a tight true residual over many cycles
+    (confirmed, so the reported residual is ||b - A x|| / ||b|| itself) and
+    agrees with the FP64-refined LU solution to the FP32 assembly's rounding;
+    capped at 5 steps, it stops after a full cycle and a 2-step one,
+    unconverged, with the true residual of that x."""
+    M = 40
+    pos, ori = random_scene(M, side=5.0, seed=31, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_RESTART = 3
+    params.GMRES_TOLERANCE = 1e-11
+    with _sim(params, pos, ori, solver="mixed-bf16", refinement_tolerance=1e-12) as sim:
+        sim.step(0)
+        _, _, x_ref = sim.system()
+    with _sim(params, pos, ori, solver="gmres", matrix_format=matrix_format, confirm_residual=True) as sim:
+        sim.step(0)
+        _, _, x = sim.system()
+        st, mv = sim.solver_stats(), sim.timings()["gemv_launches"]
+    its = st["iterations"]
+    print(f"{matrix_format}: {its} steps, {mv} products, residual {st['relative_residual']:.2e}, "
+          f"x vs LU {rel_err(x, x_ref):.1e}")
+    assert st["converged"] and st["relative_residual"] <= 1e-11 and its > 6
+    assert mv == its + (its - 1) // 3 + 1  # steps, restarts, confirmation
+    assert rel_err(x, x_ref) < 1e-5
+    params.GMRES_MAX_ITERATIONS = 5
+    with _sim(params, pos, ori, solver="gmres", matrix_format=matrix_format) as sim:
+        sim.step(0)
+        st, mv = sim.solver_stats(), sim.timings()["gemv_launches"]
+    assert st["iterations"] == 5 and not st["converged"] and st["relative_residual"] > 1e-11
+    assert mv == 5 + 2  # the products of the steps, and the true residual after each cycle
 
 
 def test_gmres_ir_uses_the_fp16_copy():
