@@ -111,6 +111,8 @@ struct rf_simulation {
   double* hdev = nullptr;
   int* flag = nullptr;
   rf::CycleCtl cyc{};  // GMRES: the device control of a restart cycle
+  int* hdone = nullptr;  // its done flag, mirrored into pinned host memory
+  hipEvent_t cev[2] = {nullptr, nullptr};  // the ends of the last two steps
   rf::GemvPlan plan{};
   rf::LuWorkspace lu{};
   rf::LuResult lu_last{};  // factorisation statistics of the last direct solve
@@ -332,9 +334,11 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
     }
     if (it >= maxit) break;
     RF_HIP(rf::axpby(n, 1.0 / beta, Vk(0), 0.0, Vk(0), s->stream));
-    // The cycle's steps are enqueued at once and the host waits once: the
-    // control runs on the device, and the products and vector kernels of the
-    // steps after the one that ends the cycle return at once.
+    // The control of the cycle runs on the device; the host stays one step
+    // ahead: it enqueues step k + 1, then waits for the end of step k and
+    // reads the done flag that step's control mirrors into host memory.  The
+    // product and vector kernels of a step enqueued after the cycle's end
+    // return at once (at most one per cycle).
     const int steps = std::min(m, maxit - it);
     int& launched = half ? s->nhalf : s->ngemv;
     const int launched0 = launched;
@@ -342,6 +346,11 @@ rf_status gmres_core(rf_simulation* s, const double* b, double* x, bool warm, do
     for (int k = 0; k < steps; ++k) {
       RF_HIP(sim_gemv(s, Vk(k), s->gpart, Vk(k + 1), nullptr, 0, half, &s->cyc.head->done));
       RF_HIP(rf::arnoldi_step(n, k, steps - 1, m, V, Vk(k + 1), s->dpart, tol, bnorm, s->cyc, s->stream));
+      RF_HIP(hipEventRecord(s->cev[k & 1], s->stream));
+      if (k > 0) {
+        RF_HIP(hipEventSynchronize(s->cev[(k - 1) & 1]));
+        if (*static_cast<volatile int*>(s->hdone)) break;
+      }
     }
     RF_HIP(rf::cycle_update(n, V, s->cyc, x, s->stream));
     rf::CycleHead hd{};
@@ -640,6 +649,9 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
   for (auto& e : s->hev)
     if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
+  for (auto& e : s->cev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(RF_ERR_DEVICE, "hipEventCreate failed"));
 
   const int64_t n = s->n;
   const size_t fiber3 = 3 * static_cast<size_t>(M);
@@ -697,6 +709,18 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     double* mem = nullptr;
     if (st == RF_OK) st = s->alloc(&mem, rf::cycle_ctl_doubles(m));
     if (st == RF_OK) s->cyc = rf::cycle_ctl(mem, m);
+    void* hp = nullptr;
+    void* dp = nullptr;
+    if (st == RF_OK && (hipHostMalloc(&hp, 64, hipHostMallocMapped) != hipSuccess ||
+                        hipHostGetDevicePointer(&dp, hp, 0) != hipSuccess)) {
+      if (hp) (void)hipHostFree(hp);
+      (void)hipGetLastError();
+      st = fail(RF_ERR_OUT_OF_MEMORY, "hipHostMalloc of the GMRES cycle flag failed");
+    } else if (st == RF_OK) {
+      s->hdone = static_cast<int*>(hp);
+      *s->hdone = 0;
+      s->cyc.hdone = static_cast<int*>(dp);
+    }
   }
   if (st == RF_OK) st = s->alloc(&s->dpart, dpart_len);
   if (st == RF_OK) st = s->alloc(&s->hdev, static_cast<size_t>(2 * kmax + 4));
@@ -736,6 +760,9 @@ void rf_destroy(rf_simulation* s) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : s->hev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : s->cev)
+    if (e) (void)hipEventDestroy(e);
+  if (s->hdone) (void)hipHostFree(s->hdone);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }

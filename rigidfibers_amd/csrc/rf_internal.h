@@ -180,9 +180,9 @@ hipError_t fill(int64_t n, double value, double* y, hipStream_t s);
 hipError_t nonfinite_count(int64_t n, const double* x, int* flag_dev, hipStream_t s);
 
 // One restart cycle of single-simulation GMRES(m) with its control on the
-// device (rf_linalg.hip): the host enqueues every step of the cycle and waits
-// once; the step that ends the cycle sets head.done, and the products and
-// vector kernels of the later steps then return at once.
+// device (rf_linalg.hip): the step that ends the cycle sets head.done (and
+// *hdone in host memory), and the products and vector kernels of steps
+// enqueued later return at once.
 struct CycleHead {
   int done;       // the cycle has ended
   int k_used;     // its Arnoldi steps
@@ -198,6 +198,7 @@ struct CycleCtl {
   double* sn;
   double* g;  // m + 1
   double* y;  // m
+  int* hdone;  // pinned host memory (device address): head.done mirrored
 };
 // Device doubles of a cycle's control (head included), and its layout in them.
 size_t cycle_ctl_doubles(int m);

@@ -864,10 +864,9 @@ hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const flo
 }
 
 // ---------------------------------------------------------------------------
-// Single-simulation GMRES(m), one restart cycle enqueued at once with its
-// control on the device (the reference's solver reads and writes the
-// Hessenberg entries on the host at every step, lib/viennacl/linalg/
-// gmres.hpp:246,281-285).  A step is 6 launches after its product: the dots
+// Single-simulation GMRES(m), a restart cycle with its control on the
+// device (the reference's solver reads and writes the Hessenberg entries on
+// the host at every step, lib/viennacl/linalg/gmres.hpp:246,281-285).  A step is 6 launches after its product: the dots
 // and the update of each Gram-Schmidt pass (the update forms the
 // coefficients from the dots' block partials itself), ||w||^2, the control
 // step and the normalisation.  Each coefficient is the sum of its block
@@ -886,6 +885,7 @@ CycleCtl cycle_ctl(double* mem, int m) {
   c.sn = c.cs + m;
   c.g = c.sn + m;
   c.y = c.g + m + 1;
+  c.hdone = nullptr;
   return c;
 }
 
@@ -906,6 +906,7 @@ __global__ void cycle_start_kernel(CycleCtl c, int m, double beta) {
     c.head->pad = 0;
     c.head->scale = 1.0;
     c.head->res = beta;
+    if (c.hdone) *c.hdone = 0;
   }
 }
 
@@ -946,6 +947,7 @@ __global__ __launch_bounds__(64) void arnoldi_step_kernel(int k, int klast, int 
     hd.nonfinite = 1;
     hd.k_used = k;
     hd.done = 1;
+    if (c.hdone) *c.hdone = 1;
     return;
   }
   double* Hk = c.H + static_cast<int64_t>(k) * (m + 1);
@@ -974,6 +976,7 @@ __global__ __launch_bounds__(64) void arnoldi_step_kernel(int k, int klast, int 
     }
     hd.k_used = K;
     hd.done = 1;
+    if (c.hdone) *c.hdone = 1;
   } else {
     hd.scale = 1.0 / hn;
   }
