@@ -416,7 +416,8 @@ def main():
     if not args.no_large:
         line["large"] = large_suspension(args)
     if not args.no_cpu_baseline:
-        iters = sum(g["iters"]) / len(g["iters"]) + 1.0 if args.solver in ITERATIVE else 1.0
+        # the matvecs the GPU's solves did per step (Arnoldi steps and true residuals)
+        iters = g["acc"]["gemv_launches"] / K if args.solver in ITERATIVE else 1.0
         line["cpu_baseline"] = cpu_baseline(args, pos, ori, iters)
     print(json.dumps(line), flush=True)
 
