@@ -139,3 +139,37 @@ def test_fibers_file_errors(tmp_path):
         read_fibers(str(f))
     with pytest.raises(FileNotFoundError):
         read_fibers(str(tmp_path / "missing.in"))
+
+
+def test_confirm_residual_switch_reaches_the_config():
+    """--confirm-residual of both runner surfaces sets rf_config.confirm_residual,
+    and rf_create rejects a value other than 0 or 1 before it touches a device."""
+    from rigidfibers_amd import _lib
+    from rigidfibers_amd.runner import build_parser
+    from rigidfibers_amd.simulation import make_config
+
+    from helpers import ref_config
+
+    parser = build_parser()
+    for platform, solver in (("cuda", "--gmres"), ("fortran", "--gmres")):
+        args = parser.parse_args([platform, "run", "config.ini", "fibers.in", solver, "--analytical",
+                                  "--confirm-residual"] + (["--D1"] if platform == "cuda" else []))
+        assert args.confirm_residual is True
+        args = parser.parse_args([platform, "run", "config.ini", "fibers.in", solver, "--analytical"]
+                                 + (["--D1"] if platform == "cuda" else []))
+        assert args.confirm_residual is False
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = 4  # the fibers file sets it in a run
+    assert make_config(params).confirm_residual == 0
+    cfg = make_config(params, confirm_residual=True)
+    assert cfg.confirm_residual == 1
+    cfg.confirm_residual = 2
+    M = params.NUMBER_OF_FIBERS
+    pos = np.zeros(3 * M)
+    ori = np.tile([0.0, 0.0, 1.0], M)
+    sim = C.c_void_p()
+    lib = _lib.load()
+    assert lib.rf_create(C.byref(cfg), pos.ctypes.data_as(C.POINTER(C.c_double)),
+                         ori.ctypes.data_as(C.POINTER(C.c_double)), C.byref(sim)) == _lib.RF_ERR_INVALID_ARGUMENT
+    assert b"confirm_residual" in lib.rf_last_error()
+    assert not sim.value
