@@ -11,8 +11,9 @@
 // stays FP32); the basis is orthogonalised with classical Gram-Schmidt
 // applied twice, which spans the same space as the reference's
 // MGS/Householder variants, and the (m+1) x m Hessenberg least-squares
-// problem is solved on the device as well: the host enqueues a whole restart
-// cycle and waits once per cycle.  BiCGStab follows
+// problem is solved on the device as well: the host stays one Arnoldi step
+// ahead of the device and never waits for a step before enqueueing the next.
+// BiCGStab follows
 // lib/viennacl/linalg/bicgstab.hpp:92-160.
 #include <hip/hip_runtime.h>
 

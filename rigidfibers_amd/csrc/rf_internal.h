@@ -171,9 +171,6 @@ hipError_t gemv(const GemvPlan& p, const float* A, const double* x, double* part
 int mdot_blocks(int64_t n);
 hipError_t mdot(int64_t n, int K, const double* V, int64_t ldv, const double* w, double* part, double* out,
                 hipStream_t s);
-// w[r] += alpha * sum_k V_k[r] * h[k]
-hipError_t axpy_multi(int64_t n, int K, const double* V, int64_t ldv, const double* h, double alpha, double* w,
-                      hipStream_t s);
 // y = alpha * x + beta * y
 hipError_t axpby(int64_t n, double alpha, const double* x, double beta, double* y, hipStream_t s);
 hipError_t fill(int64_t n, double value, double* y, hipStream_t s);

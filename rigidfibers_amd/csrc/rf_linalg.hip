@@ -329,23 +329,6 @@ hipError_t mdot(int64_t n, int K, const double* V, int64_t ldv, const double* w,
   return hipGetLastError();
 }
 
-__global__ void axpy_multi_kernel(int64_t n, int K, const double* __restrict__ V, int64_t ldv,
-                                  const double* __restrict__ h, double alpha, double* __restrict__ w) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < K; ++k) s = fma(V[static_cast<int64_t>(k) * ldv + r], h[k], s);
-  w[r] = fma(alpha, s, w[r]);
-}
-
-hipError_t axpy_multi(int64_t n, int K, const double* V, int64_t ldv, const double* h, double alpha, double* w,
-                      hipStream_t s) {
-  if (K <= 0) return hipSuccess;
-  hipLaunchKernelGGL(axpy_multi_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, n, K, V, ldv,
-                     h, alpha, w);
-  return hipGetLastError();
-}
-
 __global__ void axpby_kernel(int64_t n, double alpha, const double* __restrict__ x, double beta,
                              double* __restrict__ y) {
   const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
