@@ -357,7 +357,7 @@ def main():
     ap.add_argument("--separate-residual", action="store_true",
                     help="form the first residual of a warm start with a matvec, not in the assembly kernel")
     ap.add_argument("--confirm-residual", action="store_true",
-                    help="GMRES: confirm a converged restart cycle with the true residual (one matvec more)")
+                    help="GMRES and BiCGStab: confirm a converged solve with the true residual (one matvec more)")
     ap.add_argument("--no-precision-study", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")

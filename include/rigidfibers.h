@@ -134,16 +134,19 @@ typedef struct rf_config {
                                                          estimate, as lib/viennacl/linalg/gmres.hpp:330-335
                                                          does; 1 confirms it with the true residual b - A x
                                                          (one matvec more per solve), as fortran/sgmres.f does
-                                                         with icntl(8) = 1 */
+                                                         with icntl(8) = 1.  BiCGStab: 0 reports a converged
+                                                         solve's recurrence residual, as
+                                                         lib/viennacl/linalg/bicgstab.hpp:146-160 does; 1 the
+                                                         true residual of the returned x */
   int32_t reserved[1];
 } rf_config;
 
 typedef struct rf_solver_stats {
   int32_t iterations;          /* Krylov iterations (matvecs) or refinement sweeps of the last solve */
   int32_t converged;           /* 1 if the tolerance was met */
-  double relative_residual;    /* ||b - A x|| / ||b|| of the returned x (FP64); GMRES without
-                                  confirm_residual: the restart cycle's estimate of it when that
-                                  cycle converged, as BiCGStab's is its recurrence's */
+  double relative_residual;    /* ||b - A x|| / ||b|| of the returned x (FP64); a converged GMRES or
+                                  BiCGStab solve without confirm_residual: the restart cycle's
+                                  estimate of it, or BiCGStab's recurrence residual */
 } rf_solver_stats;
 
 typedef struct rf_timings {

@@ -547,11 +547,16 @@ rf_status solve_bicgstab(rf_simulation* s) {
     RF_HIP(rf::axpby(n, -omega, v, 1.0, p, s->stream));
     RF_HIP(rf::axpby(n, 1.0, r, beta, p, s->stream));
   }
-  // report the true residual of the returned x
-  RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
-  RF_TRY(device_norm(s, r, &res));
+  // A converged solve reports its recurrence residual, as the reference's
+  // solver does (lib/viennacl/linalg/bicgstab.hpp:146-148,160); the true
+  // residual of the returned x when it did not converge, or on request.
+  if (!converged || s->cfg.confirm_residual) {
+    RF_HIP(sim_gemv(s, s->x, s->gpart, r, s->b, 1));
+    RF_TRY(device_norm(s, r, &res));
+    converged = converged && res / bnorm <= 10.0 * tol;
+  }
   s->stats.iterations = it;
-  s->stats.converged = (converged && res / bnorm <= 10.0 * tol) ? 1 : 0;
+  s->stats.converged = converged ? 1 : 0;
   s->stats.relative_residual = res / bnorm;
   return RF_OK;
 }

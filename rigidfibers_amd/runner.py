@@ -74,7 +74,8 @@ def _add_common(p, platform, command):
     p.add_argument("--velocity-integration", choices=["follow", "numerical", "analytical"], default="follow")
     p.add_argument("--initial-guess", choices=["previous", "zero", "extrapolated"], default="previous")
     p.add_argument("--confirm-residual", action="store_true",
-                   help="GMRES: confirm a converged restart cycle with the true residual, as the Fortran solver does")
+                   help="GMRES and BiCGStab: confirm a converged solve with the true residual, as the Fortran "
+                        "solver does")
     p.add_argument("--output-dir", default=".")
     if command == "validate":
         p.add_argument("--reference-dir", default=None,

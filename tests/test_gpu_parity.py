@@ -363,6 +363,39 @@ def test_gmres_stops_on_the_cycle_estimate(matrix_format):
         assert abs(est - true) <= 1e-3 * 1e-8
 
 
+def test_bicgstab_reports_its_recurrence_residual():
+    """A converged BiCGStab solve reports its recurrence residual, as the
+    reference's solver does (lib/viennacl/linalg/bicgstab.hpp:146-160);
+    confirm_residual adds the true residual of the returned x (one matvec
+    more per solve) and changes nothing else."""
+    M = 120
+    pos, ori = random_scene(M, side=6.0, seed=29, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.BICGSTAB_TOLERANCE = 1e-8
+    steps = 3
+    out = {}
+    for confirm in (False, True):
+        with _sim(params, pos, ori, solver="bicgstab", confirm_residual=confirm) as sim:
+            res, matvecs = [], []
+            for t in range(steps):
+                sim.step(t)
+                st = sim.solver_stats()
+                assert st["converged"] and st["iterations"] > 0
+                res.append((st["iterations"], st["relative_residual"]))
+                matvecs.append(sim.timings()["gemv_launches"])
+            out[confirm] = (sim.state(), res, matvecs)
+    (s0, r0, g0), (s1, r1, g1) = out[False], out[True]
+    print("bicgstab: matvecs", g0, "/", g1, "residuals (recurrence / true)",
+          ", ".join(f"{a[1]:.6e} / {b[1]:.6e}" for a, b in zip(r0, r1)))
+    assert np.array_equal(s0[0], s1[0]) and np.array_equal(s0[1], s1[1])
+    assert [a[0] for a in r0] == [b[0] for b in r1]
+    assert [b - a for a, b in zip(g0, g1)] == [1] * steps
+    for (_, rec), (_, true) in zip(r0, r1):
+        assert rec <= 1e-8 and true <= 1e-8 * 10
+        assert abs(rec - true) <= 1e-2 * 1e-8
+
+
 @pytest.mark.parametrize("matrix_format", ["theta", "dense"])
 def test_gmres_restart_cycles_and_iteration_cap(matrix_format):
     """Restart cycles of 3 steps, each enqueued at once with its control on
