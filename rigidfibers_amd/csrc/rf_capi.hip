@@ -281,6 +281,15 @@ rf_status device_dot(rf_simulation* s, const double* u, const double* v, double*
   return RF_OK;
 }
 
+// <u, w> and <u + n, w> (two vectors at stride n) in one launch and one wait;
+// each is the value device_dot gives.
+rf_status device_dots2(rf_simulation* s, const double* u, const double* w, double out[2]) {
+  RF_HIP(rf::mdot(s->n, 2, u, s->n, w, s->dpart, s->hdev, s->stream));
+  RF_HIP(hipMemcpyAsync(out, s->hdev, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return RF_OK;
+}
+
 // Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
 // GMRES(m) for A x = b: A the FP32 system, or its FP16 copy (half).  Starts
 // from x when warm, else from 0; stops at ||b - A x|| <= tol ||b|| or after
@@ -525,22 +534,23 @@ rf_status solve_bicgstab(rf_simulation* s) {
       break;
     }
     RF_HIP(sim_gemv(s, sv, s->gpart, t, nullptr, 0));
-    double ts, tt;
-    RF_TRY(device_dot(s, t, sv, &ts));
-    RF_TRY(device_dot(s, t, t, &tt));
+    double st2[2];  // <s, t>, <t, t> (t follows s in the workspace)
+    RF_TRY(device_dots2(s, sv, t, st2));
+    const double ts = st2[0], tt = st2[1];
     const double omega = (tt == 0.0) ? 0.0 : ts / tt;
     RF_HIP(rf::axpby(n, alpha, p, 1.0, s->x, s->stream));
     RF_HIP(rf::axpby(n, omega, sv, 1.0, s->x, s->stream));
     RF_HIP(rf::axpby(n, 1.0, sv, 0.0, r, s->stream));
     RF_HIP(rf::axpby(n, -omega, t, 1.0, r, s->stream));
-    RF_TRY(device_norm(s, r, &res));
+    double rr2[2];  // <r, r>, <r0, r> (r0 follows r in the workspace)
+    RF_TRY(device_dots2(s, r, r, rr2));
+    res = std::sqrt(rr2[0]);
     if (res / bnorm <= tol) {
       converged = true;
       break;
     }
     if (omega == 0.0) break;
-    double rho_new;
-    RF_TRY(device_dot(s, r0, r, &rho_new));
+    const double rho_new = rr2[1];
     const double beta = (rho_new / rho) * (alpha / omega);
     rho = rho_new;
     // p = r + beta (p - omega v)
