@@ -849,11 +849,12 @@ hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const flo
 // ---------------------------------------------------------------------------
 // Single-simulation GMRES(m), a restart cycle with its control on the
 // device (the reference's solver reads and writes the Hessenberg entries on
-// the host at every step, lib/viennacl/linalg/gmres.hpp:246,281-285).  A step is 6 launches after its product: the dots
-// and the update of each Gram-Schmidt pass (the update forms the
-// coefficients from the dots' block partials itself), ||w||^2, the control
-// step and the normalisation.  Each coefficient is the sum of its block
-// partials in block order, the same in the update and in the control.
+// the host at every step, lib/viennacl/linalg/gmres.hpp:246,281-285).  A
+// step is 7 launches after its product: the dots and the update of each
+// Gram-Schmidt pass (the update forms the coefficients from the dots' block
+// partials itself), ||w||^2, the control step and the normalisation.  Each
+// coefficient is the sum of its block partials in block order, the same in
+// the update and in the control.
 
 size_t cycle_ctl_doubles(int m) {
   return sizeof(CycleHead) / sizeof(double) + static_cast<size_t>(m + 1) * m + 4 * static_cast<size_t>(m) + 1;
