@@ -615,3 +615,32 @@ def test_theta_form_matches_dense(analytical):
     print("A", rel_err(At, Ad), "b", rel_err(bt, bd), "x", rel_err(xt, xd), "pos", rel_err(pt, pd))
     assert rel_err(At, Ad) < 1e-6 and rel_err(bt, bd) < 1e-6
     assert rel_err(xt, xd) < 1e-6 and rel_err(pt, pd) < 1e-8 and rel_err(ot, od) < 1e-7
+
+
+@pytest.mark.parametrize("solver,integration", [("gmres", "analytical"), ("gmres", "numerical"),
+                                                ("bicgstab", "numerical"), ("direct", "numerical"),
+                                                ("mixed-bf16", "analytical"), ("gmres-ir", "numerical")])
+def test_coincident_fibers_fail_cleanly(solver, integration):
+    """Two coincident fibers make the kernel integrals infinite: the step
+    fails with RF_ERR_NONFINITE (a singular LU may report RF_ERR_SINGULAR)
+    instead of returning NaN, and the same library then steps a valid
+    scene."""
+    from rigidfibers_amd import _lib
+
+    M = 20
+    pos, ori = random_scene(M, side=4.0, seed=41, min_dist=0.4)
+    pos, ori = pos.copy(), ori.copy()
+    pos[3:6], ori[3:6] = pos[0:3], ori[0:3]  # fiber 1 on top of fiber 0
+    params = ref_config()
+    params.NUMBER_OF_FIBERS = M
+    allowed = {_lib.RF_ERR_NONFINITE} | ({_lib.RF_ERR_SINGULAR} if solver in ("direct", "mixed-bf16") else set())
+    with _sim(params, pos, ori, solver=solver, integration=integration) as sim:
+        with pytest.raises(_lib.RfError) as err:
+            sim.step(0)
+    print(solver, integration, err.value)
+    assert err.value.code in allowed
+    pos2, ori2 = random_scene(M, side=4.0, seed=41, min_dist=0.4)
+    with _sim(params, pos2, ori2, solver=solver, integration=integration) as sim:
+        sim.step(0)
+        p, o = sim.state()
+    assert np.isfinite(p).all() and np.isfinite(o).all()
