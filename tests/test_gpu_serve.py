@@ -5,7 +5,8 @@ suspensions on one GPU.
   * a served suspension follows the single-simulation path (Simulation) to
     the GMRES tolerance;
   * admission respects the memory budget and the slot count, and a request
-    whose footprint exceeds the budget is refused.
+    whose footprint exceeds the budget is refused;
+  * a request that turns non-finite fails alone, its neighbours unchanged.
 """
 import numpy as np
 import pytest
@@ -106,6 +107,29 @@ def test_served_confirm_residual_same_result():
         assert a["info"]["status"] == 0 and b["info"]["status"] == 0
         assert a["info"]["gmres_iterations"] == b["info"]["gmres_iterations"] > 3
         assert np.array_equal(a["positions"], b["positions"]) and np.array_equal(a["orientations"], b["orientations"])
+
+
+def test_bad_request_fails_alone():
+    """A request whose first two fibers coincide fails with RF_ERR_NONFINITE;
+    the requests served beside it give bitwise the results they give alone."""
+    from rigidfibers_amd import _lib
+    from rigidfibers_amd.serve import Server
+
+    reqs = _requests([23, 40, 31], steps=3, seed=13)
+    M, pos, ori, steps = reqs[1]
+    pos, ori = pos.copy(), ori.copy()
+    pos[3:6], ori[3:6] = pos[0:3], ori[0:3]
+    bad = (M, pos, ori, steps)
+    with Server(_params(), max_active=8) as srv:
+        alone = [_serve(srv, [r])[0] for r in (reqs[0], reqs[2])]
+        mixed = _serve(srv, [reqs[0], bad, reqs[2]])
+        st = srv.stats()
+    assert mixed[1]["info"]["status"] == _lib.RF_ERR_NONFINITE
+    for a, b in zip(alone, (mixed[0], mixed[2])):
+        assert a["info"]["status"] == 0 and b["info"]["status"] == 0
+        assert a["info"]["gmres_iterations"] == b["info"]["gmres_iterations"]
+        assert np.array_equal(a["positions"], b["positions"]) and np.array_equal(a["orientations"], b["orientations"])
+    assert st["active"] == 0 and st["queued"] == 0
 
 
 def test_memory_budget_and_slots():
