@@ -644,3 +644,29 @@ def test_coincident_fibers_fail_cleanly(solver, integration):
         sim.step(0)
         p, o = sim.state()
     assert np.isfinite(p).all() and np.isfinite(o).all()
+
+
+@pytest.mark.parametrize("restart", [1, 1024])
+def test_gmres_restart_extremes(restart):
+    """GMRES(1), a cycle of one step, and GMRES(1024), the largest restart,
+    longer than the n = 300 unknowns of the scene: both reach a tight true
+    residual and the FP64-refined LU solution of the same dense system."""
+    M = 20
+    pos, ori = random_scene(M, side=4.0, seed=43, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_RESTART = restart
+    params.GMRES_TOLERANCE = 1e-10
+    with _sim(params, pos, ori, solver="mixed-bf16", refinement_tolerance=1e-12) as sim:
+        sim.step(0)
+        _, _, x_ref = sim.system()
+    with _sim(params, pos, ori, solver="gmres", matrix_format="dense", confirm_residual=True) as sim:
+        sim.step(0)
+        _, _, x = sim.system()
+        st, mv = sim.solver_stats(), sim.timings()["gemv_launches"]
+    its = st["iterations"]
+    print(f"restart {restart}: {its} steps, {mv} products, residual {st['relative_residual']:.2e}, "
+          f"x vs LU {rel_err(x, x_ref):.1e}")
+    assert st["converged"] and st["relative_residual"] <= 1e-10
+    assert mv == its + (its - 1) // restart + 1  # steps, restarts, confirmation
+    assert rel_err(x, x_ref) < 1e-8
