@@ -112,7 +112,8 @@ struct rf_simulation {
   double* hdev = nullptr;
   int* flag = nullptr;
   rf::CycleCtl cyc{};  // GMRES: the device control of a restart cycle
-  int* hdone = nullptr;  // its done flag, mirrored into pinned host memory
+  rf::BicgCtl bicg{};  // BiCGStab: its scalars on the device
+  int* hdone = nullptr;  // the done flag of either, mirrored into pinned host memory
   hipEvent_t cev[2] = {nullptr, nullptr};  // the ends of the last two steps
   rf::GemvPlan plan{};
   rf::LuWorkspace lu{};
@@ -274,21 +275,6 @@ rf_status device_norm(rf_simulation* s, const double* v, double* out) {
   return RF_OK;
 }
 
-rf_status device_dot(rf_simulation* s, const double* u, const double* v, double* out) {
-  RF_HIP(rf::mdot(s->n, 1, u, s->n, v, s->dpart, s->hdev, s->stream));
-  RF_HIP(hipMemcpyAsync(out, s->hdev, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  RF_HIP(hipStreamSynchronize(s->stream));
-  return RF_OK;
-}
-
-// <u, w> and <u + n, w> (two vectors at stride n) in one launch and one wait;
-// each is the value device_dot gives.
-rf_status device_dots2(rf_simulation* s, const double* u, const double* w, double out[2]) {
-  RF_HIP(rf::mdot(s->n, 2, u, s->n, w, s->dpart, s->hdev, s->stream));
-  RF_HIP(hipMemcpyAsync(out, s->hdev, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  RF_HIP(hipStreamSynchronize(s->stream));
-  return RF_OK;
-}
 
 // Restarted GMRES(m), classical Gram-Schmidt twice, FP64 basis.
 // GMRES(m) for A x = b: A the FP32 system, or its FP16 copy (half).  Starts
@@ -515,47 +501,33 @@ rf_status solve_bicgstab(rf_simulation* s) {
   RF_HIP(rf::axpby(n, 1.0, r, 0.0, p, s->stream));
   int it = 0;
   bool converged = res / bnorm <= tol;
-  while (it < maxit && !converged) {
-    ++it;
-    RF_HIP(sim_gemv(s, p, s->gpart, v, nullptr, 0));
-    double r0v;
-    RF_TRY(device_dot(s, r0, v, &r0v));
-    if (r0v == 0.0 || !std::isfinite(r0v)) break;
-    const double alpha = rho / r0v;
-    RF_HIP(rf::axpby(n, 1.0, r, 0.0, sv, s->stream));
-    RF_HIP(rf::axpby(n, -alpha, v, 1.0, sv, s->stream));
-    double snorm;
-    RF_TRY(device_norm(s, sv, &snorm));
-    if (!std::isfinite(snorm)) return fail(RF_ERR_NONFINITE, "non-finite residual in BiCGStab");
-    if (snorm / bnorm <= tol) {
-      RF_HIP(rf::axpby(n, alpha, p, 1.0, s->x, s->stream));
-      res = snorm;
-      converged = true;
-      break;
+  if (!converged && maxit > 0) {
+    // The scalars live on the device; the host stays one iteration ahead (it
+    // enqueues iteration k + 1, then waits for the end of iteration k and reads
+    // the done flag mirrored into host memory), and the kernels of an
+    // iteration enqueued after the end return at once.
+    const rf::BicgVecs w{s->x, r, p, v, sv};
+    const int launched0 = s->ngemv;
+    RF_HIP(rf::bicg_start(s->bicg, rho, res, s->stream));
+    for (int k = 0; k < maxit; ++k) {
+      RF_HIP(sim_gemv(s, p, s->gpart, v, nullptr, 0, false, &s->bicg.head->done));
+      RF_HIP(rf::bicg_after_v(n, k + 1, w, s->dpart, tol, bnorm, s->bicg, s->stream));
+      RF_HIP(sim_gemv(s, sv, s->gpart, t, nullptr, 0, false, &s->bicg.head->done));
+      RF_HIP(rf::bicg_after_t(n, w, s->dpart, tol, bnorm, s->bicg, s->stream));
+      RF_HIP(hipEventRecord(s->cev[k & 1], s->stream));
+      if (k > 0) {
+        RF_HIP(hipEventSynchronize(s->cev[(k - 1) & 1]));
+        if (*static_cast<volatile int*>(s->hdone)) break;
+      }
     }
-    RF_HIP(sim_gemv(s, sv, s->gpart, t, nullptr, 0));
-    double st2[2];  // <s, t>, <t, t> (t follows s in the workspace)
-    RF_TRY(device_dots2(s, sv, t, st2));
-    const double ts = st2[0], tt = st2[1];
-    const double omega = (tt == 0.0) ? 0.0 : ts / tt;
-    RF_HIP(rf::axpby(n, alpha, p, 1.0, s->x, s->stream));
-    RF_HIP(rf::axpby(n, omega, sv, 1.0, s->x, s->stream));
-    RF_HIP(rf::axpby(n, 1.0, sv, 0.0, r, s->stream));
-    RF_HIP(rf::axpby(n, -omega, t, 1.0, r, s->stream));
-    double rr2[2];  // <r, r>, <r0, r> (r0 follows r in the workspace)
-    RF_TRY(device_dots2(s, r, r, rr2));
-    res = std::sqrt(rr2[0]);
-    if (res / bnorm <= tol) {
-      converged = true;
-      break;
-    }
-    if (omega == 0.0) break;
-    const double rho_new = rr2[1];
-    const double beta = (rho_new / rho) * (alpha / omega);
-    rho = rho_new;
-    // p = r + beta (p - omega v)
-    RF_HIP(rf::axpby(n, -omega, v, 1.0, p, s->stream));
-    RF_HIP(rf::axpby(n, 1.0, r, beta, p, s->stream));
+    rf::BicgHead hd{};
+    RF_HIP(hipMemcpyAsync(&hd, s->bicg.head, sizeof(hd), hipMemcpyDeviceToHost, s->stream));
+    RF_HIP(hipStreamSynchronize(s->stream));
+    s->ngemv = std::min(s->ngemv, launched0 + hd.prods);  // products that returned at once are not counted
+    if (hd.nonfinite) return fail(RF_ERR_NONFINITE, "non-finite residual in BiCGStab");
+    it = hd.its;
+    converged = hd.converged != 0;
+    res = hd.res;
   }
   // A converged solve reports its recurrence residual, as the reference's
   // solver does (lib/viennacl/linalg/bicgstab.hpp:146-148,160); the true
@@ -719,23 +691,32 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
   if (st == RF_OK) st = s->alloc(&s->work, 6 * static_cast<size_t>(n));
   const int kmax = (s->kdim > 2 ? s->kdim : 2) + 1;
   size_t dpart_len = static_cast<size_t>(rf::mdot_blocks(n)) * kmax;
-  if (s->kdim > 0) {
-    const int m = cfg->gmres_restart;
-    dpart_len = std::max(dpart_len, rf::cycle_part_doubles(n, m));
-    double* mem = nullptr;
-    if (st == RF_OK) st = s->alloc(&mem, rf::cycle_ctl_doubles(m));
-    if (st == RF_OK) s->cyc = rf::cycle_ctl(mem, m);
+  const bool bicg = cfg->solver == RF_SOLVER_BICGSTAB;
+  if (s->kdim > 0 || bicg) {
+    if (s->kdim > 0) {
+      const int m = cfg->gmres_restart;
+      dpart_len = std::max(dpart_len, rf::cycle_part_doubles(n, m));
+      double* mem = nullptr;
+      if (st == RF_OK) st = s->alloc(&mem, rf::cycle_ctl_doubles(m));
+      if (st == RF_OK) s->cyc = rf::cycle_ctl(mem, m);
+    } else {
+      dpart_len = std::max(dpart_len, rf::bicg_part_doubles(n));
+      double* mem = nullptr;
+      if (st == RF_OK) st = s->alloc(&mem, (sizeof(rf::BicgHead) + sizeof(double) - 1) / sizeof(double));
+      if (st == RF_OK) s->bicg.head = reinterpret_cast<rf::BicgHead*>(mem);
+    }
     void* hp = nullptr;
     void* dp = nullptr;
     if (st == RF_OK && (hipHostMalloc(&hp, 64, hipHostMallocMapped) != hipSuccess ||
                         hipHostGetDevicePointer(&dp, hp, 0) != hipSuccess)) {
       if (hp) (void)hipHostFree(hp);
       (void)hipGetLastError();
-      st = fail(RF_ERR_OUT_OF_MEMORY, "hipHostMalloc of the GMRES cycle flag failed");
+      st = fail(RF_ERR_OUT_OF_MEMORY, "hipHostMalloc of the solver's done flag failed");
     } else if (st == RF_OK) {
       s->hdone = static_cast<int*>(hp);
       *s->hdone = 0;
       s->cyc.hdone = static_cast<int*>(dp);
+      s->bicg.hdone = static_cast<int*>(dp);
     }
   }
   if (st == RF_OK) st = s->alloc(&s->dpart, dpart_len);

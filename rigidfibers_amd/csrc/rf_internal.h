@@ -213,4 +213,39 @@ hipError_t arnoldi_step(int64_t n, int k, int klast, int m, const double* V, dou
 // x += sum_{k < k_used} V_k y_k
 hipError_t cycle_update(int64_t n, const double* V, const CycleCtl& c, double* x, hipStream_t s);
 
+// BiCGStab with its scalars on the device (rf_linalg.hip), the same
+// arithmetic as lib/viennacl/linalg/bicgstab.hpp:92-160 plus the test of
+// ||s|| before the second product.  Iterations the host enqueues after the
+// solve's end find head.done set and return at once.
+struct BicgHead {
+  int done;       // the solve has ended
+  int converged;
+  int nonfinite;  // ||s|| was not finite
+  int its;        // iterations started
+  int prods;      // products that ran
+  int early_it;   // the iteration whose ||s|| met the tolerance (0: none)
+  int pad[2];
+  double rho, alpha, omega, beta, res;
+};
+struct BicgCtl {
+  BicgHead* head;
+  int* hdone;  // pinned host memory (device address): head.done mirrored
+};
+struct BicgVecs {
+  double* x;
+  double* r;   // r0 follows r at stride n
+  double* p;
+  double* v;
+  double* s;   // t follows s at stride n
+};
+size_t bicg_part_doubles(int64_t n);
+hipError_t bicg_start(const BicgCtl& c, double rho, double res, hipStream_t s);
+// After v = A p of iteration it (1-based): alpha, s = r - alpha v, ||s||
+// and the early end with x += alpha p.
+hipError_t bicg_after_v(int64_t n, int it, const BicgVecs& w, double* part, double tol, double bnorm,
+                        const BicgCtl& c, hipStream_t s);
+// After t = A s: omega, x and r, ||r|| and <r0, r>, the end test, p.
+hipError_t bicg_after_t(int64_t n, const BicgVecs& w, double* part, double tol, double bnorm, const BicgCtl& c,
+                        hipStream_t s);
+
 }  // namespace rf

@@ -1014,4 +1014,186 @@ hipError_t cycle_update(int64_t n, const double* V, const CycleCtl& c, double* x
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// BiCGStab with its scalars on the device.  Each scalar kernel is one wave
+// that sums the dots' block partials exactly as mdot_final_kernel does, and
+// each vector kernel repeats the axpby sequence of the host form, so the
+// iterates are the ones host-side scalars give.
+
+size_t bicg_part_doubles(int64_t n) { return 6 * static_cast<size_t>(mdot_blocks(n)); }
+
+__device__ __forceinline__ double wave_sum_parts(const double* __restrict__ part, int nb) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) s += part[b];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  return s;
+}
+
+__device__ __forceinline__ void bicg_end(const BicgCtl& c) {
+  c.head->done = 1;
+  if (c.hdone) *c.hdone = 1;
+}
+
+__global__ void bicg_start_kernel(BicgCtl c, double rho, double res) {
+  if (threadIdx.x != 0) return;
+  BicgHead& h = *c.head;
+  h.done = 0;
+  h.converged = 0;
+  h.nonfinite = 0;
+  h.its = 0;
+  h.prods = 0;
+  h.early_it = 0;
+  h.rho = rho;
+  h.alpha = h.omega = h.beta = 0.0;
+  h.res = res;
+  if (c.hdone) *c.hdone = 0;
+}
+
+// alpha = rho / <r0, v>; a zero or non-finite <r0, v> ends the solve
+__global__ __launch_bounds__(64) void bicg_alpha_kernel(int nb, const double* __restrict__ part, BicgCtl c) {
+  BicgHead& h = *c.head;
+  if (h.done) return;
+  const double r0v = wave_sum_parts(part, nb);
+  if (threadIdx.x != 0) return;
+  ++h.its;
+  ++h.prods;
+  if (r0v == 0.0 || !isfinite(r0v)) {
+    bicg_end(c);
+    return;
+  }
+  h.alpha = h.rho / r0v;
+}
+
+// s = r - alpha v
+__global__ void bicg_s_kernel(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
+                              double* __restrict__ sv, const BicgHead* __restrict__ h) {
+  if (h->done) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) sv[i] = fma(-h->alpha, v[i], 1.0 * r[i]);
+}
+
+__global__ __launch_bounds__(64) void bicg_snorm_kernel(int it, int nb, const double* __restrict__ part, double tol,
+                                                        double bnorm, BicgCtl c) {
+  BicgHead& h = *c.head;
+  if (h.done) return;
+  const double snorm = sqrt(wave_sum_parts(part, nb));
+  if (threadIdx.x != 0) return;
+  if (!isfinite(snorm)) {
+    h.nonfinite = 1;
+    bicg_end(c);
+  } else if (snorm / bnorm <= tol) {
+    h.res = snorm;
+    h.converged = 1;
+    h.early_it = it;
+    bicg_end(c);
+  }
+}
+
+// x += alpha p when iteration it ended on ||s||
+__global__ void bicg_early_kernel(int64_t n, int it, const double* __restrict__ p, double* __restrict__ x,
+                                  const BicgHead* __restrict__ h) {
+  if (h->early_it != it) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = fma(h->alpha, p[i], 1.0 * x[i]);
+}
+
+// omega = <t, s> / <t, t> (0 when <t, t> = 0)
+__global__ __launch_bounds__(64) void bicg_omega_kernel(int nb, const double* __restrict__ part, BicgCtl c) {
+  BicgHead& h = *c.head;
+  if (h.done) return;
+  const double ts = wave_sum_parts(part, nb), tt = wave_sum_parts(part + nb, nb);
+  if (threadIdx.x != 0) return;
+  ++h.prods;
+  h.omega = (tt == 0.0) ? 0.0 : ts / tt;
+}
+
+// x += alpha p + omega s, r = s - omega t
+__global__ void bicg_xr_kernel(int64_t n, const double* __restrict__ p, const double* __restrict__ sv,
+                               const double* __restrict__ t, double* __restrict__ x, double* __restrict__ r,
+                               const BicgHead* __restrict__ h) {
+  if (h->done) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = fma(h->alpha, p[i], 1.0 * x[i]);
+  x[i] = fma(h->omega, sv[i], 1.0 * xi);
+  r[i] = fma(-h->omega, t[i], 1.0 * (1.0 * sv[i]));
+}
+
+// ||r||, the end tests, beta and rho from ||r||^2 and <r0, r>
+__global__ __launch_bounds__(64) void bicg_end_kernel(int nb, const double* __restrict__ part, double tol,
+                                                      double bnorm, BicgCtl c) {
+  BicgHead& h = *c.head;
+  if (h.done) return;
+  const double rr = wave_sum_parts(part, nb), rho_new = wave_sum_parts(part + nb, nb);
+  if (threadIdx.x != 0) return;
+  const double res = sqrt(rr);
+  h.res = res;
+  if (res / bnorm <= tol) {
+    h.converged = 1;
+    bicg_end(c);
+    return;
+  }
+  if (h.omega == 0.0) {
+    bicg_end(c);
+    return;
+  }
+  h.beta = (rho_new / h.rho) * (h.alpha / h.omega);
+  h.rho = rho_new;
+}
+
+// p = r + beta (p - omega v)
+__global__ void bicg_p_kernel(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
+                              double* __restrict__ p, const BicgHead* __restrict__ h) {
+  if (h->done) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q = fma(-h->omega, v[i], 1.0 * p[i]);
+  p[i] = fma(1.0, r[i], h->beta * q);
+}
+
+hipError_t bicg_start(const BicgCtl& c, double rho, double res, hipStream_t s) {
+  hipLaunchKernelGGL(bicg_start_kernel, dim3(1), dim3(64), 0, s, c, rho, res);
+  return hipGetLastError();
+}
+
+hipError_t bicg_after_v(int64_t n, int it, const BicgVecs& w, double* part, double tol, double bnorm,
+                        const BicgCtl& c, hipStream_t s) {
+  const int nb = mdot_blocks(n);
+  const int64_t rpb = (n + nb - 1) / nb;
+  const unsigned vb = static_cast<unsigned>((n + 255) / 256);
+  const int* skip = &c.head->done;  // address arithmetic only
+  const double* r0 = w.r + n;
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 1), dim3(kDotThreads), 0, s, n, rpb, r0, n, w.v, part, skip);
+  hipLaunchKernelGGL(bicg_alpha_kernel, dim3(1), dim3(64), 0, s, nb, static_cast<const double*>(part), c);
+  hipLaunchKernelGGL(bicg_s_kernel, dim3(vb), dim3(256), 0, s, n, w.r, w.v, w.s, c.head);
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 1), dim3(kDotThreads), 0, s, n, rpb, w.s, n, w.s, part + nb,
+                     skip);
+  hipLaunchKernelGGL(bicg_snorm_kernel, dim3(1), dim3(64), 0, s, it, nb, static_cast<const double*>(part + nb), tol,
+                     bnorm, c);
+  hipLaunchKernelGGL(bicg_early_kernel, dim3(vb), dim3(256), 0, s, n, it, w.p, w.x, c.head);
+  return hipGetLastError();
+}
+
+hipError_t bicg_after_t(int64_t n, const BicgVecs& w, double* part, double tol, double bnorm, const BicgCtl& c,
+                        hipStream_t s) {
+  const int nb = mdot_blocks(n);
+  const int64_t rpb = (n + nb - 1) / nb;
+  const unsigned vb = static_cast<unsigned>((n + 255) / 256);
+  const int* skip = &c.head->done;
+  const double* t = w.s + n;
+  // <s, t>, <t, t>
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 2), dim3(kDotThreads), 0, s, n, rpb, w.s, n, t, part + 2 * nb,
+                     skip);
+  hipLaunchKernelGGL(bicg_omega_kernel, dim3(1), dim3(64), 0, s, nb, static_cast<const double*>(part + 2 * nb), c);
+  hipLaunchKernelGGL(bicg_xr_kernel, dim3(vb), dim3(256), 0, s, n, w.p, w.s, t, w.x, w.r, c.head);
+  // ||r||^2, <r0, r>
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 2), dim3(kDotThreads), 0, s, n, rpb, w.r, n, w.r, part + 4 * nb,
+                     skip);
+  hipLaunchKernelGGL(bicg_end_kernel, dim3(1), dim3(64), 0, s, nb, static_cast<const double*>(part + 4 * nb), tol,
+                     bnorm, c);
+  hipLaunchKernelGGL(bicg_p_kernel, dim3(vb), dim3(256), 0, s, n, w.r, w.v, w.p, c.head);
+  return hipGetLastError();
+}
+
 }  // namespace rf
