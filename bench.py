@@ -14,17 +14,29 @@ run untimed, then exactly K steps are timed between two device
 synchronisations.  value = M * K / elapsed.
 
 Also reported:
-  roofline      iterative solvers: the solver's matvec (HBM-bound), algorithmic
-                bytes per launch (theta form: M * M * N * (3 + 6(N-1)) * 4, the
-                FP32 numbers that define A; dense: n^2 * 4) / mean launch
-                duration (HIP events on the library's stream), against 8 TB/s.
-                Direct solvers: the LU's trailing-update GEMMs (MFMA-bound),
-                their 2 m n K flops / summed duration, against the dense MFMA
-                peak of the mode (FP32 157.3, BF16/FP16 2500 TFLOP/s);
+  roofline      the step's dominant kernel, the assembly (VALU-bound): its
+                algorithmic flops per launch, M (M - 1) ordered fiber pairs x
+                32 500 flop (analytical: 14 976 of them FP64 -- 24 nodes x 624
+                for the moment recurrences -- and 17 524 FP32) or x 63 216
+                FP32 flop (numerical), / the mean duration of the assembly
+                kernel (HIP events around it on the library's stream), against
+                the peak for that FP64/FP32 mix at MI355X's published 78.6 /
+                157.3 TFLOP/s (the time the flops take at those rates);
+  roofline_solver  iterative solvers: the solver's matvec (HBM-bound),
+                algorithmic bytes per launch (theta form: M * M * N * (3 +
+                6(N-1)) * 4, the FP32 numbers that define A; dense: n^2 * 4) /
+                mean launch duration, against 8 TB/s.  Direct solvers: the
+                LU's trailing-update GEMMs (MFMA-bound), their 2 m n K flops /
+                summed duration, against the dense MFMA peak of the mode (FP32
+                157.3, BF16/FP16 2500 TFLOP/s);
   precision     plain FP32 LU against the BF16- and FP16-MFMA LUs with FP64
                 refinement (to 1e-10) on the same scene: fiber-steps/s, the
-                speed-up, and the relative position error against the FP32-LU
-                run after the same steps;
+                speed-up, the relative position error against the FP32-LU run
+                after the same steps, and the error against the FP64 oracle:
+                the residual of the last step's solution in the oracle's FP64
+                rows of three groups of target fibers, and the position and
+                orientation error of 10 steps on a 250-fiber ball at the same
+                number density against the oracle's trajectory;
   serving       the serving stream with continuous batching against the same
                 stream one request at a time: p50/p99 step and request latency;
   large         M = 10 000: GMRES-IR on the FP16 theta copy against plain
@@ -33,6 +45,8 @@ Also reported:
                 librf_oracle32.so) on a bounded sample: the rows of S target
                 fibers assembled against all 5000 sources, their velocities,
                 and the GMRES matvecs of those rows; scaled by M/S to a step.
+                Threads: OMP_NUM_THREADS, else every CPU the process may run
+                on, bound close (OMP_PROC_BIND); the CPU model is recorded.
 """
 from __future__ import annotations
 
@@ -58,7 +72,9 @@ def build_scene(M, radius, seed):
     return ball(M, radius, min_dist=0.4, seed=seed)
 
 
-def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
+def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
+    """Runs warmup + steps timesteps; with capture_last also returns the state
+    the last step assembled its system from and that step's solution."""
     from rigidfibers_amd.config import Parameters
     from rigidfibers_amd.simulation import Simulation
 
@@ -77,11 +93,15 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
             print(f"# {tag}{solver} warmup step {t}: {sim.timings()['total']:.1f} ms, solver {sim.solver_stats()}",
                   flush=True)
         phases = {"assemble_system": 0.0, "solve_system": 0.0, "update_velocities": 0.0, "update_fibers": 0.0}
-        acc = {"gemv": 0.0, "gemv_launches": 0, "gemm": 0.0, "gemm_flop": 0.0, "gemm_launches": 0}
+        acc = {"gemv": 0.0, "gemv_launches": 0, "gemm": 0.0, "gemm_flop": 0.0, "gemm_launches": 0,
+               "assemble_kernel": 0.0}
         iters = []
         library = False
         t0 = time.perf_counter()
+        last = None
         for t in range(warmup, warmup + steps):
+            if capture_last and t == warmup + steps - 1:
+                last = sim.state()
             sim.step(t)
             tm = sim.timings()
             library = library or bool(tm["gemm_library"])
@@ -93,32 +113,27 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag=""):
         elapsed = time.perf_counter() - t0
         stats = sim.solver_stats()
         pos_end, ori_end = sim.state()
+        if last is not None:
+            last = {"pos": last[0], "ori": last[1], "x": sim.system()[2]}
         device_bytes = sim.device_bytes()
         n = sim.n
     finally:
         sim.close()
-    return {"elapsed": elapsed, "phases": phases, "acc": acc, "iters": iters, "stats": stats, "n": n,
-            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps, "library": library}
-
-
-# VALU issue of the analytical theta assembly (N = 5, 8 x 3 nodes), with the
-# read-back of its rows for the first residual: lane instructions per ordered
-# fiber pair, from its counters at M = 5000
-# (profiles/assemble_theta_M5000_counters.md: 8.83e9 wave instructions per
-# launch x 64 lanes / 5000 x 4999 pairs), against one wave instruction per 4
-# cycles on each of 1024 SIMDs at 2.4 GHz.  The read-back's 13.5 GB would take
-# 1.7 ms at 8 TB/s, under the VALU time, so the bound is the issue time.
-ASSEMBLY_LANE_INSTR_PER_PAIR = 22_600
-VALU_LANE_INSTR_PER_S = 1024 * 64 / 4 * 2.4e9
+    theta = solver in ITERATIVE and matrix_format in ("auto", "theta")
+    return {"elapsed": elapsed, "phases": phases, "acc": acc, "iters": iters, "stats": stats, "n": n, "theta": theta,
+            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps, "library": library,
+            "last": last}
 
 
 def step_bound(args, g):
-    """The step at its bounds: assembly at full VALU issue of its instruction
-    count, every matvec of the step and the stored velocity pass at 8 TB/s."""
-    if not (args.integration == "analytical" and args.solver == "gmres"):
+    """The step at its bounds: the assembly's algorithmic flops at the
+    published FP64 / FP32 peaks, every matvec of the step and the stored
+    velocity pass at 8 TB/s."""
+    if not (args.solver == "gmres" and g["theta"]):
         return None
     M, N, K = args.fibers, 5, g["steps"]
-    asm_ms = 1e3 * ASSEMBLY_LANE_INSTR_PER_PAIR * M * (M - 1) / VALU_LANE_INSTR_PER_S
+    f64, f32 = ASM_FLOP_PER_PAIR[args.integration]
+    asm_ms = 1e3 * M * (M - 1) * (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12))
     mv_ms = 1e3 * M * (3 + 6 * (N - 1)) * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
     vel_ms = 1e3 * M * 12 * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
     matvecs = g["acc"]["gemv_launches"] / K
@@ -139,6 +154,35 @@ def measured_traffic(kernel, M, N):
     except (OSError, ValueError):
         return None
     return None if rec is None else float(rec["fetch_bytes"] + rec["write_bytes"])
+
+
+# Algorithmic flops of the assembly per ordered fiber pair (N = 5, 8 x 3
+# nodes): analytical 24 nodes x 624 FP64 (moment recurrences) + 17 524 FP32
+# (recombination and projections); numerical 63 216 FP32 (576 kernel
+# evaluations and the projections).  Peaks: MI355X's published FP64 and FP32
+# rates (the same for vector and matrix instructions).
+ASM_FLOP_PER_PAIR = {"analytical": (14_976, 17_524), "numerical": (0, 63_216)}
+FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS = 78.6, 157.3
+
+
+def roofline_assembly(args, g):
+    """The assembly kernel against the time its algorithmic flops take at the
+    published FP64 / FP32 peaks."""
+    M, K = args.fibers, g["steps"]
+    f64, f32 = ASM_FLOP_PER_PAIR[args.integration]
+    pairs = float(M) * (M - 1)
+    flop = pairs * (f64 + f32)
+    peak_s = pairs * (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12))
+    avg_ms = g["acc"]["assemble_kernel"] / K
+    achieved = flop / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    peak = flop / peak_s / 1e12
+    theta = g["theta"]
+    kernel = f"assemble_{'theta_' if theta else ''}kernel<5, 8, {'true' if args.integration == 'analytical' else 'false'}>"
+    return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 2), "peak": round(peak, 2),
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 4) if peak > 0 else 0.0,
+            "traffic": measured_traffic(kernel.split("<")[0], M, 5) if theta else None,
+            "flop_per_launch": flop, "fp64_flop_per_launch": pairs * f64, "fp32_flop_per_launch": pairs * f32,
+            "time_at_peak_ms": round(1e3 * peak_s, 3), "avg_launch_ms": round(avg_ms, 4)}
 
 
 def roofline(args, solver, g):
@@ -176,8 +220,9 @@ def precision_study(args, pos, ori, plain=None):
     GMRES-IR with its inner solves on the FP16 copy of the theta-form system
     (same steps and tolerance)."""
     out, ref = {}, None
+    check = oracle_check(args) if args.oracle_check else {}
     for solver in ("direct", "mixed-bf16", "mixed-fp16"):
-        g = run_gpu(args, pos, ori, solver, args.lu_steps, 1, tag="precision ")
+        g = run_gpu(args, pos, ori, solver, args.lu_steps, 1, tag="precision ", capture_last=args.oracle_check)
         rate = args.fibers * g["steps"] / g["elapsed"]
         entry = {"fiber_steps_per_s": round(rate, 2), "ms_per_step": round(1e3 * g["elapsed"] / g["steps"], 2),
                  "solve_ms_per_step": round(g["phases"]["solve_system"] / g["steps"], 2),
@@ -189,6 +234,8 @@ def precision_study(args, pos, ori, plain=None):
             entry["speedup_vs_fp32_lu"] = round(rate / ref[2], 3)
             entry["pos_rel_err_vs_fp32_lu"] = rel_err(g["pos"], ref[0])
             entry["ori_rel_err_vs_fp32_lu"] = rel_err(g["ori"], ref[1])
+        if args.oracle_check:
+            entry["fp64_oracle"] = {"rows_m5000": oracle_rows(args, g["last"]), "ten_steps_m250": check[solver]}
         out["plain_fp32_lu" if solver == "direct" else solver.replace("-", "_")] = entry
     out["steps"] = args.lu_steps
     out["warmup"] = 1
@@ -204,6 +251,58 @@ def precision_study(args, pos, ori, plain=None):
             "speedup_vs_plain_fp32_gmres": round(rate / plain_rate, 3),
             "pos_rel_err_vs_plain_fp32_gmres": rel_err(g["pos"], plain["pos"]),
             "ori_rel_err_vs_plain_fp32_gmres": rel_err(g["ori"], plain["ori"])}
+    return out
+
+
+def oracle_rows(args, last):
+    """The last step's solution in the FP64 oracle's rows of three groups of
+    three target fibers (fortran/assemble_matrix.incl restated in FP64, at the
+    state that step assembled from): ||b - A x|| / ||b|| over those rows."""
+    import numpy as np
+
+    from oracle.pyoracle import Oracle, make_params
+
+    o = Oracle(64)
+    M = args.fibers
+    p = make_params(M, threads=cpu_threads())
+    res = []
+    for i0 in (0, M // 2 - 1, M - 3):
+        A, b = o.assemble_rows(p, last["pos"], last["ori"], i0, i0 + 3, args.integration == "analytical")
+        res.append(float(np.linalg.norm(b - A @ last["x"]) / np.linalg.norm(b)))
+    return {"targets": [0, M // 2 - 1, M - 3], "fibers_per_group": 3, "relative_residual": res}
+
+
+def oracle_check(args, M=250, steps=10):
+    """Each LU form, 10 steps on a ball of M fibers at the number density of
+    the headline ball (same generator, integrals and tolerances), against the
+    FP64 oracle's trajectory (its GMRES at 1e-13, which is its LU solution):
+    the largest relative position and orientation error over the steps."""
+    import numpy as np
+
+    from oracle.pyoracle import Oracle, make_params
+    from rigidfibers_amd.config import Parameters
+    from rigidfibers_amd.simulation import Simulation
+
+    radius = args.radius * (M / args.fibers) ** (1.0 / 3.0)
+    pos, ori = build_scene(M, radius, args.seed)
+    rec = []
+    Oracle(64).simulate(make_params(M, threads=cpu_threads()), pos, ori, steps, solver="gmres",
+                        analytical=args.integration == "analytical", gmres=(30, 2000, 1e-13), record=rec)
+    params = Parameters(NUMBER_OF_FIBERS=M, TIMESTEP=0.1, SLENDERNESS=0.01, NUMBER_OF_TERMS_IN_FORCE_EXPANSION=5,
+                        NUMBER_OF_QUADRATURE_INTERVALS=8)
+    out = {}
+    for solver in ("direct", "mixed-bf16", "mixed-fp16"):
+        ep = eo = 0.0
+        with Simulation(params, pos, ori, solver=solver, integration=args.integration, refinement_tolerance=1e-10,
+                        lu_gemm=args.lu_gemm) as sim:
+            for t in range(steps):
+                sim.step(t)
+                p, o = sim.state()
+                ep, eo = max(ep, rel_err(p, rec[t]["pos"])), max(eo, rel_err(o, rec[t]["ori"]))
+        out[solver] = {"workload": f"M={M} ball of radius {radius:.2f}, {steps} steps",
+                       "pos_rel_err": ep, "ori_rel_err": eo}
+        print(f"# oracle check {solver}: M={M}, {steps} steps: positions {ep:.2e}, orientations {eo:.2e}",
+              flush=True)
     return out
 
 
@@ -292,13 +391,29 @@ def large_suspension(args):
     return out
 
 
+def cpu_threads():
+    """OMP_NUM_THREADS, else every CPU this process may run on."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, pos, ori, iters_per_step):
     """FP32 OpenMP port of the reference Fortran on a bounded sample of rows."""
     import numpy as np
 
     from oracle.pyoracle import Oracle, make_params
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     o = Oracle(32)
     M = args.fibers
     p = make_params(M, threads=threads)
@@ -327,6 +442,8 @@ def cpu_baseline(args, pos, ori, iters_per_step):
         "value": M / step_s,
         "unit": "fiber-steps/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
+        "cpus_available": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": (f"FP32 OpenMP port of the reference Fortran (oracle/librf_oracle32.so), {threads} threads: "
                    f"assembly of the rows of {S} of {M} target fibers ({t_asm:.2f} s), their velocities "
@@ -359,6 +476,8 @@ def main():
     ap.add_argument("--confirm-residual", action="store_true",
                     help="GMRES and BiCGStab: confirm a converged solve with the true residual (one matvec more)")
     ap.add_argument("--no-precision-study", action="store_true")
+    ap.add_argument("--no-oracle-check", dest="oracle_check", action="store_false",
+                    help="precision study without its comparisons against the FP64 oracle")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=64, help="suspensions stepped together")
@@ -377,6 +496,10 @@ def main():
               "(use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    # the oracle's OpenMP threads stay on neighbouring cores (read when the
+    # oracle library first starts its runtime)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     pos, ori = build_scene(args.fibers, args.radius, args.seed)
     g = run_gpu(args, pos, ori, args.solver, args.steps, args.warmup)
     M, K = args.fibers, args.steps
@@ -400,11 +523,17 @@ def main():
                                f"{args.solver} tol {args.tol:g} (restart {args.restart}), dt=0.1",
                    "fibers": M, "unknowns": g["n"], "solver": args.solver, "integration": args.integration,
                    "matrix_format": fmt},
-        "roofline": roofline(args, args.solver, g),
+        "roofline": None,
+        "roofline_solver": roofline(args, args.solver, g),
         "kernels": {k: round(v / K, 3) for k, v in g["phases"].items()},
         "solver": {"iterations_per_step": g["iters"], "last": g["stats"]},
         "device_bytes": g["device_bytes"],
     }
+    # the roofline of the step's dominant kernel: the assembly for the
+    # iterative solvers, the LU's trailing updates for the direct ones
+    asm = roofline_assembly(args, g)
+    line["roofline_assembly"] = asm
+    line["roofline"] = asm if args.solver in ITERATIVE else line["roofline_solver"]
     bound = step_bound(args, g)
     if bound is not None:
         line["step_bound"] = bound

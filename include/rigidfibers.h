@@ -165,6 +165,7 @@ typedef struct rf_timings {
   double gemv_half;        /* GMRES-IR: summed duration of the FP16 matrix-vector kernels */
   int32_t gemv_half_launches;
   int32_t gemm_library;    /* 1: the trailing updates ran through hipBLASLt (rf_config.lu_gemm) */
+  double assemble_kernel;  /* the assembly kernel alone (assemble_system less its partial-sum kernels) */
 } rf_timings;
 
 typedef struct rf_simulation rf_simulation;

@@ -87,6 +87,7 @@ class RfTimings(C.Structure):
         ("gemv_half", C.c_double),
         ("gemv_half_launches", C.c_int32),
         ("gemm_library", C.c_int32),
+        ("assemble_kernel", C.c_double),
     ]
 
 

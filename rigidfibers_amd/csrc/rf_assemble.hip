@@ -653,25 +653,28 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(th
 }
 
 // Dense A (column-major, leading dimension n) from the theta form; identity
-// diagonal blocks.  One lane per (i, j, k_j).
+// diagonal blocks.  One lane per (i, j, k_j).  Only the columns of source
+// fibers j0 .. j0 + jcount - 1 (all n rows) are written, the column of source
+// fiber j0's first unknown at A[0].
 template <int N>
 __global__ void expand_theta_kernel(int M, int64_t Lp, const float* __restrict__ T, const double* __restrict__ ori,
-                                    PhysConsts pc, float* __restrict__ A) {
+                                    PhysConsts pc, int j0, int jcount, float* __restrict__ A) {
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int ROWS = 3 * N;
   const int64_t id = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const int64_t total = static_cast<int64_t>(M) * M * N;
+  const int64_t total = static_cast<int64_t>(M) * jcount * N;
   if (id >= total) return;
   const int kj = static_cast<int>(id % N);
   const int64_t ij = id / N;
-  const int j = static_cast<int>(ij % M), i = static_cast<int>(ij / M);
+  const int jl = static_cast<int>(ij % jcount), i = static_cast<int>(ij / jcount);
+  const int j = j0 + jl;
   const int64_t n = static_cast<int64_t>(ROWS) * M;
   const int64_t e = static_cast<int64_t>(j) * N + kj;
   const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
   const float t[3] = {static_cast<float>(ori[3 * i]), static_cast<float>(ori[3 * i + 1]),
                       static_cast<float>(ori[3 * i + 2])};
   for (int b = 0; b < 3; ++b) {
-    float* col = A + (static_cast<int64_t>(ROWS) * j + 3 * kj + b) * n + static_cast<int64_t>(ROWS) * i;
+    float* col = A + (static_cast<int64_t>(ROWS) * jl + 3 * kj + b) * n + static_cast<int64_t>(ROWS) * i;
     if (i == j) {
       for (int r = 0; r < ROWS; ++r) col[r] = (r == 3 * kj + b) ? 1.0f : 0.0f;
       continue;
@@ -841,13 +844,15 @@ hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t
 #undef RF_TH_CASE
 
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
-                               float* A, hipStream_t s) {
-  const int64_t total = static_cast<int64_t>(M) * M * N;
+                               float* A, hipStream_t s, int j0, int jcount) {
+  if (jcount < 0) jcount = M - j0;
+  if (j0 < 0 || jcount < 1 || j0 + jcount > M) return hipErrorInvalidValue;
+  const int64_t total = static_cast<int64_t>(M) * jcount * N;
   const unsigned blocks = static_cast<unsigned>((total + 255) / 256);
   switch (N) {
 #define RF_EX(NN) \
   case NN:        \
-    hipLaunchKernelGGL(expand_theta_kernel<NN>, dim3(blocks), dim3(256), 0, s, M, Lp, T, ori, pc, A); \
+    hipLaunchKernelGGL(expand_theta_kernel<NN>, dim3(blocks), dim3(256), 0, s, M, Lp, T, ori, pc, j0, jcount, A); \
     break;
     RF_EX(1)
     RF_EX(2)

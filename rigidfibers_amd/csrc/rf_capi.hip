@@ -62,7 +62,8 @@ struct rf_simulation {
   bool analytical = false, vel_analytical = false;
   rf::PhysConsts pc{};
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // phase boundaries of rf_step (0..4) and the assembly kernel's own (5, 6)
+  hipEvent_t ev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int device = 0;
 
   double* pos[3] = {nullptr, nullptr, nullptr};
@@ -81,6 +82,10 @@ struct rf_simulation {
   float* A = nullptr;     // dense system (direct solvers / RF_MATRIX_DENSE)
   bool theta = false;     // system held in theta form (T) instead of A
   float* T = nullptr;
+  // orientation buffer of the last theta assembly (its dense blocks depend
+  // on t_i), and whether that buffer still holds those orientations
+  int tori = 0;
+  bool tori_valid = false;
   int64_t Lp = 0;         // theta component stride
   int bgroups = 1;        // RHS partial slots written by the last assembly
   bool vstored = false;   // the theta assembly also stores the velocity operator
@@ -778,14 +783,20 @@ rf_status rf_assemble_system(rf_simulation* s) {
       }
       if (s->guess_warm) x0 = s->x;
     }
+    RF_HIP(hipEventRecord(s->ev[5], s->stream));
     RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
                                      s->bpart, s->Tv, s->cpart, x0, x0 ? s->apart : nullptr, s->pc, s->stream,
                                      &s->bgroups));
+    RF_HIP(hipEventRecord(s->ev[6], s->stream));
+    s->tori = s->cur;
+    s->tori_valid = true;
     s->vvalid = s->vstored;
     if (s->T16) RF_HIP(rf::theta_to_fp16(s->N, s->M, s->Lp, s->T, s->T16, s->S16, s->stream));
   } else {
+    RF_HIP(hipEventRecord(s->ev[5], s->stream));
     RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
                                s->bpart, s->pc, s->stream));
+    RF_HIP(hipEventRecord(s->ev[6], s->stream));
     s->bgroups = rf::pair_grid(s->M, 192 / (3 * s->NQI)).groups;
   }
   RF_HIP(rf::launch_sum_groups(s->bgroups, s->n, s->bpart, s->b, s->stream));
@@ -837,6 +848,7 @@ rf_status rf_update_fibers(rf_simulation* s, int32_t first_timestep) {
   RF_HIP(rf::launch_update(first, s->M, s->cfg.timestep, s->pos[s->prev], s->pos[s->cur], s->ori[s->prev],
                            s->ori[s->cur], s->tv[s->vprev], s->tv[s->vcur], s->rv[s->vprev], s->rv[s->vcur],
                            s->pos[s->next], s->ori[s->next], s->stream));
+  if (s->next == s->tori) s->tori_valid = false;
   s->plast = s->next;
   s->have_next = true;
   return RF_OK;
@@ -874,6 +886,7 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.update_velocities = elapsed(s->ev[2], s->ev[3]);
   s->timings.update_fibers = elapsed(s->ev[3], s->ev[4]);
   s->timings.total = elapsed(s->ev[0], s->ev[4]);
+  s->timings.assemble_kernel = elapsed(s->ev[5], s->ev[6]);
   s->timings.gemv = 0.0;
   for (int k = 0; k < s->ngemv; ++k) s->timings.gemv += elapsed(s->gev[2 * k], s->gev[2 * k + 1]);
   s->timings.gemv_launches = s->ngemv;
@@ -936,6 +949,7 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
     RF_HIP(hipMemcpyAsync(s->rv[s->vprev], pr, bytes, hipMemcpyHostToDevice, s->stream));
   }
   RF_HIP(hipStreamSynchronize(s->stream));
+  if (s->tori == s->cur || (history && s->tori == s->prev)) s->tori_valid = false;
   s->have_history = history != 0;
   s->have_next = false;
   s->vvalid = false;
@@ -948,6 +962,9 @@ rf_status rf_set_state(rf_simulation* s, const double* positions, const double* 
 rf_status rf_get_system(rf_simulation* s, float* a_matrix, double* b_vector, double* x_vector) {
   if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
   const int64_t n = s->n;
+  if (a_matrix && s->theta && !s->tori_valid)
+    return fail(RF_ERR_INVALID_ARGUMENT, "no theta-form system to expand (none assembled, or the state it was "
+                                         "assembled from has been overwritten)");
   if (a_matrix && s->theta) {
     // rebuild the dense blocks in a temporary device buffer
     float* tmp = nullptr;
@@ -956,7 +973,7 @@ rf_status rf_get_system(rf_simulation* s, float* a_matrix, double* b_vector, dou
       (void)hipGetLastError();
       return fail(RF_ERR_OUT_OF_MEMORY, "no device memory to expand the theta-form matrix");
     }
-    e = rf::launch_expand_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], s->pc, tmp, s->stream);
+    e = rf::launch_expand_theta(s->N, s->M, s->Lp, s->T, s->ori[s->tori], s->pc, tmp, s->stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(a_matrix, tmp, sizeof(float) * static_cast<size_t>(n) * n, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

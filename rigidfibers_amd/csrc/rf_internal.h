@@ -47,8 +47,11 @@ hipError_t launch_residual_groups(int groups, int64_t len, const double* apart, 
 hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, const double* cpart, int groups,
                                     const double* ori, const double* x, const PhysConsts& pc, double* tvel,
                                     double* rvel, hipStream_t s);
+// Dense columns of source fibers j0 .. j0 + jcount - 1 (jcount < 0: up to M)
+// from the theta form into A (leading dimension n, source fiber j0's first
+// column at A[0]).
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
-                               float* A, hipStream_t s);
+                               float* A, hipStream_t s, int j0 = 0, int jcount = -1);
 // FP16 copy of T with one FP32 scale per (target, source) block, S[i * M + j]
 // (rf_linalg.hip), and the theta GEMV on it.
 hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s);

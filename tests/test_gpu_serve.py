@@ -160,3 +160,51 @@ def test_oversized_request_refused():
         with pytest.raises(_lib.RfError) as e:
             srv.submit(pos, ori, steps)
     assert e.value.code == _lib.RF_ERR_OUT_OF_MEMORY
+
+
+def test_served_mix_matches_oracle(oracle64):
+    """A mix spanning the serving stream's range of sizes, M = 16 .. 400 (cube
+    at number density 0.1, GMRES tolerance 1e-8, restart 10, each solve
+    starting from the previous solution), served together for 10 steps:
+    every suspension's state against the FP64 oracle's own trajectory
+    (fortran/fibers.f95:182-349 per suspension, as the reference runs one
+    suspension per process, src/main.cu:83-99) within 1e-5."""
+    from oracle.pyoracle import make_params
+    from rigidfibers_amd.serve import Server
+
+    steps = 10
+    reqs = _requests([16, 400, 57, 230, 121, 333], steps=steps, seed=31)
+    with Server(_params(tol=1e-8), max_active=8) as srv:
+        served = _serve(srv, reqs)
+    for (M, pos, ori, _), r in zip(reqs, served):
+        assert r["info"]["status"] == 0
+        p_ref, o_ref = oracle64.simulate(make_params(M, threads=8), pos, ori, steps, solver="gmres",
+                                         gmres=(10, 1000, 1e-8))
+        ep, eo = rel_err(r["positions"], p_ref), rel_err(r["orientations"], o_ref)
+        print(f"M={M}: positions {ep:.1e}, orientations {eo:.1e} against the FP64 oracle")
+        assert ep <= 1e-5 and eo <= 1e-5
+
+
+def test_full_request_stream():
+    """The whole serving stream -- 512 suspensions, M ~ U[16, 400], 20 steps
+    each, GMRES tolerance 1e-6 -- with arrivals ten times faster than its 50
+    per second: every request finishes with status OK after its 20 steps, and
+    eight of them, run again alone, give bitwise the same final state."""
+    from rigidfibers_amd.serve import Server, request_stream, serve_stream
+
+    reqs = request_stream(count=512, m_range=(16, 400), density=0.1, rate=50.0, steps=20, seed=2026)
+    with Server(_params(tol=1e-6), max_active=64) as srv:
+        res, wall = serve_stream(srv, reqs, time_scale=0.1)
+        st = srv.stats()
+        print(f"512 requests in {wall:.2f} s, {st['ticks']} ticks, peak {st['peak_bytes'] / 1e6:.0f} MB")
+        assert len(res) == 512 and all(r["status"] == 0 for r in res)
+        assert all(len(r["step_ms"]) == 20 for r in res)
+        assert st["completed"] == 512 and st["queued"] == 0 and st["active"] == 0
+        assert st["fiber_steps"] == 20 * sum(M for _, M, _, _, _ in reqs)
+        for k in (0, 1, 100, 255, 256, 300, 450, 511):
+            _, M, pos, ori, steps = reqs[k]
+            alone = _serve(srv, [(M, pos, ori, steps)])[0]
+            assert alone["info"]["status"] == 0
+            assert np.array_equal(alone["positions"], res[k]["positions"])
+            assert np.array_equal(alone["orientations"], res[k]["orientations"])
+            assert alone["info"]["gmres_iterations"] == res[k]["gmres_iterations"]
