@@ -85,7 +85,7 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
     matrix_format = args.matrix_format if solver in ITERATIVE else "auto"
     sim = Simulation(params, pos, ori, solver=solver, integration=args.integration, matrix_format=matrix_format,
                      velocity_operator=args.velocity_operator, refinement_tolerance=1e-10,
-                     initial_guess=args.initial_guess, lu_gemm=args.lu_gemm,
+                     initial_guess=args.initial_guess,
                      separate_residual=args.separate_residual, confirm_residual=args.confirm_residual)
     try:
         for t in range(warmup):
@@ -96,7 +96,6 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
         acc = {"gemv": 0.0, "gemv_launches": 0, "gemm": 0.0, "gemm_flop": 0.0, "gemm_launches": 0,
                "assemble_kernel": 0.0}
         iters = []
-        library = False
         t0 = time.perf_counter()
         last = None
         for t in range(warmup, warmup + steps):
@@ -104,7 +103,6 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
                 last = sim.state()
             sim.step(t)
             tm = sim.timings()
-            library = library or bool(tm["gemm_library"])
             for k in phases:
                 phases[k] += tm[k]
             for k in acc:
@@ -121,7 +119,7 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
         sim.close()
     theta = solver in ITERATIVE and matrix_format in ("auto", "theta")
     return {"elapsed": elapsed, "phases": phases, "acc": acc, "iters": iters, "stats": stats, "n": n, "theta": theta,
-            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps, "library": library,
+            "device_bytes": device_bytes, "pos": pos_end, "ori": ori_end, "steps": steps,
             "last": last}
 
 
@@ -201,7 +199,7 @@ def roofline(args, solver, g):
                 "bytes_per_launch": per_launch, "avg_launch_ms": round(avg_ms, 4)}
     peak = MFMA_PEAK_TFLOPS[solver]
     achieved = acc["gemm_flop"] / (acc["gemm"] * 1e-3) / 1e12 if acc["gemm"] > 0 else 0.0
-    kernel = "hipBLASLt TN GEMM (trailing updates)" if g.get("library") else GEMM_KERNEL[solver]
+    kernel = GEMM_KERNEL[solver]
     return {"bound": "mfma", "kernel": kernel, "achieved": round(achieved, 1), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
             "flop_per_step": acc["gemm_flop"] / g["steps"], "launches_per_step": acc["gemm_launches"] / g["steps"],
@@ -293,8 +291,8 @@ def oracle_check(args, M=250, steps=10):
     out = {}
     for solver in ("direct", "mixed-bf16", "mixed-fp16"):
         ep = eo = 0.0
-        with Simulation(params, pos, ori, solver=solver, integration=args.integration, refinement_tolerance=1e-10,
-                        lu_gemm=args.lu_gemm) as sim:
+        with Simulation(params, pos, ori, solver=solver, integration=args.integration,
+                        refinement_tolerance=1e-10) as sim:
             for t in range(steps):
                 sim.step(t)
                 p, o = sim.state()
@@ -469,8 +467,6 @@ def main():
     ap.add_argument("--initial-guess", default="previous", choices=["previous", "zero", "extrapolated"],
                     help="iterative solvers' start: the previous solution or 2 x_n - x_{n-1}")
     ap.add_argument("--lu-steps", type=int, default=2, help="timed steps per solver of the precision study")
-    ap.add_argument("--lu-gemm", default="library", choices=["library", "builtin"],
-                    help="mixed LUs' trailing updates: hipBLASLt when it has an algorithm, or the built-in kernel")
     ap.add_argument("--separate-residual", action="store_true",
                     help="form the first residual of a warm start with a matvec, not in the assembly kernel")
     ap.add_argument("--confirm-residual", action="store_true",

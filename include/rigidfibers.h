@@ -124,9 +124,7 @@ typedef struct rf_config {
   int32_t matrix_format;                              /* rf_matrix_format */
   int32_t velocity_operator;                          /* rf_velocity_operator */
   double inner_tolerance;                             /* GMRES-IR: relative residual of each inner solve (0: 1e-4) */
-  int32_t lu_gemm;                                    /* mixed LUs' trailing updates: 0 hipBLASLt when it has an
-                                                         algorithm (else the built-in kernel), 1 the built-in
-                                                         MFMA kernel always */
+  int32_t reserved0;                                  /* unused: 0 */
   int32_t separate_residual;                          /* iterative solvers on the theta form: 0 the first residual
                                                          b - A x0 of a warm start comes from the assembly kernel
                                                          (one matvec fewer per solve), 1 from a matvec */
@@ -159,12 +157,13 @@ typedef struct rf_timings {
   double total;
   double gemv;             /* summed duration of the matrix-vector kernels of the solve */
   int32_t gemv_launches;   /* how many there were */
-  int32_t gemm_launches;   /* direct solvers: trailing-update GEMMs of the LU factorisation */
+  int32_t gemm_launches;   /* direct solvers: trailing-update GEMMs of the LU factorisation (built-in MFMA kernel) */
   double gemm;             /* their summed duration */
   double gemm_flop;        /* their algorithmic flops, 2 m n K each */
   double gemv_half;        /* GMRES-IR: summed duration of the FP16 matrix-vector kernels */
   int32_t gemv_half_launches;
-  int32_t gemm_library;    /* 1: the trailing updates ran through hipBLASLt (rf_config.lu_gemm) */
+  int32_t direct_fallback; /* direct solvers: 1 when the refinement did not converge and GMRES on the dense
+                              system finished the solve (iterations then count both) */
   double assemble_kernel;  /* the assembly kernel alone (assemble_system less its partial-sum kernels) */
 } rf_timings;
 

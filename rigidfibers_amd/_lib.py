@@ -61,7 +61,7 @@ class RfConfig(C.Structure):
         ("matrix_format", C.c_int32),
         ("velocity_operator", C.c_int32),
         ("inner_tolerance", C.c_double),
-        ("lu_gemm", C.c_int32),
+        ("reserved0", C.c_int32),
         ("separate_residual", C.c_int32),
         ("confirm_residual", C.c_int32),
         ("reserved", C.c_int32 * 1),
@@ -86,7 +86,7 @@ class RfTimings(C.Structure):
         ("gemm_flop", C.c_double),
         ("gemv_half", C.c_double),
         ("gemv_half_launches", C.c_int32),
-        ("gemm_library", C.c_int32),
+        ("direct_fallback", C.c_int32),
         ("assemble_kernel", C.c_double),
     ]
 

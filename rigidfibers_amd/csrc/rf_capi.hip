@@ -123,6 +123,7 @@ struct rf_simulation {
   rf::GemvPlan plan{};
   rf::LuWorkspace lu{};
   rf::LuResult lu_last{};  // factorisation statistics of the last direct solve
+  bool direct_fallback = false;  // ... which GMRES finished
 
   rf_solver_stats stats{};
   rf_timings timings{};
@@ -176,7 +177,6 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown integration");
   if (c->matrix_format < RF_MATRIX_AUTO || c->matrix_format > RF_MATRIX_THETA)
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown matrix format");
-  if (c->lu_gemm != 0 && c->lu_gemm != 1) return fail(RF_ERR_INVALID_ARGUMENT, "lu_gemm must be 0 or 1");
   if (c->separate_residual != 0 && c->separate_residual != 1)
     return fail(RF_ERR_INVALID_ARGUMENT, "separate_residual must be 0 or 1");
   if (c->confirm_residual != 0 && c->confirm_residual != 1)
@@ -190,7 +190,8 @@ rf_status validate_config(const rf_config* c) {
     return fail(RF_ERR_INVALID_ARGUMENT, "unknown velocity operator");
   if (c->solver == RF_SOLVER_GMRES_IR && !(c->inner_tolerance >= 0.0 && c->inner_tolerance < 1.0))
     return fail(RF_ERR_INVALID_ARGUMENT, "GMRES-IR inner tolerance must be in [0, 1)");
-  if (c->solver == RF_SOLVER_GMRES || c->solver == RF_SOLVER_GMRES_IR) {
+  // GMRES settings: its own, GMRES-IR's, and the direct solvers' fallback
+  if (c->solver != RF_SOLVER_BICGSTAB) {
     if (c->gmres_restart < 1 || c->gmres_restart > 1024)
       return fail(RF_ERR_INVALID_ARGUMENT, "GMRES RESTART must be in [1..1024]");
     if (c->gmres_max_iterations < 1) return fail(RF_ERR_INVALID_ARGUMENT, "GMRES MAX_ITERATIONS must be >= 1");
@@ -232,9 +233,9 @@ void make_consts(const rf_config& c, rf::PhysConsts* pc) {
   }
 }
 
-int solver_kdim(const rf_config& c) {
-  return (c.solver == RF_SOLVER_GMRES || c.solver == RF_SOLVER_GMRES_IR) ? c.gmres_restart + 1 : 0;
-}
+// Krylov basis vectors: GMRES and GMRES-IR, and the direct solvers' GMRES
+// fallback.
+int solver_kdim(const rf_config& c) { return c.solver == RF_SOLVER_BICGSTAB ? 0 : c.gmres_restart + 1; }
 
 // y = A x or y = b - A x on the simulation's stream, timed with an event pair
 // while the pool lasts.
@@ -558,11 +559,24 @@ rf_status solve_direct(rf_simulation* s) {
                               s->cfg.refinement_tolerance, s->cfg.refinement_max_iterations, s->stream, &r);
   if (e != hipSuccess) return fail(RF_ERR_DEVICE, std::string("LU: ") + hipGetErrorString(e));
   s->lu_last = r;
-  if (r.info != 0) return fail(RF_ERR_SINGULAR, "zero pivot at column " + std::to_string(r.info));
   s->stats.iterations = r.refinement_iterations;
   s->stats.relative_residual = r.relative_residual;
   s->stats.converged = r.converged ? 1 : 0;
-  if (!std::isfinite(r.relative_residual)) return fail(RF_ERR_NONFINITE, "non-finite solution in LU solve");
+  s->direct_fallback = false;
+  if (r.info == 0 && r.converged) return RF_OK;
+  // The pivots are chosen within 64-row diagonal blocks; when that choice
+  // leaves the factors too inaccurate for the refinement (an exact zero
+  // pivot, or element growth), GMRES on the dense system finishes the solve
+  // from the refined x (from zero after a zero pivot) to the refinement
+  // tolerance, so the step never returns an unverified x.
+  const bool warm = r.info == 0 && std::isfinite(r.relative_residual);
+  rf_solver_stats st{};
+  RF_TRY(gmres_core(s, s->b, s->x, warm, s->cfg.refinement_tolerance, s->cfg.gmres_max_iterations, false, &st));
+  s->direct_fallback = true;
+  s->stats.iterations = r.refinement_iterations + st.iterations;
+  s->stats.relative_residual = st.relative_residual;
+  s->stats.converged = st.converged;
+  if (!std::isfinite(st.relative_residual)) return fail(RF_ERR_NONFINITE, "non-finite solution in the direct solve");
   return RF_OK;
 }
 
@@ -733,7 +747,7 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
     const rf::LuMode mode = cfg->solver == RF_SOLVER_DIRECT ? rf::LuMode::kFP32
                             : cfg->solver == RF_SOLVER_DIRECT_MIXED_BF16 ? rf::LuMode::kBF16
                                                                           : rf::LuMode::kFP16;
-    hipError_t e = rf::lu_workspace_create(mode, n, s->lda, cfg->lu_gemm == 0, &s->lu);
+    hipError_t e = rf::lu_workspace_create(mode, n, s->lda, &s->lu);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return cleanup(fail(e == hipErrorOutOfMemory ? RF_ERR_OUT_OF_MEMORY : RF_ERR_DEVICE,
@@ -896,7 +910,7 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.gemm_launches = s->lu_last.gemm_launches;
   s->timings.gemm = s->lu_last.gemm_ms;
   s->timings.gemm_flop = s->lu_last.gemm_flop;
-  s->timings.gemm_library = s->lu_last.library ? 1 : 0;
+  s->timings.direct_fallback = s->direct_fallback ? 1 : 0;
   RF_TRY(rf_advance(s));
   if (!s->stats.converged) {
     g_last_error = "solver did not reach its tolerance (relative residual " +

@@ -1,4 +1,4 @@
-// rf_gemm.hip -- C -= A B on MFMA for the updates of the LU factorisation.
+// rf_gemm.hip -- C -= A B (or C = A B) on MFMA for the LU factorisation.
 //
 // The operands are packed K-contiguous: row r of At holds A[r][0..K) and row c
 // of Bt holds B[0..K)[c] (the LU writes these packs while it solves its
@@ -7,6 +7,11 @@
 //   __bf16   v_mfma_f32_32x32x16_bf16
 //   _Float16 v_mfma_f32_32x32x16_f16
 // always with FP32 accumulation into the FP32 C (column-major, ldc).
+//
+// Per-block scaling: the packs of the mixed modes may hold each 32-row block
+// of A (32-column block of B) multiplied by a power of two, sa[r / 32]
+// (sb[c / 32]); the epilogue multiplies each 32 x 32 accumulator block by
+// 1 / (sa sb), exactly, before it updates C.
 //
 // Tiling: a workgroup of WM x WN waves owns a 256x256 tile of C; each wave
 // computes a (256/WM) x (256/WN) sub-tile as 32x32 MFMA blocks.  K advances
@@ -36,12 +41,13 @@ constexpr int kTile = 256;
 constexpr int kStageBytes = 128;
 constexpr int kLdsRowBytes = kStageBytes + 16;
 
-template <typename T, int WM, int WN, int PF>
+template <typename T, int WM, int WN, int PF, bool STORE>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, int n, int K,
                                                                        const T* __restrict__ At, int64_t ldak,
                                                                        const T* __restrict__ Bt, int64_t ldbk,
                                                                        float* __restrict__ C, int64_t ldc,
-                                                                       int vec4) {
+                                                                       int vec4, const float* __restrict__ sa,
+                                                                       const float* __restrict__ sb) {
   constexpr int NT = 64 * WM * WN;              // threads
   constexpr int E = 16 / sizeof(T);             // elements per 16-byte chunk
   constexpr int BK = kStageBytes / sizeof(T);   // K per stage
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
     }
   }
 
-  // C -= acc
+  // C -= acc / (sa sb), or C = acc (STORE)
 #pragma unroll
   for (int ta = 0; ta < TM; ++ta)
 #pragma unroll
@@ -166,20 +172,38 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
       const int64_t col = n0 + wn + 32 * tb + (lane & 31);
       if (col >= n) continue;
       float* cc = C + col * ldc;
+      float w = 1.0f;
+      if (!STORE && sa) {  // exact: powers of two
+        const int64_t rb = (m0 + wm + 32 * ta) >> 5;
+        w = 1.0f / (sa[rb] * sb[col >> 5]);
+      }
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int64_t row = m0 + wm + 32 * ta + 8 * g + 4 * (lane >> 5);
         if (vec4 && row + 3 < m) {
-          v4f c = *reinterpret_cast<const v4f*>(cc + row);
-          c[0] -= acc[ta][tb][4 * g + 0];
-          c[1] -= acc[ta][tb][4 * g + 1];
-          c[2] -= acc[ta][tb][4 * g + 2];
-          c[3] -= acc[ta][tb][4 * g + 3];
+          v4f c;
+          if constexpr (STORE) {
+            c[0] = acc[ta][tb][4 * g + 0];
+            c[1] = acc[ta][tb][4 * g + 1];
+            c[2] = acc[ta][tb][4 * g + 2];
+            c[3] = acc[ta][tb][4 * g + 3];
+          } else {
+            c = *reinterpret_cast<const v4f*>(cc + row);
+            c[0] = fmaf(-w, acc[ta][tb][4 * g + 0], c[0]);
+            c[1] = fmaf(-w, acc[ta][tb][4 * g + 1], c[1]);
+            c[2] = fmaf(-w, acc[ta][tb][4 * g + 2], c[2]);
+            c[3] = fmaf(-w, acc[ta][tb][4 * g + 3], c[3]);
+          }
           *reinterpret_cast<v4f*>(cc + row) = c;
         } else {
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (row + i < m) cc[row + i] -= acc[ta][tb][4 * g + i];
+            if (row + i < m) {
+              if constexpr (STORE)
+                cc[row + i] = acc[ta][tb][4 * g + i];
+              else
+                cc[row + i] = fmaf(-w, acc[ta][tb][4 * g + i], cc[row + i]);
+            }
         }
       }
     }
@@ -197,15 +221,29 @@ struct GemmShape {
 int gemm_stage_k(LuMode mode) { return mode == LuMode::kFP32 ? kStageBytes / 4 : kStageBytes / 2; }
 
 template <typename T>
-static void launch_gemm(dim3 grid, int m, int n, int K, const void* At, int64_t ldak, const void* Bt, int64_t ldbk,
-                        float* C, int64_t ldc, int vec4, hipStream_t s) {
+static void launch_gemm(dim3 grid, bool store, int m, int n, int K, const void* At, int64_t ldak, const void* Bt,
+                        int64_t ldbk, float* C, int64_t ldc, int vec4, const float* sa, const float* sb,
+                        hipStream_t s) {
   using S = GemmShape<T>;
-  hipLaunchKernelGGL((gemm_packed_sub_kernel<T, S::WM, S::WN, S::PF>), grid, dim3(64 * S::WM * S::WN), 0, s, m, n,
-                     K, static_cast<const T*>(At), ldak, static_cast<const T*>(Bt), ldbk, C, ldc, vec4);
+  if (store)
+    hipLaunchKernelGGL((gemm_packed_sub_kernel<T, S::WM, S::WN, S::PF, true>), grid, dim3(64 * S::WM * S::WN), 0, s,
+                       m, n, K, static_cast<const T*>(At), ldak, static_cast<const T*>(Bt), ldbk, C, ldc, vec4, sa,
+                       sb);
+  else
+    hipLaunchKernelGGL((gemm_packed_sub_kernel<T, S::WM, S::WN, S::PF, false>), grid, dim3(64 * S::WM * S::WN), 0,
+                       s, m, n, K, static_cast<const T*>(At), ldak, static_cast<const T*>(Bt), ldbk, C, ldc, vec4, sa,
+                       sb);
 }
 
 hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
-                           const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s) {
+                           const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s, const float* sa,
+                           const float* sb) {
+  return gemm_packed(mode, false, m, n, K, At, ldak, Bt, ldbk, C, ldc, s, sa, sb);
+}
+
+hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
+                       const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s, const float* sa,
+                       const float* sb) {
   if (m <= 0 || n <= 0 || K <= 0) return hipSuccess;
   const int bk = gemm_stage_k(mode);
   const int esz = mode == LuMode::kFP32 ? 4 : 2;
@@ -219,15 +257,16 @@ hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const v
   // (measured faster than grouping tiles per XCD)
   const dim3 grid(static_cast<unsigned>((m + kTile - 1) / kTile), static_cast<unsigned>((n + kTile - 1) / kTile));
   const int mi = static_cast<int>(m), ni = static_cast<int>(n), ki = static_cast<int>(K);
+  if ((sa == nullptr) != (sb == nullptr)) return hipErrorInvalidValue;
   switch (mode) {
     case LuMode::kFP32:
-      launch_gemm<float>(grid, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, s);
+      launch_gemm<float>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
       break;
     case LuMode::kBF16:
-      launch_gemm<__bf16>(grid, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, s);
+      launch_gemm<__bf16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
       break;
     case LuMode::kFP16:
-      launch_gemm<_Float16>(grid, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, s);
+      launch_gemm<_Float16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
       break;
   }
   return hipGetLastError();

@@ -22,14 +22,20 @@ struct LuWorkspace {
   int* perm = nullptr;     // perm[k0 + c] = original row of factored row k0 + c (block-local)
   int* info = nullptr;     // device: first zero pivot (1-based) or 0
   double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
-  void* At = nullptr;      // packed L panel: n rows x kLuPanel, K-contiguous, mode's element type
-  void* Bt = nullptr;      // packed U panel: n columns x kLuPanel
-  void* At2 = nullptr;     // the same for the next panel, factored while the current one's
-  void* Bt2 = nullptr;     // trailing update runs (look-ahead)
+  // packs of kLuPanel values per row, K-contiguous, in the mode's element type
+  void* At = nullptr;      // L21 of the current panel: row r - KE = L[r][K0 .. KE), per-32-row scales sA
+  void* At2 = nullptr;     // the same for the next panel, formed while the current one's trailing update
+  float* sA = nullptr;     // runs (look-ahead)
+  float* sA2 = nullptr;
+  void* Bt = nullptr;      // U12: row c - KE = U[K0 .. KE)[c], per-32-column scales sB
+  float* sB = nullptr;
+  void* Xm = nullptr;      // A12 before its solve (pack of columns), main stream
+  void* Ya = nullptr;      // A21 before its solve (pack of rows), aux stream
+  void* Tinv = nullptr;    // kLuPanel^2: the forward operator of the panel's diagonal block, row-major
+  void* Uinv = nullptr;    // kLuPanel^2: U_D^-1 by columns
   hipStream_t aux = nullptr;  // the look-ahead's stream (default priority)
   hipEvent_t next_cols = nullptr, next_done = nullptr;
   double* Dinv = nullptr;  // per diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 FP64 each, [j][c]
-  void* blt = nullptr;     // hipBLASLt state for the mixed modes' trailing updates (none: built-in kernel)
   static constexpr int kEvents = 256;  // event pairs around the trailing updates
   hipEvent_t ev[2 * kEvents] = {};
   int64_t bytes = 0;
@@ -44,28 +50,23 @@ struct LuResult {
   double gemm_ms = 0.0;           // summed duration of the trailing-update GEMM launches
   double gemm_flop = 0.0;         // their algorithmic flops (2 m n K each)
   int gemm_launches = 0;
-  bool library = false;           // the trailing updates ran through hipBLASLt
 };
 
-// C -= A B with K-contiguous packs At (m rows, row stride ldak elements) and Bt
-// (n rows, stride ldbk) of the mode's element type; K a multiple of
-// gemm_stage_k(mode), pack rows 16-byte aligned.  C FP32 column-major.
+// C -= A B (store: C = A B) with K-contiguous packs At (m rows, row stride
+// ldak elements) and Bt (n rows, stride ldbk) of the mode's element type; K a
+// multiple of gemm_stage_k(mode), pack rows 16-byte aligned.  C FP32
+// column-major.  sa, sb (both or neither; subtracting form only): the packs
+// hold row block r / 32 of A times sa[r / 32] and column block c / 32 of B
+// times sb[c / 32] (powers of two), which the update divides out.
 int gemm_stage_k(LuMode mode);
+hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
+                       const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s,
+                       const float* sa = nullptr, const float* sb = nullptr);
 hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
-                           const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s);
+                           const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s,
+                           const float* sa = nullptr, const float* sb = nullptr);
 
-// The same update through hipBLASLt (mixed modes): blt_create makes the
-// library state (handle, descriptor, workspace of *bytes), blt_gemm_sub runs
-// it and returns hipErrorNotSupported when the library has no algorithm.
-hipError_t blt_create(LuMode mode, void** out, int64_t* bytes);
-void blt_destroy(void* p);
-hipError_t blt_gemm_sub(void* p, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak, const void* Bt,
-                        int64_t ldbk, float* C, int64_t ldc, hipStream_t s);
-
-// library: run the mixed modes' trailing updates through hipBLASLt when it is
-// available (otherwise, and for FP32 and the in-panel updates, the built-in
-// kernel).
-hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, bool library, LuWorkspace* ws);
+hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws);
 int64_t lu_workspace_bytes(const LuWorkspace& ws);
 void lu_workspace_destroy(LuWorkspace* ws);
 

@@ -5,54 +5,41 @@
 // (fortran/solve_system.incl:3): LU with row pivoting, then L y = P b, U x = y.
 //
 // Factorisation of a copy F of A (A itself is kept for the residuals), in
-// outer panels of kLuPanel columns made of diagonal blocks of kLuNb:
-//   for each block k0 of the panel:
-//     diag    one wavefront factors the 64 x 64 diagonal block in registers
-//             with partial pivoting among the block's rows, keeping the row
-//             order as a permutation (perm[k0 + c] = source row of row c);
-//     l21     L21 = A21 U11^-1 for every row below, written to F and packed
-//             K-contiguous into At (the GEMM's A operand, mode's type);
-//     u12     the block's permutation is gathered and U12 = L11^-1 A12 solved
-//             for the panel's remaining columns, written to F and packed into Bt;
-//     gemm    A22 -= L21 U12 over the panel's remaining columns (K = 64).
-//   then the panel's rows of U for the trailing columns (u12 + gemm per block)
-//   and one trailing update A22 -= L21 U12 with K = kLuPanel on MFMA.
-// The trailing updates are the O(n^3) part; their K = 1024 keeps the FP32 C
-// read-modify-write traffic at 8 n^3 / (3 K) bytes.
+// outer panels of kLuPanel = W columns.  Panel [K0, KE):
+//   1. its W x W diagonal block D is factored in place, 64-column blocks at a
+//      time: diag (one wavefront, partial pivoting among the block's rows,
+//      the row order kept as a permutation perm[k0 + c] = source row of row
+//      c), l21 and u12 (the block's triangular solves for D's rows and
+//      columns), dupdate (D's Schur complement, FP32);
+//   2. the inverses of D's factors, Uinv = U_D^-1 and Tfwd = the forward
+//      operator of D (the block permutations and L_D^-1 as the forward solve
+//      applies them), by running the panel solves on the unit vectors;
+//   3. the panel's rows below D, L21 = A21 Uinv, and its rows of U for the
+//      trailing columns, U12 = Tfwd A12, each one MFMA GEMM with K = W from
+//      packed copies of A21 / A12 (written to F);
+//   4. the trailing update A22 -= L21 U12 (K = W) on MFMA from the packs of
+//      L21 and U12 in the mode's element type, each 32-row block of L21 and
+//      32-column block of U12 scaled by a power of two that puts its largest
+//      magnitude in [2^14, 2^15) (the update divides the scales out exactly).
+// A one-panel look-ahead overlaps steps 1-3 of panel p + 1 (aux stream) with
+// the trailing update of panel p on the columns right of panel p + 1.
 //
-// Pivoting is restricted to the diagonal block (the system's identity-
+// Pivoting is restricted to the 64-row diagonal blocks (the system's identity-
 // dominated diagonal blocks make that pivot choice safe; the refinement below
-// verifies every solve).  The columns left of a block are not permuted: the
-// forward solve subtracts their contribution in the block's original row
-// order and then applies the block's permutation, which is the same L y = P b.
+// verifies every solve and the solver falls back to GMRES when it does not
+// converge).  The columns left of a block are not permuted: the forward solve
+// subtracts their contribution in the block's original row order and then
+// applies the block's permutation, which is the same L y = P b.
 //
 // Every solve is followed by FP64 iterative refinement with residuals of the
 // FP32 A (r = b - A x accumulated in FP64), so the returned x carries a
-// verified ||b - A x|| / ||b||.  The trailing updates round L21 and U12 to the
-// mode's type (unit scale: the factors of this operator are O(1), far inside
-// the FP16 range; an overflow would surface as a non-finite residual).
+// verified ||b - A x|| / ||b||.
 #include <cmath>
 #include <vector>
 
 #include "rf_lu.h"
 
 namespace rf {
-
-// 64 FP32 values -> 16-byte stores of the pack type.
-template <typename T>
-__device__ __forceinline__ void store_packed(const float* x, T* __restrict__ dst) {
-  constexpr int E = 16 / sizeof(T);
-#pragma unroll
-  for (int c = 0; c < kLuNb; c += E) {
-    union {
-      T v[E];
-      uint4 u;
-    } pk;
-#pragma unroll
-    for (int e = 0; e < E; ++e) pk.v[e] = static_cast<T>(x[c + e]);
-    *reinterpret_cast<uint4*>(dst + c) = pk.u;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // diag: GEPP of the kb x kb block at (k0, k0) by one wavefront, lane = row.
@@ -146,11 +133,9 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restr
   if (lane == 0 && zero_col != 0 && info[0] == 0) info[0] = static_cast<int>(k0) + zero_col;
 }
 
-// l21: rows [r0, r1), lane = row: x U11 = a, written to F and packed into
-// At[(r - K0) * kLuPanel + (k0 - K0) + c].
-template <typename T>
+// l21: rows [r0, r1), lane = row: x U11 = a, written to F.
 __global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
-                                                     int64_t k0, int64_t K0, T* __restrict__ At) {
+                                                     int64_t k0) {
   __shared__ float Ut[kLuNb][kLuNb];  // Ut[c][j] = U[j][c]
   __shared__ float rd[kLuNb];
   for (int e = threadIdx.x; e < kLuNb * kLuNb; e += 256) {
@@ -176,16 +161,13 @@ __global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int
   }
 #pragma unroll
   for (int c = 0; c < kLuNb; ++c) F[(k0 + c) * lda + r] = x[c];
-  store_packed<T>(x, At + (r - K0) * kLuPanel + (k0 - K0));
 }
 
 // u12: columns [c0, c1), 64 per workgroup (lane = column for the solve): rows
 // k0.. of the block gathered by its permutation, L11 y = a (unit lower),
-// written to F and packed into Bt[(c - K0) * kLuPanel + (k0 - K0) + r].
-template <typename T>
+// written to F.
 __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
-                                                    int64_t k0, int64_t K0, const int* __restrict__ perm,
-                                                    T* __restrict__ Bt) {
+                                                    int64_t k0, const int* __restrict__ perm) {
   __shared__ float S[kLuNb][kLuNb + 1];   // [column][row]
   __shared__ __attribute__((aligned(16))) float Lr[kLuNb][kLuNb + 4];  // Lr[r][j] = L[r][j], 16-byte rows
   const int lane = threadIdx.x;
@@ -209,8 +191,6 @@ __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int6
     for (int j = 0; j < r; ++j) v = fmaf(-Lr[r][j], x[j], v);
     x[r] = v;
   }
-  const int64_t mycol = cb + lane;
-  if (mycol < c1) store_packed<T>(x, Bt + (mycol - K0) * kLuPanel + (k0 - K0));
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) S[lane][r] = x[r];
   __syncthreads();
@@ -218,6 +198,162 @@ __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int6
   for (int cc = 0; cc < kLuNb; ++cc) {
     const int64_t col = cb + cc;
     if (col < c1) F[col * lda + k0 + lane] = S[cc][lane];
+  }
+}
+
+// dupdate: F[k1:e, k1:e] -= L[k1:e, k0:k1] U[k0:k1, k1:e] (the diagonal
+// block's Schur complement after block k0, K = 64, FP32).  64 x 64 tile per
+// workgroup, 4 x 4 outputs per thread.
+__global__ __launch_bounds__(256) void lu_dupdate_kernel(int64_t lda, float* __restrict__ F, int64_t k0, int64_t k1,
+                                                         int64_t e) {
+  __shared__ float Ls[kLuNb][kLuNb + 1];  // [k][row]
+  __shared__ float Us[kLuNb][kLuNb + 1];  // [k][col]
+  const int tid = threadIdx.x;
+  const int64_t r0 = k1 + static_cast<int64_t>(blockIdx.x) * kLuNb;
+  const int64_t cc0 = k1 + static_cast<int64_t>(blockIdx.y) * kLuNb;
+  for (int q = tid; q < kLuNb * kLuNb; q += 256) {
+    const int k = q >> 6, i = q & 63;
+    Ls[k][i] = (r0 + i < e) ? F[(k0 + k) * lda + r0 + i] : 0.0f;
+    Us[i][k] = (cc0 + k < e) ? F[(cc0 + k) * lda + k0 + i] : 0.0f;  // q: column k, row i of U
+  }
+  __syncthreads();
+  const int tr = (tid & 15) * 4, tc = (tid >> 4) * 4;
+  float acc[4][4] = {};
+#pragma unroll 8
+  for (int k = 0; k < kLuNb; ++k) {
+    float a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = Ls[k][tr + i];
+      b[i] = Us[k][tc + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t col = cc0 + tc + j;
+    if (col >= e) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t row = r0 + tr + i;
+      if (row < e) F[col * lda + row] -= acc[i][j];
+    }
+  }
+}
+
+// Power-of-two scale that puts amax in [2^14, 2^15) (1 for amax = 0 or not
+// finite), exponent clamped to [-60, 60] so that products of two scales and
+// their reciprocals stay normal FP32 numbers.
+__device__ __forceinline__ float block_scale(float amax) {
+  if (!(amax > 0.0f) || !(amax < INFINITY)) return 1.0f;
+  int e = 0;
+  (void)frexpf(amax, &e);  // amax in [2^(e-1), 2^e)
+  const int k = min(max(15 - e, -60), 60);
+  return ldexpf(1.0f, k);
+}
+
+template <typename T>
+__device__ __forceinline__ void store8(T* dst, const float* v) {
+  if constexpr (sizeof(T) == 2) {
+    union {
+      T h[8];
+      uint4 u;
+    } pk;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) pk.h[i] = static_cast<T>(v[i]);
+    *reinterpret_cast<uint4*>(dst) = pk.u;
+  } else {
+    *reinterpret_cast<float4*>(dst) = make_float4(v[0], v[1], v[2], v[3]);
+    *reinterpret_cast<float4*>(dst + 4) = make_float4(v[4], v[5], v[6], v[7]);
+  }
+}
+
+// Packs of W = kLuPanel values per row, K-contiguous, in the GEMM's element
+// type; one workgroup (256 threads) per 32 rows of the pack.  With scale
+// != nullptr each 32-row block is multiplied by block_scale(its largest
+// magnitude), which is stored in scale[block].
+//   rows of F:    pack row q = F[r0 + q, k0 .. k0 + W)   (strided in F: LDS transpose)
+//   columns of F: pack row q = F[k0 .. k0 + W, c0 + q]   (contiguous in F)
+template <typename T>
+__global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const float* __restrict__ F, int64_t r0,
+                                                           int64_t r1, int64_t k0, T* __restrict__ dst,
+                                                           float* __restrict__ scale) {
+  __shared__ float tile[32][64 + 1];
+  __shared__ float red[8];
+  const int tid = threadIdx.x, lane32 = tid & 31, grp = tid >> 5;
+  const int64_t rb = r0 + static_cast<int64_t>(blockIdx.x) * 32;
+  const int64_t r = rb + lane32;
+  const bool live = r < r1;
+  float s = 1.0f;
+  if (scale) {
+    float m = 0.0f;
+    if (live)
+      for (int k = grp; k < kLuPanel; k += 8) m = fmaxf(m, fabsf(F[(k0 + k) * lda + r]));
+    for (int o = 16; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 32));
+    if (lane32 == 0) red[grp] = m;
+    __syncthreads();
+    m = red[0];
+    for (int g = 1; g < 8; ++g) m = fmaxf(m, red[g]);
+    s = block_scale(m);
+    if (tid == 0) scale[blockIdx.x] = s;
+  }
+  const int q = tid >> 3, part = tid & 7;  // writer: pack row q, 8 values at 8 part
+  for (int kc = 0; kc < kLuPanel; kc += 64) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = grp * 8 + i;
+      tile[lane32][k] = live ? F[(k0 + kc + k) * lda + r] : 0.0f;
+    }
+    __syncthreads();
+    if (rb + q < r1) {
+      float v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = tile[q][part * 8 + i] * s;
+      store8<T>(dst + (rb - r0 + q) * kLuPanel + kc + part * 8, v);
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lu_pack_cols_kernel(int64_t lda, const float* __restrict__ F, int64_t c0,
+                                                           int64_t c1, int64_t k0, T* __restrict__ dst,
+                                                           float* __restrict__ scale) {
+  __shared__ float red[8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * 32;
+  float s = 1.0f;
+  if (scale) {
+    float m = 0.0f;
+    for (int q = wave; q < 32; q += 4) {
+      const int64_t c = cb + q;
+      if (c >= c1) break;
+      const float4* col = reinterpret_cast<const float4*>(F + c * lda + k0);
+      for (int k = lane; k < kLuPanel / 4; k += 64) {
+        const float4 v = col[k];
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) red[wave] = m;
+    __syncthreads();
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    s = block_scale(m);
+    if (tid == 0) scale[blockIdx.x] = s;
+  }
+  for (int q = wave; q < 32; q += 4) {  // a wave per column: 64 lanes x 16 values
+    const int64_t c = cb + q;
+    if (c >= c1) break;
+    const float* col = F + c * lda + k0;
+    for (int k = lane * 8; k < kLuPanel; k += 512) {
+      const float4 a = *reinterpret_cast<const float4*>(col + k);
+      const float4 b = *reinterpret_cast<const float4*>(col + k + 4);
+      const float v[8] = {a.x * s, a.y * s, a.z * s, a.w * s, b.x * s, b.y * s, b.z * s, b.w * s};
+      store8<T>(dst + (c - c0) * kLuPanel + k, v);
+    }
   }
 }
 
@@ -233,16 +369,17 @@ __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int6
 // Dinv[b][0][c][r] = (L_bb^-1)[r][c] and Dinv[b][1][c][r] = (U_bb^-1)[r][c],
 // so that lane r of a solve reads row r of the inverse coalesced.
 __global__ __launch_bounds__(64) void lu_block_inverse_kernel(int64_t n, int64_t lda, const float* __restrict__ F,
-                                                              double* __restrict__ Dinv) {
+                                                              int64_t first_block, double* __restrict__ Dinv) {
   __shared__ float blk[kLuNb][kLuNb + 1];  // [col][row]
   const int lane = threadIdx.x;
-  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kLuNb;
+  const int64_t bidx = first_block + blockIdx.x;
+  const int64_t k0 = bidx * kLuNb;
   const int kb = static_cast<int>(min(static_cast<int64_t>(kLuNb), n - k0));
 #pragma unroll 8
   for (int c = 0; c < kLuNb; ++c)
     blk[c][lane] = (c < kb && lane < kb) ? F[(k0 + c) * lda + k0 + lane] : (c == lane ? 1.0f : 0.0f);
   __syncthreads();
-  double* Li = Dinv + static_cast<int64_t>(blockIdx.x) * 2 * kLuNb * kLuNb;
+  double* Li = Dinv + bidx * 2 * kLuNb * kLuNb;
   double* Ui = Li + kLuNb * kLuNb;
   __shared__ double X[kLuNb][kLuNb];  // X[r][c]: lane c's column (one-off kernel: LDS, not registers)
   // L X = I (unit lower): x_r = delta_rc - sum_{j<r} L[r][j] x_j
@@ -375,96 +512,175 @@ __global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const f
   y[r] -= (s[0] + s[1]) + (s[2] + s[3]);
 }
 
+// The inverses of a factored W x W diagonal block D = F[K0:K0+W, K0:K0+W]:
+// the panel solves above applied to unit vectors, kInvCols of them per
+// workgroup (FP32 vectors in LDS, FP64 block-inverse products).
+//   FWD: Tfwd = the forward operator (block permutations, L_D^-1), written
+//        row-major: out[r W + k] = Tfwd[r][k] (the A operand of U12 = Tfwd A12);
+//   BWD: Uinv = U_D^-1, written by columns: out[c W + r] = Uinv[r][c] (the
+//        B operand of L21 = A21 Uinv).
+constexpr int kInvCols = 8;
+template <typename T, bool FWD>
+__global__ __launch_bounds__(256) void lu_panel_inverse_kernel(int64_t lda, const float* __restrict__ F,
+                                                               const int* __restrict__ perm,
+                                                               const double* __restrict__ Dinv, int64_t K0,
+                                                               T* __restrict__ out) {
+  constexpr int W = kLuPanel;
+  __shared__ __attribute__((aligned(16))) float ys[kInvCols][W];
+  __shared__ float tb[kInvCols][kLuNb];
+  const int tid = threadIdx.x;
+  const int j0 = blockIdx.x * kInvCols;
+  for (int q = tid; q < kInvCols * W; q += 256) {
+    const int c = q / W, r = q % W;
+    ys[c][r] = (r == j0 + c) ? 1.0f : 0.0f;
+  }
+  __syncthreads();
+  constexpr int nb = W / kLuNb;
+  for (int bi = 0; bi < nb; ++bi) {
+    const int b0 = (FWD ? bi : nb - 1 - bi) * kLuNb;
+    const int64_t k0 = K0 + b0;
+    const double* Minv = Dinv + (k0 / kLuNb) * 2 * kLuNb * kLuNb + (FWD ? 0 : kLuNb * kLuNb);
+    for (int q = tid; q < kInvCols * kLuNb; q += 256) {
+      const int c = q >> 6, i = q & 63;
+      const int src = FWD ? min(max(perm[k0 + i] - static_cast<int>(k0), 0), kLuNb - 1) : i;
+      tb[c][i] = ys[c][b0 + src];
+    }
+    __syncthreads();
+    for (int q = tid; q < kInvCols * kLuNb; q += 256) {
+      const int c = q >> 6, i = q & 63;
+      double acc = 0.0;
+#pragma unroll 8
+      for (int j = 0; j < kLuNb; ++j) acc = fma(Minv[j * kLuNb + i], static_cast<double>(tb[c][j]), acc);
+      ys[c][b0 + i] = static_cast<float>(acc);
+    }
+    __syncthreads();
+    const int ra = FWD ? b0 + kLuNb : 0, rz = FWD ? W : b0;
+    for (int r = ra + tid; r < rz; r += 256) {
+      const float* p = F + k0 * lda + K0 + r;
+      float acc[kInvCols] = {};
+#pragma unroll 4
+      for (int j = 0; j < kLuNb; ++j) {
+        const float f = p[j * lda];
+#pragma unroll
+        for (int c = 0; c < kInvCols; ++c) acc[c] = fmaf(f, ys[c][b0 + j], acc[c]);
+      }
+#pragma unroll
+      for (int c = 0; c < kInvCols; ++c) ys[c][r] -= acc[c];
+    }
+    __syncthreads();
+  }
+  if (FWD) {
+    for (int r = tid; r < W; r += 256) {
+      float v[kInvCols];
+#pragma unroll
+      for (int c = 0; c < kInvCols; ++c) v[c] = ys[c][r];
+      store8<T>(out + static_cast<int64_t>(r) * W + j0, v);
+    }
+  } else {
+    for (int q = tid; q < kInvCols * W / 8; q += 256) {
+      const int c = q / (W / 8), r = (q % (W / 8)) * 8;
+      store8<T>(out + static_cast<int64_t>(j0 + c) * W + r, &ys[c][r]);
+    }
+  }
+}
+static_assert(kInvCols == 8, "the inverse kernel stores 8 values per thread");
+
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
 
 // Blocked right-looking LU with pivoting inside the 64-wide diagonal blocks,
-// panels of kLuPanel columns, and a one-panel look-ahead: the trailing update
-// of panel p first updates the next panel's columns, then the aux stream
-// factors that panel (diagonal blocks, L21, the in-panel U12 and K = 64
-// updates: latency-bound small kernels) while the main stream updates the
-// remaining columns.  The two streams touch disjoint columns of F and
-// alternate between two pairs of pack buffers.
+// panels of W = kLuPanel columns and a one-panel look-ahead: the trailing
+// update of panel p first updates the next panel's columns, then the aux
+// stream factors that panel (its diagonal block, inverses and L21) while the
+// main stream updates the remaining columns.  The two streams touch disjoint
+// columns of F; the packs of L21 alternate between two buffers.
 template <typename T>
 static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
   const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
+  const bool scaled = mode != LuMode::kFP32;
   float* F = ws.F;
   T* Ab[2] = {static_cast<T*>(ws.At), static_cast<T*>(ws.At2)};
-  T* Bb[2] = {static_cast<T*>(ws.Bt), static_cast<T*>(ws.Bt2)};
+  float* sA[2] = {ws.sA, ws.sA2};
+  T* Bp = static_cast<T*>(ws.Bt);
+  T* X = static_cast<T*>(ws.Xm);   // A12 copy (main stream)
+  T* Y = static_cast<T*>(ws.Ya);   // A21 copy (aux stream)
+  T* Ti = static_cast<T*>(ws.Tinv);
+  T* Ui = static_cast<T*>(ws.Uinv);
   auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
   hipError_t e = hipSuccess;
   int nev = 0;
-  // the panel [K0, KE) itself, on stream st, packs into At / Bt
-  auto factor_panel = [&](int64_t K0, T* At, T* Bt, hipStream_t st) -> hipError_t {
+  // panel [K0, KE) on stream st: diagonal block, block inverses and, when
+  // columns remain, the inverses of D's factors, L21 into F and its pack
+  auto lower_part = [&](int64_t K0, int slot, hipStream_t st) -> hipError_t {
     const int64_t KE = std::min<int64_t>(n, K0 + W);
     for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
       const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
       const int64_t k1 = k0 + kb;
       hipLaunchKernelGGL(lu_diag_kernel, dim3(1), dim3(64), 0, st, lda, F, k0, kb, ws.perm, ws.info);
-      if (k1 < n)  // a full block (only the last block of the matrix is short)
-        hipLaunchKernelGGL((lu_l21_kernel<T>), dim3(grid_of(n - k1, 256)), dim3(256), 0, st, k1, n, lda, F, k0, K0,
-                           At);
-      if (k1 < KE) {
-        hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, st, k1, KE, lda, F, k0,
-                           K0, ws.perm, Bt);
-        hipError_t ee = gemm_packed_sub(mode, n - k1, KE - k1, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
-                                        Bt + (k1 - K0) * W + (k0 - K0), W, at(k1, k1), lda, st);
-        if (ee != hipSuccess) return ee;
+      if (k1 < KE) {  // a full block (only the last block of the matrix is short)
+        hipLaunchKernelGGL(lu_l21_kernel, dim3(grid_of(KE - k1, 256)), dim3(256), 0, st, k1, KE, lda, F, k0);
+        hipLaunchKernelGGL(lu_u12_kernel, dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, st, k1, KE, lda, F, k0,
+                           ws.perm);
+        hipLaunchKernelGGL(lu_dupdate_kernel, dim3(grid_of(KE - k1, kLuNb), grid_of(KE - k1, kLuNb)), dim3(256), 0,
+                           st, lda, F, k0, k1, KE);
       }
     }
+    hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(KE - K0, kLuNb)), dim3(64), 0, st, n, lda, F,
+                       K0 / kLuNb, ws.Dinv);
+    if (KE >= n) return hipGetLastError();
+    hipLaunchKernelGGL((lu_panel_inverse_kernel<T, true>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
+                       ws.Dinv, K0, Ti);
+    hipLaunchKernelGGL((lu_panel_inverse_kernel<T, false>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
+                       ws.Dinv, K0, Ui);
+    // L21 = A21 Uinv
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0, Y,
+                       nullptr);
+    hipError_t ee = gemm_packed(mode, true, n - KE, W, W, Y, W, Ui, W, at(KE, K0), lda, st);
+    if (ee != hipSuccess) return ee;
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
+                       Ab[slot], scaled ? sA[slot] : nullptr);
     return hipGetLastError();
   };
-  // C[KE:n, c0:c1] -= L[KE:n, K0:KE] U[K0:KE, c0:c1], K = KE - K0 (timed)
-  auto trailing = [&](int64_t K0, int64_t KE, int64_t c0, int64_t c1, T* At, T* Bt) -> hipError_t {
+  // the panel's rows of U for the trailing columns, U12 = Tfwd A12, and its pack
+  auto upper_part = [&](int64_t K0) -> hipError_t {
+    const int64_t KE = K0 + W;
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, s, lda, F, KE, n, K0, X,
+                       nullptr);
+    hipError_t ee = gemm_packed(mode, true, W, n - KE, W, Ti, W, X, W, at(K0, KE), lda, s);
+    if (ee != hipSuccess) return ee;
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, s, lda, F, KE, n, K0, Bp,
+                       scaled ? ws.sB : nullptr);
+    return hipGetLastError();
+  };
+  // C[KE:n, c0:c1] -= L[KE:n, K0:KE] U[K0:KE, c0:c1], K = W (timed)
+  auto trailing = [&](int64_t KE, int64_t c0, int64_t c1, int slot) -> hipError_t {
     const bool timed = nev < LuWorkspace::kEvents;
     if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-    hipError_t ee = ws.blt ? blt_gemm_sub(ws.blt, n - KE, c1 - c0, KE - K0, At + (KE - K0) * W, W,
-                                          Bt + (c0 - K0) * W, W, at(KE, c0), lda, s)
-                           : hipErrorNotSupported;
-    if (ee == hipSuccess) {
-      res->library = true;
-    } else if (ee == hipErrorNotSupported) {
-      if ((ee = gemm_packed_sub(mode, n - KE, c1 - c0, KE - K0, At + (KE - K0) * W, W, Bt + (c0 - K0) * W, W,
-                                at(KE, c0), lda, s)) != hipSuccess)
-        return ee;
-    } else {
-      return ee;
-    }
+    hipError_t ee = gemm_packed(mode, false, n - KE, c1 - c0, W, Ab[slot], W, Bp + (c0 - KE) * W, W, at(KE, c0),
+                                lda, s, scaled ? sA[slot] : nullptr, scaled ? ws.sB + (c0 - KE) / 32 : nullptr);
+    if (ee != hipSuccess) return ee;
     if (timed) {
       (void)hipEventRecord(ws.ev[2 * nev + 1], s);
       ++nev;
-      res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(c1 - c0) *
-                        static_cast<double>(KE - K0);
+      res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(c1 - c0) * static_cast<double>(W);
     }
     return hipSuccess;
   };
-  if ((e = factor_panel(0, Ab[0], Bb[0], s)) != hipSuccess) return e;
-  for (int64_t K0 = 0, p = 0; K0 < n; K0 += W, ++p) {
-    const int64_t KE = std::min<int64_t>(n, K0 + W);
-    if (KE >= n) break;
-    T* At = Ab[p & 1];
-    T* Bt = Bb[p & 1];
-    // the panel's rows of U for the trailing columns
-    for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
-      const int64_t k1 = k0 + kLuNb;
-      hipLaunchKernelGGL((lu_u12_kernel<T>), dim3(grid_of(n - KE, kLuNb)), dim3(64), 0, s, KE, n, lda, F, k0, K0,
-                         ws.perm, Bt);
-      if (k1 < KE && (e = gemm_packed_sub(mode, KE - k1, n - KE, kLuNb, At + (k1 - K0) * W + (k0 - K0), W,
-                                          Bt + (KE - K0) * W + (k0 - K0), W, at(k1, KE), lda, s)) != hipSuccess)
-        return e;
-    }
+  if ((e = lower_part(0, 0, s)) != hipSuccess) return e;
+  for (int64_t K0 = 0, p = 0; K0 + W < n; K0 += W, ++p) {
+    const int64_t KE = K0 + W;
+    if ((e = upper_part(K0)) != hipSuccess) return e;
     // trailing update: the next panel's columns, then (look-ahead) that panel
     // on the aux stream while the main stream updates the rest
     const int64_t NE = std::min<int64_t>(n, KE + W);
-    if ((e = trailing(K0, KE, KE, NE, At, Bt)) != hipSuccess) return e;
+    if ((e = trailing(KE, KE, NE, p & 1)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_cols, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(ws.aux, ws.next_cols, 0)) != hipSuccess) return e;
-    if ((e = factor_panel(KE, Ab[(p + 1) & 1], Bb[(p + 1) & 1], ws.aux)) != hipSuccess) return e;
+    if ((e = lower_part(KE, (p + 1) & 1, ws.aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
-    if (NE < n && (e = trailing(K0, KE, NE, n, At, Bt)) != hipSuccess) return e;
+    if (NE < n && (e = trailing(KE, NE, n, p & 1)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
   res->gemm_launches = nev;
-  hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(n, kLuNb)), dim3(64), 0, s, n, lda, F, ws.Dinv);
   return hipGetLastError();
 }
 
@@ -501,59 +717,52 @@ static hipError_t lu_apply_inverse(LuWorkspace& ws, double* y, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, bool library, LuWorkspace* ws) {
+hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws) {
   ws->mode = mode;
   ws->n = n;
   ws->lda = lda;
   const size_t esz = mode == LuMode::kFP32 ? 4 : 2;
   const size_t pack = esz * static_cast<size_t>(n) * kLuPanel;
+  const size_t inv = esz * static_cast<size_t>(kLuPanel) * kLuPanel;
+  const size_t scales = sizeof(float) * static_cast<size_t>((n + 31) / 32);
+  const size_t dinv = sizeof(double) * 2 * kLuNb * kLuNb * static_cast<size_t>((n + kLuNb - 1) / kLuNb);
   hipError_t e;
   if ((e = hipMalloc(&ws->F, sizeof(float) * static_cast<size_t>(lda) * n)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->perm, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->info, sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->At, pack)) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->Bt, pack)) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->At2, pack)) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->Bt2, pack)) != hipSuccess) return e;
+  for (void** p : {&ws->At, &ws->At2, &ws->Bt, &ws->Xm, &ws->Ya})
+    if ((e = hipMalloc(p, pack)) != hipSuccess) return e;
+  for (void** p : {&ws->Tinv, &ws->Uinv})
+    if ((e = hipMalloc(p, inv)) != hipSuccess) return e;
+  for (float** p : {&ws->sA, &ws->sA2, &ws->sB})
+    if ((e = hipMalloc(p, scales)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&ws->next_cols, hipEventDisableTiming)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&ws->next_done, hipEventDisableTiming)) != hipSuccess) return e;
-  const size_t dinv = sizeof(double) * 2 * kLuNb * kLuNb * static_cast<size_t>((n + kLuNb - 1) / kLuNb);
   if ((e = hipMalloc(&ws->Dinv, dinv)) != hipSuccess) return e;
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n) + 4 * pack + dinv);
-  if (library && mode != LuMode::kFP32) {
-    int64_t wb = 0;
-    if (blt_create(mode, &ws->blt, &wb) == hipSuccess)
-      ws->bytes += wb;
-    else
-      (void)hipGetLastError();  // no library state: the built-in kernel runs every update
-  }
+                                   sizeof(double) * 3 * static_cast<size_t>(n) + 5 * pack + 2 * inv + 3 * scales +
+                                   dinv);
   return hipSuccess;
 }
 
 int64_t lu_workspace_bytes(const LuWorkspace& ws) { return ws.bytes; }
 
 void lu_workspace_destroy(LuWorkspace* ws) {
-  if (ws->F) (void)hipFree(ws->F);
-  if (ws->perm) (void)hipFree(ws->perm);
-  if (ws->info) (void)hipFree(ws->info);
-  if (ws->vec) (void)hipFree(ws->vec);
   if (ws->aux) (void)hipStreamSynchronize(ws->aux);
-  if (ws->At) (void)hipFree(ws->At);
-  if (ws->Bt) (void)hipFree(ws->Bt);
-  if (ws->At2) (void)hipFree(ws->At2);
-  if (ws->Bt2) (void)hipFree(ws->Bt2);
+  for (void* p : {static_cast<void*>(ws->F), static_cast<void*>(ws->perm), static_cast<void*>(ws->info),
+                  static_cast<void*>(ws->vec), ws->At, ws->At2, ws->Bt, ws->Xm, ws->Ya, ws->Tinv, ws->Uinv,
+                  static_cast<void*>(ws->sA), static_cast<void*>(ws->sA2), static_cast<void*>(ws->sB),
+                  static_cast<void*>(ws->Dinv)})
+    if (p) (void)hipFree(p);
   if (ws->next_cols) (void)hipEventDestroy(ws->next_cols);
   if (ws->next_done) (void)hipEventDestroy(ws->next_done);
   if (ws->aux) (void)hipStreamDestroy(ws->aux);
-  if (ws->Dinv) (void)hipFree(ws->Dinv);
   for (auto& ev : ws->ev)
     if (ev) (void)hipEventDestroy(ev);
-  if (ws->blt) blt_destroy(ws->blt);
   *ws = LuWorkspace{};
 }
 

@@ -23,7 +23,7 @@ def _dptr(a):
 def make_config(params: Parameters, solver="gmres", integration="numerical", velocity_integration="follow",
                 external_force=(0.0, 0.0, -1.0), refinement_tolerance=1e-10, refinement_max_iterations=50,
                 initial_guess="previous", matrix_format="auto", velocity_operator="stored", inner_tolerance=0.0,
-                lu_gemm="library", separate_residual=False, confirm_residual=False):
+                separate_residual=False, confirm_residual=False):
     if solver not in _lib.SOLVERS:
         raise ValueError(f"unknown solver {solver!r}")
     if integration not in _lib.INTEGRATIONS:
@@ -52,7 +52,6 @@ def make_config(params: Parameters, solver="gmres", integration="numerical", vel
     cfg.matrix_format = _lib.MATRIX_FORMATS[matrix_format]
     cfg.velocity_operator = _lib.VELOCITY_OPERATORS[velocity_operator]
     cfg.inner_tolerance = inner_tolerance
-    cfg.lu_gemm = {"library": 0, "builtin": 1}[lu_gemm]
     cfg.separate_residual = 1 if separate_residual else 0
     cfg.confirm_residual = 1 if confirm_residual else 0
     return cfg

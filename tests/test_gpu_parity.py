@@ -290,34 +290,35 @@ def test_solvers_agree_with_oracle(oracle64, solver, M):
     assert rel_err(x, x_ref) < 1e-6
 
 
-@pytest.mark.parametrize("solver", ["mixed-bf16", "mixed-fp16"])
-def test_lu_trailing_updates_library_and_builtin(solver):
-    """The mixed LUs' trailing updates through hipBLASLt and through the
-    built-in MFMA kernel give the same refined solution, and the timings say
-    which one ran.  n = 4500: four panels have a trailing update, each split
+def test_lu_modes_agree_across_panels():
+    """n = 4500 (five 1024-column panels, the last one ragged): the FP32 LU
+    and the BF16 / FP16 LUs with per-block-scaled packs give the same refined
+    solution; each panel but the last has one K = 1024 trailing update, split
     in two -- the next panel's columns first, so that the next panel is
     factored meanwhile, then the rest -- except where the next panel is the
-    last one: 7 launches."""
+    last one: 7 launches.  No solve needs the GMRES fallback."""
     M = 300
     pos, ori = random_scene(M, side=5.0 * (M / 60) ** (1 / 3), seed=8, min_dist=0.4)
     params = ref_config()
     params.NUMBER_OF_FIBERS = M
     out = {}
-    for lu_gemm in ("library", "builtin"):
-        with _sim(params, pos, ori, solver=solver, lu_gemm=lu_gemm, refinement_tolerance=1e-11) as sim:
+    for solver in ("direct", "mixed-bf16", "mixed-fp16"):
+        with _sim(params, pos, ori, solver=solver, refinement_tolerance=1e-11) as sim:
             sim.step(0)
             _, _, x = sim.system()
             st, tm = sim.solver_stats(), sim.timings()
+        print(solver, st, tm["gemm_launches"], tm["direct_fallback"])
         assert st["converged"] and st["relative_residual"] <= 1e-11
-        out[lu_gemm] = (x, tm["gemm_library"], tm["gemm_launches"])
-    assert out["library"][1] == 1 and out["builtin"][1] == 0
+        assert tm["direct_fallback"] == 0
+        out[solver] = (x, tm["gemm_launches"])
     n, W = 15 * M, 1024
     launches = sum(1 + (K0 + 2 * W < n) for K0 in range(0, n, W) if K0 + W < n)
     assert launches == 7
-    assert out["library"][2] == out["builtin"][2] == launches
-    e = rel_err(out["library"][0], out["builtin"][0])
-    print(f"{solver}: library vs built-in trailing updates, x {e:.1e}")
-    assert e < 1e-9
+    for solver, (x, nl) in out.items():
+        assert nl == launches
+        e = rel_err(x, out["direct"][0])
+        print(f"{solver} against the FP32 LU: x {e:.1e}")
+        assert e < 1e-9
 
 
 @pytest.mark.parametrize("matrix_format", ["theta", "dense"])
@@ -670,3 +671,61 @@ def test_gmres_restart_extremes(restart):
     assert st["converged"] and st["relative_residual"] <= 1e-10
     assert mv == its + (its - 1) // restart + 1  # steps, restarts, confirmation
     assert rel_err(x, x_ref) < 1e-8
+
+
+def _near_coincident_scene(d):
+    """64 fibers (n = 960, fifteen 64-row pivot blocks) with fiber 40 moved to
+    a distance d of fiber 0, parallel to it: rows 0-14 and 600-614 of A,
+    which lie in different pivot blocks, couple with entries ~ 1/d.  Partial
+    pivoting over the whole column (sgesv, fortran/solve_system.incl:3) would
+    exchange rows across blocks; the pivots here stay within each block, so
+    the factors carry an element growth ~ 1/d that the refinement, or the
+    GMRES fallback, has to overcome."""
+    M = 64
+    pos, ori = random_scene(M, side=5.0, seed=3, min_dist=0.4)
+    pos, ori = pos.reshape(M, 3).copy(), ori.reshape(M, 3).copy()
+    perp = np.cross(ori[0], [1.0, 0.0, 0.0])
+    perp /= np.linalg.norm(perp)
+    pos[40] = pos[0] + d * perp
+    ori[40] = ori[0]
+    return M, pos.ravel(), ori.ravel()
+
+
+@pytest.mark.parametrize("solver", ["direct", "mixed-bf16", "mixed-fp16"])
+@pytest.mark.parametrize("d,max_sweeps", [(1e-3, 50), (1e-4, 50), (1e-4, 1)])
+def test_direct_solve_without_cross_block_pivoting(oracle64, solver, d, max_sweeps):
+    """Systems that need cross-block row exchanges.  d = 1e-3 (entries up to
+    5e3, condition number 1.5e4): the in-block LU's element growth is
+    overcome by the FP64 refinement and x matches the oracle's LU solution
+    with full partial pivoting.  d = 1e-4 (entries up to 5e6, condition
+    number 1.5e7, past what an FP32-stored system resolves): the step either
+    converges or reports RF_WARN_NOT_CONVERGED, and the residual it reports is
+    the true one of the returned x in the system it solved -- never a
+    silently wrong x; capped at one refinement sweep the solve hands over to
+    GMRES on the dense system (rf_timings.direct_fallback)."""
+    from oracle.pyoracle import make_params
+    from rigidfibers_amd import _lib
+
+    M, pos, ori = _near_coincident_scene(d)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.GMRES_MAX_ITERATIONS = 2000
+    A_ref, b_ref = oracle64.assemble(make_params(M), pos, ori, False)
+    x_ref = oracle64.solve_direct(A_ref, b_ref)
+    with _sim(params, pos, ori, solver=solver, refinement_tolerance=1e-10,
+              refinement_max_iterations=max_sweeps) as sim:
+        code = sim.step(0)
+        A, b, x = sim.system(with_matrix=True)
+        st, tm = sim.solver_stats(), sim.timings()
+    e = rel_err(x, x_ref)
+    res = np.linalg.norm(b - A.astype(np.float64) @ x) / np.linalg.norm(b)
+    print(f"{solver} d={d:g} max sweeps {max_sweeps}: status {code}, {st}, fallback {tm['direct_fallback']}, "
+          f"residual of x in the solved system {res:.1e}, x against the oracle {e:.1e}")
+    if max_sweeps == 1:
+        assert tm["direct_fallback"] == 1
+    if d == 1e-3:
+        assert code == _lib.RF_OK and st["converged"] and tm["direct_fallback"] == 0 and e <= 1e-6
+    if st["converged"]:
+        assert code == _lib.RF_OK and st["relative_residual"] <= 1e-10 and res <= 1e-9
+    else:
+        assert code == _lib.RF_WARN_NOT_CONVERGED and abs(res - st["relative_residual"]) <= 1e-3 * res
