@@ -214,6 +214,17 @@ RF_API rf_status rf_set_state(rf_simulation* sim, const double* positions, const
  * (dumpSolutionSystem, :747-768).  a_matrix: n*n floats column-major, or NULL;
  * b_vector, x_vector: n doubles each, or NULL. */
 RF_API rf_status rf_get_system(rf_simulation* sim, float* a_matrix, double* b_vector, double* x_vector);
+/* rf_get_system without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Where
+ * rf_get_system expands a theta-form matrix through a temporary n*n device
+ * buffer, this one expands it a range of source fibers at a time through a
+ * 16 MiB staging buffer that the library holds in static device memory (one
+ * per GPU, kept until the process ends); RF_ERR_UNSUPPORTED when one source
+ * fiber's 3N columns exceed it (3N*n*4 bytes > 16 MiB, n > 279 620 for
+ * N = 5).  Calls that overlap on one GPU share that buffer and race.  Same
+ * arguments and results as rf_get_system. */
+RF_API rf_status rf_get_system_noalloc(rf_simulation* sim, float* a_matrix, double* b_vector, double* x_vector);
 /* Number of unknowns n = 3*N*M. */
 RF_API int64_t rf_system_size(const rf_simulation* sim);
 

@@ -31,7 +31,7 @@ VELOCITY_OPERATORS = {"stored": 0, "pairwise": 1}
 EXPORTS = (
     "rf_config_init", "rf_create", "rf_destroy", "rf_step", "rf_assemble_system", "rf_solve_system",
     "rf_update_velocities", "rf_update_fibers", "rf_advance", "rf_get_state", "rf_get_next_state",
-    "rf_get_velocities", "rf_set_state", "rf_get_system", "rf_system_size", "rf_get_solver_stats",
+    "rf_get_velocities", "rf_set_state", "rf_get_system", "rf_get_system_noalloc", "rf_system_size", "rf_get_solver_stats",
     "rf_get_timings", "rf_device_bytes", "rf_last_error", "rf_version",
     "rf_server_create", "rf_server_destroy", "rf_server_submit", "rf_server_wait", "rf_server_collect",
     "rf_server_get_stats", "rf_server_clock",
@@ -157,6 +157,7 @@ def load(path: str = LIB_PATH):
         "rf_get_velocities": ([sim, dp, dp], C.c_int),
         "rf_set_state": ([sim, dp, dp, C.c_int32, dp, dp, dp, dp], C.c_int),
         "rf_get_system": ([sim, C.POINTER(C.c_float), dp, dp], C.c_int),
+        "rf_get_system_noalloc": ([sim, C.POINTER(C.c_float), dp, dp], C.c_int),
         "rf_system_size": ([sim], C.c_int64),
         "rf_get_solver_stats": ([sim, C.POINTER(RfSolverStats)], C.c_int),
         "rf_get_timings": ([sim, C.POINTER(RfTimings)], C.c_int),

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -588,6 +589,58 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+// Staging buffer of rf_get_system_noalloc, which may not allocate device
+// memory: 16 MiB of file-scope device memory that each GPU the library runs
+// on holds from the library's load until the process ends.  Host code uses
+// it only through the address hipGetSymbolAddress returns for the current
+// device (the array's own name is a host-side stand-in).  Every call on a
+// GPU shares that GPU's copy, so calls that overlap on one GPU race.
+constexpr size_t kScratchFloats = size_t{4} << 20;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current; looked up
+// once per device.
+rf_status scratch_address(int device, float** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(RF_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(RF_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return RF_OK;
+}
+
+// Makes a device current for a scope and restores the caller's afterwards.
+struct DeviceScope {
+  int saved = -1;
+  bool switched = false;
+  rf_status enter(int device) {
+    RF_HIP(hipGetDevice(&saved));
+    if (saved != device) {
+      RF_HIP(hipSetDevice(device));
+      switched = true;
+    }
+    return RF_OK;
+  }
+  ~DeviceScope() {
+    if (switched) (void)hipSetDevice(saved);
+  }
+};
+
+}  // namespace
+
 namespace rf {
 rf_status set_error(rf_status code, const std::string& msg) { return fail(code, msg); }
 rf_status check_config(const rf_config* c) { return validate_config(c); }
@@ -1001,6 +1054,35 @@ rf_status rf_get_system(rf_simulation* s, float* a_matrix, double* b_vector, dou
   if (x_vector) RF_HIP(hipMemcpyAsync(x_vector, s->x, n * sizeof(double), hipMemcpyDeviceToHost, s->stream));
   RF_HIP(hipStreamSynchronize(s->stream));
   return RF_OK;
+}
+
+rf_status rf_get_system_noalloc(rf_simulation* s, float* a_matrix, double* b_vector, double* x_vector) {
+  if (!s) return fail(RF_ERR_INVALID_ARGUMENT, "simulation is NULL");
+  if (!a_matrix || !s->theta) return rf_get_system(s, a_matrix, b_vector, x_vector);  // no staging needed
+  if (!s->tori_valid)
+    return fail(RF_ERR_INVALID_ARGUMENT, "no theta-form system to expand (none assembled, or the state it was "
+                                         "assembled from has been overwritten)");
+  const int64_t fiber_cols = 3LL * s->N * s->n;  // floats of one source fiber's columns
+  if (fiber_cols > static_cast<int64_t>(kScratchFloats))
+    return fail(RF_ERR_UNSUPPORTED, "rf_get_system_noalloc: the 3N columns of one source fiber (" +
+                                        std::to_string(fiber_cols * 4) + " bytes) exceed its " +
+                                        std::to_string(kScratchFloats * 4) + "-byte staging buffer");
+  DeviceScope scope;
+  RF_TRY(scope.enter(s->device));
+  float* stage = nullptr;
+  RF_TRY(scratch_address(s->device, &stage));
+  const int chunk = static_cast<int>(std::min<int64_t>(s->M, static_cast<int64_t>(kScratchFloats) / fiber_cols));
+  // Stream order keeps each expansion behind the copy of the chunk before it;
+  // the expansion writes every element of a chunk, so nothing is cleared.
+  for (int j0 = 0; j0 < s->M; j0 += chunk) {
+    const int jc = std::min(chunk, s->M - j0);
+    RF_HIP(rf::launch_expand_theta(s->N, s->M, s->Lp, s->T, s->ori[s->tori], s->pc, stage, s->stream, j0, jc));
+    RF_HIP(hipMemcpyAsync(a_matrix + static_cast<int64_t>(j0) * fiber_cols, stage,
+                          sizeof(float) * static_cast<size_t>(jc) * static_cast<size_t>(fiber_cols),
+                          hipMemcpyDeviceToHost, s->stream));
+  }
+  RF_HIP(hipStreamSynchronize(s->stream));
+  return rf_get_system(s, nullptr, b_vector, x_vector);
 }
 
 int64_t rf_system_size(const rf_simulation* s) { return s ? s->n : 0; }
