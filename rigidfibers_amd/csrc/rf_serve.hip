@@ -422,9 +422,13 @@ bool solve_lockstep(rf_server* S) {
                                      st) != hipSuccess)
     return false;
   if (!S->cfg.eager && !ensure_iteration_graph(S)) return false;
-  // every suspension stops by itself after at most maxit Arnoldi steps and
-  // one true residual per cycle; the bound only guards the host loop
-  const int64_t max_rounds = (static_cast<int64_t>(maxit) + maxit / std::max(m, 1) + 4) / kRound + 2;
+  // Every suspension stops by itself: at most maxit Arnoldi steps, and a true
+  // residual after each restart cycle, which holds at least one Arnoldi step
+  // (cycles can end early, on a zero ||w|| or, with confirm_residual, on an
+  // estimate the true residual does not confirm), so at most 2 maxit + 1
+  // iterations, one to apply the last update and the start step.  The bound
+  // only guards the host loop.
+  const int64_t max_rounds = (2 * static_cast<int64_t>(maxit) + 4) / kRound + 2;
   bool any = true;
   for (int64_t r = 0; any; ++r) {
     if (r >= max_rounds) return false;

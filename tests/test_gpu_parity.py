@@ -696,8 +696,10 @@ def _near_coincident_scene(d):
 def test_direct_solve_without_cross_block_pivoting(oracle64, solver, d, max_sweeps):
     """Systems that need cross-block row exchanges.  d = 1e-3 (entries up to
     5e3, condition number 1.5e4): the in-block LU's element growth is
-    overcome by the FP64 refinement and x matches the oracle's LU solution
-    with full partial pivoting.  d = 1e-4 (entries up to 5e6, condition
+    overcome by the FP64 refinement: x is the solution of the FP32 system it
+    solved (against an LU with full partial pivoting of that system, in
+    FP64) and matches the oracle's to what the FP32 rounding of the assembly
+    leaves at that condition number.  d = 1e-4 (entries up to 5e6, condition
     number 1.5e7, past what an FP32-stored system resolves): the step either
     converges or reports RF_WARN_NOT_CONVERGED, and the residual it reports is
     the true one of the returned x in the system it solved -- never a
@@ -718,13 +720,17 @@ def test_direct_solve_without_cross_block_pivoting(oracle64, solver, d, max_swee
         A, b, x = sim.system(with_matrix=True)
         st, tm = sim.solver_stats(), sim.timings()
     e = rel_err(x, x_ref)
-    res = np.linalg.norm(b - A.astype(np.float64) @ x) / np.linalg.norm(b)
+    A64 = A.astype(np.float64)
+    res = np.linalg.norm(b - A64 @ x) / np.linalg.norm(b)
+    e_sys = rel_err(x, np.linalg.solve(A64, b))
     print(f"{solver} d={d:g} max sweeps {max_sweeps}: status {code}, {st}, fallback {tm['direct_fallback']}, "
-          f"residual of x in the solved system {res:.1e}, x against the oracle {e:.1e}")
+          f"residual of x in the solved system {res:.1e}, x against its solution {e_sys:.1e}, "
+          f"against the oracle {e:.1e}")
     if max_sweeps == 1:
         assert tm["direct_fallback"] == 1
     if d == 1e-3:
-        assert code == _lib.RF_OK and st["converged"] and tm["direct_fallback"] == 0 and e <= 1e-6
+        assert code == _lib.RF_OK and st["converged"] and tm["direct_fallback"] == 0
+        assert e_sys <= 1e-8 and e <= 1e-4
     if st["converged"]:
         assert code == _lib.RF_OK and st["relative_residual"] <= 1e-10 and res <= 1e-9
     else:

@@ -208,3 +208,25 @@ def test_full_request_stream():
             assert np.array_equal(alone["positions"], res[k]["positions"])
             assert np.array_equal(alone["orientations"], res[k]["orientations"])
             assert alone["info"]["gmres_iterations"] == res[k]["gmres_iterations"]
+
+
+def test_iteration_cap_with_confirmation_fails_no_neighbour():
+    """GMRES(2) capped at 3 Arnoldi steps per solve, confirming each cycle's
+    estimate with the true residual, against an unreachable tolerance: every
+    suspension ends each step unconverged (status RF_WARN_NOT_CONVERGED, not
+    a device error), its result is the same alone and in the mix, and the
+    capped steps count."""
+    from rigidfibers_amd import _lib
+    from rigidfibers_amd.serve import Server
+
+    params = _params(tol=1e-14)
+    params.GMRES_RESTART = 2
+    params.GMRES_MAX_ITERATIONS = 3
+    reqs = _requests([23, 64, 17], steps=3, seed=13)
+    with Server(params, max_active=4, confirm_residual=True) as srv:
+        alone = [_serve(srv, [r])[0] for r in reqs]
+        mixed = _serve(srv, reqs)
+    for a, b in zip(alone, mixed):
+        assert a["info"]["status"] == b["info"]["status"] == _lib.RF_WARN_NOT_CONVERGED
+        assert a["info"]["gmres_iterations"] == b["info"]["gmres_iterations"] == 3 * 3
+        assert np.array_equal(a["positions"], b["positions"]) and np.array_equal(a["orientations"], b["orientations"])
