@@ -15,13 +15,18 @@ synchronisations.  value = M * K / elapsed.
 
 Also reported:
   roofline      the step's dominant kernel, the assembly (VALU-bound): its
-                algorithmic flops per launch, M (M - 1) ordered fiber pairs x
-                32 500 flop (analytical: 14 976 of them FP64 -- 24 nodes x 624
-                for the moment recurrences -- and 17 524 FP32) or x 63 216
-                FP32 flop (numerical), / the mean duration of the assembly
-                kernel (HIP events around it on the library's stream), against
-                the peak for that FP64/FP32 mix at MI355X's published 78.6 /
-                157.3 TFLOP/s (the time the flops take at those rates);
+                algorithmic flops per launch / the mean duration of the
+                assembly kernel (HIP events around it on the library's
+                stream), against the peak for its FP64/FP32 mix at MI355X's
+                published 78.6 / 157.3 TFLOP/s (the time the flops take at
+                those rates).  Flops: numerical, 63 216 FP32 per ordered fiber
+                pair; analytical, per (pair, target node): 624 FP64 (moment
+                recurrences) + 400 FP32 (recombination) where the node lies
+                within the far-field radius of the source's centre, else 576
+                FP32 (8-point far-field rule: 8 kernel evaluations of 30 flop,
+                6 mirror sums and 36 FMAs per node pair), plus 7 920 FP32 per
+                pair for the projections; near and far nodes are counted on
+                the initial scene;
   roofline_solver  iterative solvers: the solver's matvec (HBM-bound),
                 algorithmic bytes per launch (theta form: M * M * N * (3 +
                 6(N-1)) * 4, the FP32 numbers that define A; dense: n^2 * 4) /
@@ -60,6 +65,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# CPUs this process may run on, read before the oracle's OpenMP runtime binds
+# the main thread to one place
+CPUS_AVAILABLE = len(os.sched_getaffinity(0))
 MFMA_PEAK_TFLOPS = {"direct": 157.3, "mixed-bf16": 2500.0, "mixed-fp16": 2500.0}
 GEMM_KERNEL = {"direct": "gemm_packed_sub_kernel<float>", "mixed-bf16": "gemm_packed_sub_kernel<__bf16>",
                "mixed-fp16": "gemm_packed_sub_kernel<_Float16>"}
@@ -123,6 +131,37 @@ def run_gpu(args, pos, ori, solver, steps, warmup, tag="", capture_last=False):
             "last": last}
 
 
+def far_field_radius(N=5):
+    """kFarField[N] of the library's device header (rf_device.h)."""
+    import re
+
+    with open(os.path.join(ROOT, "rigidfibers_amd", "csrc", "rf_device.h")) as fh:
+        src = fh.read()
+    vals = re.search(r"kFarField\[kMaxTerms \+ 1\] = \{([^}]*)\}", src).group(1).split(",")
+    return float(vals[N])
+
+
+def assembly_flops(integration, pos, ori, N=5, nqi=8):
+    """(FP64, FP32) algorithmic flops of one assembly of this scene and the
+    number of near (pair, node) combinations (see the module docstring)."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+
+    M = pos.size // 3
+    pairs = float(M) * (M - 1)
+    if integration == "numerical":
+        return 0.0, pairs * 63_216.0, 0
+    g = np.sqrt(15.0) / 5.0
+    h = 2.0 / nqi
+    s = np.array([(2.0 * (-1.0 + iv * h) + h + gp * h) / 2.0 for iv in range(nqi) for gp in (-g, 0.0, g)])
+    P, T = pos.reshape(M, 3), ori.reshape(M, 3)
+    X = (P[:, None, :] + s[None, :, None] * T[:, None, :]).reshape(-1, 3)
+    # source centres within the radius of each target node, less the node's own fiber
+    near = int(cKDTree(P).query_ball_point(X, r=far_field_radius(N), return_length=True).sum()) - M * s.size
+    far = pairs * s.size - near
+    return 624.0 * near, 400.0 * near + 576.0 * far + 7_920.0 * pairs, near
+
+
 def step_bound(args, g):
     """The step at its bounds: the assembly's algorithmic flops at the
     published FP64 / FP32 peaks, every matvec of the step and the stored
@@ -130,8 +169,8 @@ def step_bound(args, g):
     if not (args.solver == "gmres" and g["theta"]):
         return None
     M, N, K = args.fibers, 5, g["steps"]
-    f64, f32 = ASM_FLOP_PER_PAIR[args.integration]
-    asm_ms = 1e3 * M * (M - 1) * (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12))
+    f64, f32, _ = g["asm_flops"]
+    asm_ms = 1e3 * (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12))
     mv_ms = 1e3 * M * (3 + 6 * (N - 1)) * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
     vel_ms = 1e3 * M * 12 * M * N * 4.0 / (HBM_PEAK_GBS * 1e9)
     matvecs = g["acc"]["gemv_launches"] / K
@@ -154,12 +193,8 @@ def measured_traffic(kernel, M, N):
     return None if rec is None else float(rec["fetch_bytes"] + rec["write_bytes"])
 
 
-# Algorithmic flops of the assembly per ordered fiber pair (N = 5, 8 x 3
-# nodes): analytical 24 nodes x 624 FP64 (moment recurrences) + 17 524 FP32
-# (recombination and projections); numerical 63 216 FP32 (576 kernel
-# evaluations and the projections).  Peaks: MI355X's published FP64 and FP32
-# rates (the same for vector and matrix instructions).
-ASM_FLOP_PER_PAIR = {"analytical": (14_976, 17_524), "numerical": (0, 63_216)}
+# MI355X's published FP64 and FP32 rates (the same for vector and matrix
+# instructions).
 FP64_PEAK_TFLOPS, FP32_PEAK_TFLOPS = 78.6, 157.3
 
 
@@ -167,10 +202,10 @@ def roofline_assembly(args, g):
     """The assembly kernel against the time its algorithmic flops take at the
     published FP64 / FP32 peaks."""
     M, K = args.fibers, g["steps"]
-    f64, f32 = ASM_FLOP_PER_PAIR[args.integration]
+    f64, f32, near = g["asm_flops"]
     pairs = float(M) * (M - 1)
-    flop = pairs * (f64 + f32)
-    peak_s = pairs * (f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12))
+    flop = f64 + f32
+    peak_s = f64 / (FP64_PEAK_TFLOPS * 1e12) + f32 / (FP32_PEAK_TFLOPS * 1e12)
     avg_ms = g["acc"]["assemble_kernel"] / K
     achieved = flop / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     peak = flop / peak_s / 1e12
@@ -179,7 +214,8 @@ def roofline_assembly(args, g):
     return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 2), "peak": round(peak, 2),
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4) if peak > 0 else 0.0,
             "traffic": measured_traffic(kernel.split("<")[0], M, 5) if theta else None,
-            "flop_per_launch": flop, "fp64_flop_per_launch": pairs * f64, "fp32_flop_per_launch": pairs * f32,
+            "flop_per_launch": flop, "fp64_flop_per_launch": f64, "fp32_flop_per_launch": f32,
+            "near_pair_nodes": near, "pair_nodes": pairs * 24,
             "time_at_peak_ms": round(1e3 * peak_s, 3), "avg_launch_ms": round(avg_ms, 4)}
 
 
@@ -391,7 +427,17 @@ def large_suspension(args):
 
 def cpu_threads():
     """OMP_NUM_THREADS, else every CPU this process may run on."""
-    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or CPUS_AVAILABLE
+
+
+def cpu_quota():
+    """CPUs of the cgroup's quota (cpu.max), or None."""
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, p = fh.read().split()[:2]
+        return None if q == "max" else float(q) / float(p)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_model():
@@ -439,9 +485,11 @@ def cpu_baseline(args, pos, ori, iters_per_step):
     return {
         "value": M / step_s,
         "unit": "fiber-steps/s",
-        "cores": threads,
+        "cores": min([threads, CPUS_AVAILABLE] + ([cpu_quota()] if cpu_quota() else [])),
+        "threads": threads,
         "cpu_model": cpu_model(),
-        "cpus_available": len(os.sched_getaffinity(0)),
+        "cpus_available": CPUS_AVAILABLE,
+        "cpu_quota": cpu_quota(),
         "kind": "port",
         "sample": (f"FP32 OpenMP port of the reference Fortran (oracle/librf_oracle32.so), {threads} threads: "
                    f"assembly of the rows of {S} of {M} target fibers ({t_asm:.2f} s), their velocities "
@@ -498,6 +546,7 @@ def main():
     os.environ.setdefault("OMP_PLACES", "cores")
     pos, ori = build_scene(args.fibers, args.radius, args.seed)
     g = run_gpu(args, pos, ori, args.solver, args.steps, args.warmup)
+    g["asm_flops"] = assembly_flops(args.integration, pos, ori)
     M, K = args.fibers, args.steps
     value = M * K / g["elapsed"]
     fmt = ("theta" if args.matrix_format in ("auto", "theta") else "dense") if args.solver in ITERATIVE else "dense"

@@ -77,12 +77,34 @@ __device__ __forceinline__ void inner_integrals(const double xi[3], const double
     }
   } else {
     const double R0[3] = {xi[0] - xj0 + sq_d * tid_[0], xi[1] - xj1 + sq_d * tid_[1], xi[2] - xj2 + sq_d * tid_[2]};
-    const double tjd[3] = {tj0, tj1, tj2};
-    const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
     const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
-    double I1[N + 3], I3[N + 3], I5[N + 3];
-    moments<N>(bb, ll, I1, I3, I5);
-    analytic_modes<N, NQI, N>(node_products(R0, tjd), pc.eps2_d, I1, I3, I5, acc);
+    if (ll >= kFarField[N] * kFarField[N]) {
+      // far field: 8-point Gauss-Legendre over the source fiber (FP32)
+      constexpr FarQuad<N> far{};
+      const float rx = static_cast<float>(R0[0]), ry = static_cast<float>(R0[1]), rz = static_cast<float>(R0[2]);
+      const float tx = static_cast<float>(tj0), ty = static_cast<float>(tj1), tz = static_cast<float>(tj2);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const float s = far.s[h];
+        const float ux = s * tx, uy = s * ty, uz = s * tz;
+        float Kp[6], Km[6];
+        stokes_kernel(rx - ux, ry - uy, rz - uz, pc.eps2, Kp);  // node +s
+        stokes_kernel(rx + ux, ry + uy, rz + uz, pc.eps2, Km);  // node -s
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+          const float E = Kp[c] + Km[c];
+          const float O = Kp[c] - Km[c];
+#pragma unroll
+          for (int m = 0; m < NM; ++m) acc[m][c] = fmaf(far.W[m][h], (m & 1) ? O : E, acc[m][c]);
+        }
+      }
+    } else {
+      const double tjd[3] = {tj0, tj1, tj2};
+      const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
+      double I1[N + 3], I3[N + 3], I5[N + 3];
+      moments<N>(bb, ll, I1, I3, I5);
+      analytic_modes<N, NQI, N>(node_products(R0, tjd), pc.eps2_d, I1, I3, I5, acc);
+    }
   }
 }
 

@@ -77,6 +77,31 @@ struct Quad {
   }
 };
 
+// Far field of the analytical inner integral.  When the target node lies at
+// least kFarField[N] from the source fiber's centre (|R0| >= kFarField[N]),
+// the integrand P_m(s) K(R0 - s t) is analytic on a Bernstein ellipse around
+// [-1, 1] far enough out that the 8-point Gauss-Legendre rule integrates it,
+// for every mode m <= N, to < 1e-8 of the largest G_0 component (worst
+// direction at the threshold; the FP32 rounding of the kernel values, ~1e-7,
+// dominates).  Higher modes need a larger radius.  The rule's nodes come in
+// mirror pairs +-s_k, so even modes take K(+s) + K(-s), odd modes K(+s) - K(-s).
+constexpr double kFarField[kMaxTerms + 1] = {0.0, 3.0, 3.0, 3.0, 3.0, 3.0, 3.5, 4.5, 5.0};
+constexpr double kGL8Node[4] = {0.18343464249564978, 0.52553240991632899, 0.79666647741362674,
+                                0.96028985649753623};
+constexpr double kGL8Weight[4] = {0.36268378337836177, 0.31370664587788705, 0.22238103445337434,
+                                  0.10122853629037669};
+template <int N>
+struct FarQuad {
+  float s[4];          // positive nodes
+  float W[N + 1][4];   // w_k P_m(s_k), m = 0..N
+  constexpr FarQuad() : s(), W() {
+    for (int k = 0; k < 4; ++k) {
+      s[k] = static_cast<float>(kGL8Node[k]);
+      for (int m = 0; m <= N; ++m) W[m][k] = static_cast<float>(kGL8Weight[k] * legendre_value(m, kGL8Node[k]));
+    }
+  }
+};
+
 // Node s_q in FP64 and weight w_q in FP32 for a runtime node index (the
 // per-lane target node), computed exactly as Quad<> does.
 template <int NQI>

@@ -121,13 +121,48 @@ __global__ __launch_bounds__(kPairThreads) void velocity_kernel(int M, const dou
     } else {
       const double R0[3] = {xi[0] - xj0 + sq_d * tid_[0], xi[1] - xj1 + sq_d * tid_[1],
                             xi[2] - xj2 + sq_d * tid_[2]};
+      const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
+      const double* C = fm + static_cast<int64_t>(j) * (N + 1) * 3;
+      if (ll >= kFarField[N] * kFarField[N]) {
+        // far field: the assembly's 8-point Gauss-Legendre rule over the
+        // source fiber, f_j(+-s_k) from its monomial coefficients
+        constexpr FarQuad<N> far{};
+        const float rx = static_cast<float>(R0[0]), ry = static_cast<float>(R0[1]), rz = static_cast<float>(R0[2]);
+        const float tx = static_cast<float>(tj0), ty = static_cast<float>(tj1), tz = static_cast<float>(tj2);
+        double acc[3] = {0, 0, 0};
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+          const double sk = kGL8Node[h], wk = kGL8Weight[h];
+          double fp[3] = {0, 0, 0}, fn[3] = {0, 0, 0}, pw = 1.0;
+#pragma unroll
+          for (int q = 0; q <= N; ++q) {
+            const double sgn = (q & 1) ? -pw : pw;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              fp[a] = fma(C[3 * q + a], pw, fp[a]);
+              fn[a] = fma(C[3 * q + a], sgn, fn[a]);
+            }
+            pw *= sk;
+          }
+          const float s = far.s[h];
+          const float ux = s * tx, uy = s * ty, uz = s * tz;
+          float Kp[6], Km[6];
+          stokes_kernel(rx - ux, ry - uy, rz - uz, pc.eps2, Kp);  // node +s
+          stokes_kernel(rx + ux, ry + uy, rz + uz, pc.eps2, Km);  // node -s
+          acc[0] += wk * (Kp[0] * fp[0] + Kp[3] * fp[1] + Kp[4] * fp[2] + Km[0] * fn[0] + Km[3] * fn[1] + Km[4] * fn[2]);
+          acc[1] += wk * (Kp[3] * fp[0] + Kp[1] * fp[1] + Kp[5] * fp[2] + Km[3] * fn[0] + Km[1] * fn[1] + Km[5] * fn[2]);
+          acc[2] += wk * (Kp[4] * fp[0] + Kp[5] * fp[1] + Kp[2] * fp[2] + Km[4] * fn[0] + Km[5] * fn[1] + Km[2] * fn[2]);
+        }
+        g0 += acc[0];
+        g1 += acc[1];
+        g2 += acc[2];
+        continue;
+      }
       const double t[3] = {tj0, tj1, tj2};
       const double bb = -2.0 * (R0[0] * tj0 + R0[1] * tj1 + R0[2] * tj2);
-      const double ll = R0[0] * R0[0] + R0[1] * R0[1] + R0[2] * R0[2];
       double I1[N + 3], I3[N + 3], I5[N + 3];
       moments<N>(bb, ll, I1, I3, I5);
       const double two_e2 = 2.0 * pc.eps2_d;
-      const double* C = fm + static_cast<int64_t>(j) * (N + 1) * 3;
       double acc[3] = {0, 0, 0};
 #pragma unroll
       for (int q = 0; q <= N; ++q) {
