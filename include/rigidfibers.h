@@ -165,6 +165,9 @@ typedef struct rf_timings {
   int32_t direct_fallback; /* direct solvers: 1 when the refinement did not converge and GMRES on the dense
                               system finished the solve (iterations then count both) */
   double assemble_kernel;  /* the assembly kernel alone (assemble_system less its partial-sum kernels) */
+  int32_t solver_restarts; /* BiCGStab: restarts from the true residual in the last solve (a degenerate search
+                              direction, or every 200 iterations, as lib/viennacl/linalg/bicgstab.hpp:150-152) */
+  int32_t reserved1;
 } rf_timings;
 
 typedef struct rf_simulation rf_simulation;

@@ -88,6 +88,8 @@ class RfTimings(C.Structure):
         ("gemv_half_launches", C.c_int32),
         ("direct_fallback", C.c_int32),
         ("assemble_kernel", C.c_double),
+        ("solver_restarts", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 

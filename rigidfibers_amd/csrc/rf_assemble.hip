@@ -318,12 +318,14 @@ typedef float floatx2 __attribute__((ext_vector_type(2)));
 // apart[gy n + row] (the theta GEMV's arithmetic: FP32 values times FP64
 // x0, FP64 sums).  The read is HBM traffic while the other workgroups of the
 // CU keep its VALUs busy, which is what makes it cheaper than a matvec.
-template <int N, int NQI, bool ANALYTIC>
+template <int N, int NQI, bool ANALYTIC, bool HALF = false>
 __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64_t Lp, const double* __restrict__ pos,
                                                      const double* __restrict__ ori, float* __restrict__ T,
                                                      double* __restrict__ bpart, int jgroup, const PhysConsts& pc,
                                                      float* __restrict__ Tv, double* __restrict__ cpart,
                                                      const double* __restrict__ x0, double* __restrict__ apart,
+                                                     _Float16* __restrict__ T16 = nullptr,
+                                                     float* __restrict__ S16 = nullptr,
                                                      const SusDev* __restrict__ serving = nullptr) {
   constexpr Quad<N, NQI> quad{};
   constexpr int Q = 3 * NQI;
@@ -342,6 +344,10 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ __align__(16) float Pj[NM * RP];
   __shared__ double bred[PAIRS][NR];
   __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
+  // FP16 copy: largest |value| of each source's block (float bits), in 8
+  // replicas per source to spread the atomics, and its reciprocal scale
+  __shared__ unsigned smax[PAIRS][8];
+  __shared__ float sinv[PAIRS];
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -361,6 +367,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   // RHS (and velocity-constant) sums of each source slot, kept in LDS; the
   // mode factors gamma_k, eigen_k as a table the RHS lanes index by k
   for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
+  if (tid < PAIRS * 8) smax[tid / 8][tid % 8] = 0u;
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t)
     if (tid == t) {
@@ -409,7 +416,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     // sweep's first column of component 0): self-pair columns zeroed, columns
     // past the sweep's last source not written.  The conditions are uniform
     // over the workgroup except in the sweep that holds them.
-    auto put = [&](float* base, uint32_t off, int pm0, floatx4 val) {
+    auto put = [&](float* base, uint32_t off, int pm0, floatx4& val) {
       if (self_here) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
@@ -427,9 +434,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     const uint32_t lp32 = static_cast<uint32_t>(Lp);
     // system components: u < 3 Q0_u = (Theta_0 t_i)_u (Theta_0 = the P_1
     // projection), then Theta_{k_i} for k_i = 1..N-1 (projection k_i + 1)
-    for (int job = tid; job < NC * NV; job += kPairThreads) {
-      const int u = job / NV;
-      const int pm0 = 4 * (job - u * NV);
+    auto sys_val = [&](int u, int pm0) {
       floatx4 val;
       if (u < 3) {
         const float* b1 = Pj + RP + PAIRS + pm0;
@@ -443,7 +448,68 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         const int c = (u - 3) - 6 * (kk - 2);
         val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
       }
+      return val;
+    };
+    for (int job = tid; job < NC * NV; job += kPairThreads) {
+      const int u = job / NV;
+      const int pm0 = 4 * (job - u * NV);
+      floatx4 val = sys_val(u, pm0);
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
+      if constexpr (HALF) {  // each source's largest magnitude (self pair zeroed by put)
+        // the 4 columns span at most two sources: one atomic for each
+        const int sa = pm0 / N;
+        unsigned ma = 0u, mb = 0u;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const unsigned bits = (pm0 + e < npm) ? __float_as_uint(fabsf(val[e])) : 0u;
+          if ((pm0 + e) / N == sa)
+            ma = max(ma, bits);
+          else
+            mb = max(mb, bits);
+        }
+        atomicMax(&smax[sa][tid & 7], ma);
+        if (mb != 0u) atomicMax(&smax[sa + 1][tid & 7], mb);
+      }
+    }
+    // FP16 copy of the system components with one FP32 scale per (target,
+    // source) block, S16[i M + j] = max |T| / 32768, values T / S16 (as the
+    // GEMV on the FP16 copy reads them; zero blocks keep scale 0).  The values
+    // are formed again from the projections rather than kept in registers.
+    if constexpr (HALF) {
+      __syncthreads();
+      if (tid < PAIRS) {
+        unsigned m = 0u;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          m = max(m, smax[tid][r]);
+          smax[tid][r] = 0u;  // for the next sweep (its atomics follow two barriers)
+        }
+        const float sv = __uint_as_float(m) * (1.0f / 32768.0f);
+        sinv[tid] = (sv > 0.0f) ? 1.0f / sv : 0.0f;
+        if (j0 + tid < je) S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
+      }
+      __syncthreads();
+      typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+      _Float16* const Hcol = T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N;
+      for (int job = tid; job < NC * NV; job += kPairThreads) {
+        const int u = job / NV;
+        const int pm0 = 4 * (job - u * NV);
+        const floatx4 val = sys_val(u, pm0);
+        halfx4 h;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const bool self = self_here && static_cast<unsigned>(pm0 + e - sself) < static_cast<unsigned>(N);
+          h[e] = static_cast<_Float16>(self ? 0.0f : val[e] * sinv[(pm0 + e) / N]);
+        }
+        _Float16* dst = Hcol + static_cast<int64_t>(u) * Lp + pm0;
+        if (full || pm0 + 3 < npm) {
+          *reinterpret_cast<halfx4*>(dst) = h;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (pm0 + e < npm) dst[e] = h[e];
+        }
+      }
     }
     // velocity operator: Theta_0 (projection 1), then Phi_0 (projection 0).
     // Analytical form: its jobs start on the lanes after the system
@@ -458,7 +524,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         const int pm0 = 4 * (job - w * NV);
         const int kk = w < 6 ? 1 : 0;
         const int c = w < 6 ? w : w - 6;
-        const floatx4 val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
+        floatx4 val = *reinterpret_cast<const floatx4*>(Pj + kk * RP + c * CB + PAIRS + pm0);
         put(Vcol, static_cast<uint32_t>(w) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
       }
     }
@@ -653,13 +719,13 @@ constexpr int theta_min_waves() {
   return (blocks >= 5 && !ANALYTIC) ? 4 : (blocks >= 4 ? 3 : (blocks >= 3 ? 2 : 1));
 }
 
-template <int N, int NQI, bool ANALYTIC>
+template <int N, int NQI, bool ANALYTIC, bool HALF>
 __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(theta_min_waves<N, NQI, ANALYTIC>(), 8))) void assemble_theta_kernel(
     int M, int64_t Lp, const double* __restrict__ pos, const double* __restrict__ ori, float* __restrict__ T,
     double* __restrict__ bpart, int jgroup, PhysConsts pc, float* __restrict__ Tv, double* __restrict__ cpart,
-    const double* __restrict__ x0, double* __restrict__ apart) {
-  assemble_theta_block<N, NQI, ANALYTIC>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv, cpart, x0,
-                                         apart);
+    const double* __restrict__ x0, double* __restrict__ apart, _Float16* __restrict__ T16, float* __restrict__ S16) {
+  assemble_theta_block<N, NQI, ANALYTIC, HALF>(blockIdx.x, blockIdx.y, M, Lp, pos, ori, T, bpart, jgroup, pc, Tv,
+                                               cpart, x0, apart, T16, S16);
 }
 
 // Serving: workgroup = (suspension, target fiber), all sources in one group,
@@ -671,7 +737,7 @@ __global__ __launch_bounds__(kPairThreads) __attribute__((amdgpu_waves_per_eu(th
   const int2 e = targets[blockIdx.x];
   const SusDev& d = sus[e.x];
   assemble_theta_block<N, NQI, ANALYTIC>(e.y, 0, d.M, d.Lp, d.pos, d.ori, d.T, d.b, d.M, pc, d.Tv, d.cpart, nullptr,
-                                         nullptr, &d);
+                                         nullptr, nullptr, nullptr, &d);
 }
 
 // Dense A (column-major, leading dimension n) from the theta form; identity
@@ -823,33 +889,38 @@ hipError_t launch_assemble(int N, int NQI, bool analytical, int M, int64_t lda, 
 template <int N, int NQI, bool AN>
 static hipError_t launch_theta_t(int M, int64_t Lp, const double* pos, const double* ori, float* T, double* bpart,
                                  float* Tv, double* cpart, const double* x0, double* apart, const PhysConsts& pc,
-                                 hipStream_t s, int* groups) {
+                                 hipStream_t s, int* groups, _Float16* T16, float* S16) {
   constexpr int PAIRS = kPairThreads / (3 * NQI);
   const PairGrid g = theta_grid(M, PAIRS);
   *groups = g.groups;
-  hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M, Lp, pos,
-                     ori, T, bpart, g.jgroup, pc, Tv, cpart, x0, apart);
+  if (T16)  // the FP16 copy of GMRES-IR (a separate instantiation: the plain kernel keeps its registers)
+    hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN, true>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M,
+                       Lp, pos, ori, T, bpart, g.jgroup, pc, Tv, cpart, x0, apart, T16, S16);
+  else
+    hipLaunchKernelGGL((assemble_theta_kernel<N, NQI, AN, false>), dim3(M, g.groups), dim3(kPairThreads), 0, s, M,
+                       Lp, pos, ori, T, bpart, g.jgroup, pc, Tv, cpart, x0, apart, nullptr, nullptr);
   return hipGetLastError();
 }
 
 template <int N, int NQI>
 static hipError_t launch_theta_n(bool an, int M, int64_t Lp, const double* pos, const double* ori, float* T,
                                  double* bpart, float* Tv, double* cpart, const double* x0, double* apart,
-                                 const PhysConsts& pc, hipStream_t s, int* groups) {
-  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups)
-            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups);
+                                 const PhysConsts& pc, hipStream_t s, int* groups, _Float16* T16, float* S16) {
+  return an ? launch_theta_t<N, NQI, true>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups, T16, S16)
+            : launch_theta_t<N, NQI, false>(M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups, T16, S16);
 }
 
 #define RF_TH_CASE(NN)                                                                             \
   case NN:                                                                                         \
-    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
-    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
-    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups); \
+    if (NQI == 4) return launch_theta_n<NN, 4>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups, T16, S16); \
+    if (NQI == 8) return launch_theta_n<NN, 8>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups, T16, S16); \
+    if (NQI == 16) return launch_theta_n<NN, 16>(analytical, M, Lp, pos, ori, T, bpart, Tv, cpart, x0, apart, pc, s, groups, T16, S16); \
     return hipErrorInvalidValue;
 
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
                                  const double* ori, float* T, double* bpart, float* Tv, double* cpart,
-                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups) {
+                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups,
+                                 _Float16* T16, float* S16) {
   switch (N) {
     RF_TH_CASE(1)
     RF_TH_CASE(2)

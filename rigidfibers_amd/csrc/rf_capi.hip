@@ -125,6 +125,7 @@ struct rf_simulation {
   rf::LuWorkspace lu{};
   rf::LuResult lu_last{};  // factorisation statistics of the last direct solve
   bool direct_fallback = false;  // ... which GMRES finished
+  int bicg_restarts = 0;         // BiCGStab: restarts in the last solve
 
   rf_solver_stats stats{};
   rf_timings timings{};
@@ -471,6 +472,7 @@ rf_status solve_gmres_ir(rf_simulation* s) {
 // Unpreconditioned BiCGStab, x0 = 0 (lib/viennacl/linalg/bicgstab.hpp:92-160).
 rf_status solve_bicgstab(rf_simulation* s) {
   const int64_t n = s->n;
+  s->bicg_restarts = 0;
   const int maxit = s->cfg.bicgstab_max_iterations;
   const double tol = s->cfg.bicgstab_tolerance;
   double* r = s->work;
@@ -517,6 +519,15 @@ rf_status solve_bicgstab(rf_simulation* s) {
     const int launched0 = s->ngemv;
     RF_HIP(rf::bicg_start(s->bicg, rho, res, s->stream));
     for (int k = 0; k < maxit; ++k) {
+      if (k > 0) {  // a pending restart: r = b - A x (untimed), r0 = p = r, rho = ||r||^2
+        if (s->theta)
+          RF_HIP(rf::gemv_theta(s->N, s->M, s->Lp, s->T, s->ori[s->cur], s->x, r, s->b, 1, s->pc, s->stream,
+                                nullptr, nullptr, &s->bicg.head->no_restart));
+        else
+          RF_HIP(rf::gemv(s->plan, s->A, s->x, s->gpart, r, s->b, 1, s->stream, nullptr, nullptr,
+                          &s->bicg.head->no_restart));
+        RF_HIP(rf::bicg_restart(n, w, s->dpart, s->bicg, s->stream));
+      }
       RF_HIP(sim_gemv(s, p, s->gpart, v, nullptr, 0, false, &s->bicg.head->done));
       RF_HIP(rf::bicg_after_v(n, k + 1, w, s->dpart, tol, bnorm, s->bicg, s->stream));
       RF_HIP(sim_gemv(s, sv, s->gpart, t, nullptr, 0, false, &s->bicg.head->done));
@@ -531,6 +542,7 @@ rf_status solve_bicgstab(rf_simulation* s) {
     RF_HIP(hipMemcpyAsync(&hd, s->bicg.head, sizeof(hd), hipMemcpyDeviceToHost, s->stream));
     RF_HIP(hipStreamSynchronize(s->stream));
     s->ngemv = std::min(s->ngemv, launched0 + hd.prods);  // products that returned at once are not counted
+    s->bicg_restarts = hd.restarts;
     if (hd.nonfinite) return fail(RF_ERR_NONFINITE, "non-finite residual in BiCGStab");
     it = hd.its;
     converged = hd.converged != 0;
@@ -853,12 +865,11 @@ rf_status rf_assemble_system(rf_simulation* s) {
     RF_HIP(hipEventRecord(s->ev[5], s->stream));
     RF_HIP(rf::launch_assemble_theta(s->N, s->NQI, s->analytical, s->M, s->Lp, s->pos[s->cur], s->ori[s->cur], s->T,
                                      s->bpart, s->Tv, s->cpart, x0, x0 ? s->apart : nullptr, s->pc, s->stream,
-                                     &s->bgroups));
+                                     &s->bgroups, s->T16, s->S16));
     RF_HIP(hipEventRecord(s->ev[6], s->stream));
     s->tori = s->cur;
     s->tori_valid = true;
     s->vvalid = s->vstored;
-    if (s->T16) RF_HIP(rf::theta_to_fp16(s->N, s->M, s->Lp, s->T, s->T16, s->S16, s->stream));
   } else {
     RF_HIP(hipEventRecord(s->ev[5], s->stream));
     RF_HIP(rf::launch_assemble(s->N, s->NQI, s->analytical, s->M, s->lda, s->pos[s->cur], s->ori[s->cur], s->A,
@@ -964,6 +975,7 @@ rf_status rf_step(rf_simulation* s, int64_t current_timestep) {
   s->timings.gemm = s->lu_last.gemm_ms;
   s->timings.gemm_flop = s->lu_last.gemm_flop;
   s->timings.direct_fallback = s->direct_fallback ? 1 : 0;
+  s->timings.solver_restarts = s->bicg_restarts;
   RF_TRY(rf_advance(s));
   if (!s->stats.converged) {
     g_last_error = "solver did not reach its tolerance (relative residual " +

@@ -36,9 +36,12 @@ PairGrid theta_grid(int M, int sources_per_iteration);
 int64_t theta_stride(int M, int N);
 hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t Lp, const double* pos,
                                  const double* ori, float* T, double* bpart, float* Tv, double* cpart,
-                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups);
+                                 const double* x0, double* apart, const PhysConsts& pc, hipStream_t s, int* groups,
+                                 _Float16* T16 = nullptr, float* S16 = nullptr);
 // With x0, the theta assembly also writes each group's off-diagonal product
-// rows sum_{j in group} A_ij x0_j into apart[group len + row].
+// rows sum_{j in group} A_ij x0_j into apart[group len + row].  With T16 it
+// also writes the FP16 copy of T with one FP32 scale per (target, source)
+// block, S16[i * M + j] = max |T_ij| / 32768 (the GMRES-IR inner system).
 // r = b - (x0 + sum_g apart[g]): the residual of x0 from those partials.
 hipError_t launch_residual_groups(int groups, int64_t len, const double* apart, const double* b, const double* x0,
                                   double* r, hipStream_t s);
@@ -52,9 +55,7 @@ hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, c
 // column at A[0]).
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
                                float* A, hipStream_t s, int j0 = 0, int jcount = -1);
-// FP16 copy of T with one FP32 scale per (target, source) block, S[i * M + j]
-// (rf_linalg.hip), and the theta GEMV on it.
-hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s);
+// The theta GEMV on the FP16 copy (rf_linalg.hip).
 hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
                         const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,
                         hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr, const int* skip = nullptr);
@@ -225,9 +226,12 @@ struct BicgHead {
   int converged;
   int nonfinite;  // ||s|| was not finite
   int its;        // iterations started
-  int prods;      // products that ran
+  int prods;      // products that ran (the iterations' v and t)
   int early_it;   // the iteration whose ||s|| met the tolerance (0: none)
-  int pad[2];
+  int restart;    // the next iteration restarts from the true residual
+  int no_restart; // 1 unless a restart is pending and the solve goes on (the skip flag of the restart's kernels)
+  int last_restart;  // iteration of the last restart (the start counts as one)
+  int restarts;      // restarts so far
   double rho, alpha, omega, beta, res;
 };
 struct BicgCtl {
@@ -243,6 +247,10 @@ struct BicgVecs {
 };
 size_t bicg_part_doubles(int64_t n);
 hipError_t bicg_start(const BicgCtl& c, double rho, double res, hipStream_t s);
+// A pending restart (lib/viennacl/linalg/bicgstab.hpp:119-131): after the
+// caller's r = b - A x (a product skipped unless head.no_restart is 0),
+// r0 = p = r and rho = ||r||^2; nothing when no restart is pending.
+hipError_t bicg_restart(int64_t n, const BicgVecs& w, double* part, const BicgCtl& c, hipStream_t s);
 // After v = A p of iteration it (1-based): alpha, s = r - alpha v, ||s||
 // and the early end with x += alpha p.
 hipError_t bicg_after_v(int64_t n, int it, const BicgVecs& w, double* part, double tol, double bnorm,

@@ -664,75 +664,12 @@ hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const
 
 
 // ---------------------------------------------------------------------------
-// FP16 storage of the theta form (GMRES-IR inner solves).  Every (target i,
-// source j) block -- the 27 x N numbers of a 3N x 3N block of A -- shares one
-// FP32 scale S[i][j] = max|T| / 32768, so the stored values use the FP16 range
-// whatever the block's distance (far blocks would otherwise sink into FP16's
-// subnormals).  Same component layout as T.
-
-template <int N>
-__global__ __launch_bounds__(256) void theta_to_fp16_kernel(int M, int64_t Lp, const float* __restrict__ T,
-                                                            _Float16* __restrict__ T16, float* __restrict__ S) {
-  constexpr int NC = 3 + 6 * (N - 1);
-  constexpr int JS = 256 / N;  // sources per sweep (whole blocks only)
-  __shared__ float mx[256];
-  __shared__ float sc[JS];
-  const int i = blockIdx.x;
-  const float* Ti = T + static_cast<int64_t>(i) * NC * Lp;
-  _Float16* Hi = T16 + static_cast<int64_t>(i) * NC * Lp;
-  const int el = threadIdx.x;
-  for (int j0 = 0; j0 < M; j0 += JS) {
-    const int j = j0 + el / N;
-    const bool active = el < JS * N && j < M;
-    const int64_t e = static_cast<int64_t>(j) * N + el % N;
-    float v[NC];
-    float m = 0.0f;
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      v[c] = active ? Ti[c * Lp + e] : 0.0f;
-      m = fmaxf(m, fabsf(v[c]));
-    }
-    mx[el] = m;
-    __syncthreads();
-    if (el < JS && j0 + el < M) {
-      float bm = 0.0f;
-#pragma unroll
-      for (int k = 0; k < N; ++k) bm = fmaxf(bm, mx[el * N + k]);
-      const float sv = bm * (1.0f / 32768.0f);
-      sc[el] = sv;
-      S[static_cast<int64_t>(i) * M + j0 + el] = sv;
-    }
-    __syncthreads();
-    if (active) {
-      const float sv = sc[el / N];
-      const float inv = (sv > 0.0f) ? 1.0f / sv : 0.0f;
-#pragma unroll
-      for (int c = 0; c < NC; ++c) Hi[c * Lp + e] = static_cast<_Float16>(v[c] * inv);
-    }
-    __syncthreads();
-  }
-}
-
-hipError_t theta_to_fp16(int N, int M, int64_t Lp, const float* T, _Float16* T16, float* S, hipStream_t s) {
-  switch (N) {
-#define RF_TH16(NN)                                                                                 \
-  case NN:                                                                                          \
-    hipLaunchKernelGGL(theta_to_fp16_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T, T16, S);     \
-    break;
-    RF_TH16(1)
-    RF_TH16(2)
-    RF_TH16(3)
-    RF_TH16(4)
-    RF_TH16(5)
-    RF_TH16(6)
-    RF_TH16(7)
-    RF_TH16(8)
-#undef RF_TH16
-    default:
-      return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
+// FP16 storage of the theta form (GMRES-IR inner solves), written by the
+// theta assembly.  Every (target i, source j) block -- the 27 x N numbers of
+// a 3N x 3N block of A -- shares one FP32 scale S[i][j] = max|T| / 32768, so
+// the stored values use the FP16 range whatever the block's distance (far
+// blocks would otherwise sink into FP16's subnormals).  Same component
+// layout as T.
 
 // y = A16 x (mode 0) or b - A16 x (mode 1): the theta GEMV on the FP16 copy.
 // Each lane takes two consecutive entries (one 4-byte load per component, a
@@ -1032,8 +969,13 @@ __device__ __forceinline__ double wave_sum_parts(const double* __restrict__ part
 
 __device__ __forceinline__ void bicg_end(const BicgCtl& c) {
   c.head->done = 1;
+  c.head->no_restart = 1;
   if (c.hdone) *c.hdone = 1;
 }
+
+// Iterations after which BiCGStab restarts from the true residual
+// (max_iterations_before_restart, lib/viennacl/linalg/bicgstab.hpp:52).
+constexpr int kBicgRestartEvery = 200;
 
 __global__ void bicg_start_kernel(BicgCtl c, double rho, double res) {
   if (threadIdx.x != 0) return;
@@ -1044,6 +986,10 @@ __global__ void bicg_start_kernel(BicgCtl c, double rho, double res) {
   h.its = 0;
   h.prods = 0;
   h.early_it = 0;
+  h.restart = 0;
+  h.no_restart = 1;
+  h.last_restart = 0;
+  h.restarts = 0;
   h.rho = rho;
   h.alpha = h.omega = h.beta = 0.0;
   h.res = res;
@@ -1120,7 +1066,10 @@ __global__ void bicg_xr_kernel(int64_t n, const double* __restrict__ p, const do
   r[i] = fma(-h->omega, t[i], 1.0 * (1.0 * sv[i]));
 }
 
-// ||r||, the end tests, beta and rho from ||r||^2 and <r0, r>
+// ||r||, the end tests, beta and rho from ||r||^2 and <r0, r>.  A degenerate
+// direction (<r0, r> = 0 or omega = 0) or more than kBicgRestartEvery
+// iterations since the last restart sets up a restart from the true residual
+// at the next iteration (lib/viennacl/linalg/bicgstab.hpp:150-152).
 __global__ __launch_bounds__(64) void bicg_end_kernel(int nb, const double* __restrict__ part, double tol,
                                                       double bnorm, BicgCtl c) {
   BicgHead& h = *c.head;
@@ -1129,31 +1078,76 @@ __global__ __launch_bounds__(64) void bicg_end_kernel(int nb, const double* __re
   if (threadIdx.x != 0) return;
   const double res = sqrt(rr);
   h.res = res;
+  if (!isfinite(res)) {
+    h.nonfinite = 1;
+    bicg_end(c);
+    return;
+  }
   if (res / bnorm <= tol) {
     h.converged = 1;
     bicg_end(c);
     return;
   }
-  if (h.omega == 0.0) {
-    bicg_end(c);
+  if (rho_new == 0.0 || h.omega == 0.0 || (h.its - 1) - h.last_restart > kBicgRestartEvery) {
+    h.restart = 1;
+    h.no_restart = 0;
     return;
   }
   h.beta = (rho_new / h.rho) * (h.alpha / h.omega);
   h.rho = rho_new;
 }
 
-// p = r + beta (p - omega v)
+// p = r + beta (p - omega v) (a pending restart sets p itself)
 __global__ void bicg_p_kernel(int64_t n, const double* __restrict__ r, const double* __restrict__ v,
                               double* __restrict__ p, const BicgHead* __restrict__ h) {
-  if (h->done) return;
+  if (h->done || h->restart) return;
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double q = fma(-h->omega, v[i], 1.0 * p[i]);
   p[i] = fma(1.0, r[i], h->beta * q);
 }
 
+// restart: r0 = p = r (r = b - A x from the caller's product)
+__global__ void bicg_restart_copy_kernel(int64_t n, const double* __restrict__ r, double* __restrict__ r0,
+                                         double* __restrict__ p, const BicgHead* __restrict__ h) {
+  if (h->no_restart) return;
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  r0[i] = r[i];
+  p[i] = r[i];
+}
+
+// restart: rho = ||r||^2, the restart recorded
+__global__ __launch_bounds__(64) void bicg_restart_kernel(int nb, const double* __restrict__ part, BicgCtl c) {
+  BicgHead& h = *c.head;
+  if (h.no_restart) return;
+  const double rr = wave_sum_parts(part, nb);
+  if (threadIdx.x != 0) return;
+  h.rho = rr;
+  h.res = sqrt(rr);
+  h.restart = 0;
+  h.no_restart = 1;
+  h.last_restart = h.its;  // the index of the iteration that starts now
+  ++h.restarts;
+  if (!isfinite(rr)) {
+    h.nonfinite = 1;
+    bicg_end(c);
+  }
+}
+
 hipError_t bicg_start(const BicgCtl& c, double rho, double res, hipStream_t s) {
   hipLaunchKernelGGL(bicg_start_kernel, dim3(1), dim3(64), 0, s, c, rho, res);
+  return hipGetLastError();
+}
+
+hipError_t bicg_restart(int64_t n, const BicgVecs& w, double* part, const BicgCtl& c, hipStream_t s) {
+  const int nb = mdot_blocks(n);
+  const int64_t rpb = (n + nb - 1) / nb;
+  const unsigned vb = static_cast<unsigned>((n + 255) / 256);
+  const int* skip = &c.head->no_restart;  // address arithmetic only
+  hipLaunchKernelGGL(bicg_restart_copy_kernel, dim3(vb), dim3(256), 0, s, n, w.r, w.r + n, w.p, c.head);
+  hipLaunchKernelGGL(mdot_partial_kernel, dim3(nb, 1), dim3(kDotThreads), 0, s, n, rpb, w.r, n, w.r, part, skip);
+  hipLaunchKernelGGL(bicg_restart_kernel, dim3(1), dim3(64), 0, s, nb, static_cast<const double*>(part), c);
   return hipGetLastError();
 }
 

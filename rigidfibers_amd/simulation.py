@@ -154,7 +154,7 @@ class Simulation:
     def timings(self):
         t = _lib.RfTimings()
         _lib.check(self._lib.rf_get_timings(self._h, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in _lib.RfTimings._fields_ if k != "reserved"}
+        return {k: getattr(t, k) for k, _ in _lib.RfTimings._fields_ if not k.startswith("reserved")}
 
     def device_bytes(self):
         return int(self._lib.rf_device_bytes(self._h))

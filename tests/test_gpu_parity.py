@@ -735,3 +735,33 @@ def test_direct_solve_without_cross_block_pivoting(oracle64, solver, d, max_swee
         assert code == _lib.RF_OK and st["relative_residual"] <= 1e-10 and res <= 1e-9
     else:
         assert code == _lib.RF_WARN_NOT_CONVERGED and abs(res - st["relative_residual"]) <= 1e-3 * res
+
+
+def test_bicgstab_restarts_from_the_true_residual():
+    """BiCGStab restarts from the true residual every 200 iterations and on
+    a degenerate search direction, as ViennaCL's solver does
+    (lib/viennacl/linalg/bicgstab.hpp:119-131,150-152).  On the
+    near-coincident system of condition number 1.5e7 it needs more than 200
+    iterations: the solve restarts, and either converges -- its true
+    residual then meets the tolerance -- or reports non-convergence with the
+    true residual of the returned x."""
+    from rigidfibers_amd import _lib
+
+    M, pos, ori = _near_coincident_scene(1e-4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    params.BICGSTAB_TOLERANCE = 1e-9
+    params.BICGSTAB_MAX_ITERATIONS = 1000
+    with _sim(params, pos, ori, solver="bicgstab", matrix_format="dense", confirm_residual=True) as sim:
+        code = sim.step(0)
+        A, b, x = sim.system(with_matrix=True)
+        st, tm = sim.solver_stats(), sim.timings()
+    res = np.linalg.norm(b - A.astype(np.float64) @ x) / np.linalg.norm(b)
+    print(f"bicgstab: status {code}, {st}, restarts {tm['solver_restarts']}, true residual {res:.1e}")
+    # periodic restarts before iterations 202, 404, ... (0-based), and any degenerate ones
+    assert st["iterations"] > 202 and tm["solver_restarts"] >= (st["iterations"] - 1) // 202
+    assert abs(res - st["relative_residual"]) <= 1e-3 * res
+    if st["converged"]:
+        assert code == _lib.RF_OK and res <= 1e-8
+    else:
+        assert code == _lib.RF_WARN_NOT_CONVERGED
