@@ -75,20 +75,41 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
 
-  uint4 ra[PF][CH], rb[PF][CH];
-  auto load = [&](int slot, int k0) {
+  // Rows past m (columns past n) of a ragged edge tile load the last valid
+  // row (column) instead: they only feed accumulator rows (columns) that the
+  // epilogue does not store, and the loads stay branch-free.
+  // two register slots, named apart: a slot index that is a run-time value
+  // anywhere would put the slots in scratch memory
+  uint4 ra0[CH], rb0[CH], ra1[CH], rb1[CH];
+  const T* arow[CH];
+  const T* brow[CH];
+#pragma unroll
+  for (int i = 0; i < CH; ++i) {
+    const int64_t r = min(m0 + srow + RSTEP * i, static_cast<int64_t>(m) - 1);
+    const int64_t c = min(n0 + srow + RSTEP * i, static_cast<int64_t>(n) - 1);
+    arow[i] = At + r * ldak + spart * E;
+    brow[i] = Bt + c * ldbk + spart * E;
+  }
+  auto load = [&](auto slot_c, int k0) {
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      const int64_t r = m0 + srow + RSTEP * i, c = n0 + srow + RSTEP * i;
-      ra[slot][i] = (r < m) ? *reinterpret_cast<const uint4*>(At + r * ldak + k0 + spart * E) : make_uint4(0, 0, 0, 0);
-      rb[slot][i] = (c < n) ? *reinterpret_cast<const uint4*>(Bt + c * ldbk + k0 + spart * E) : make_uint4(0, 0, 0, 0);
+      const uint4 a = *reinterpret_cast<const uint4*>(arow[i] + k0);
+      const uint4 b = *reinterpret_cast<const uint4*>(brow[i] + k0);
+      if constexpr (decltype(slot_c)::value == 0) {
+        ra0[i] = a;
+        rb0[i] = b;
+      } else {
+        ra1[i] = a;
+        rb1[i] = b;
+      }
     }
   };
-  auto store = [&](int slot, int buf) {
+  auto store = [&](auto slot_c, int buf) {
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      *reinterpret_cast<uint4*>(&As[buf][srow + RSTEP * i][spart * E]) = ra[slot][i];
-      *reinterpret_cast<uint4*>(&Bs[buf][srow + RSTEP * i][spart * E]) = rb[slot][i];
+      constexpr bool first = decltype(slot_c)::value == 0;
+      *reinterpret_cast<uint4*>(&As[buf][srow + RSTEP * i][spart * E]) = first ? ra0[i] : ra1[i];
+      *reinterpret_cast<uint4*>(&Bs[buf][srow + RSTEP * i][spart * E]) = first ? rb0[i] : rb1[i];
     }
   };
   auto compute = [&](int cur) {
@@ -134,33 +155,35 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
     }
   };
 
+  using Slot0 = std::integral_constant<int, 0>;
+  using Slot1 = std::integral_constant<int, 1>;
   const int stages = K / BK;
-  load(0, 0);
-  store(0, 0);
+  load(Slot0{}, 0);
+  store(Slot0{}, 0);
   if constexpr (PF == 1) {
     __syncthreads();
     for (int s = 0; s < stages; ++s) {
       const int cur = s & 1;
-      if (s + 1 < stages) load(0, (s + 1) * BK);
+      if (s + 1 < stages) load(Slot0{}, (s + 1) * BK);
       compute(cur);
-      if (s + 1 < stages) store(0, cur ^ 1);
+      if (s + 1 < stages) store(Slot0{}, cur ^ 1);
       __syncthreads();
     }
   } else {
     // register slot (s & 1) holds stage s until it is stored, then stage s + 2;
     // the loop is unrolled by two so that every slot index is a constant
-    if (stages > 1) load(1, BK);
+    if (stages > 1) load(Slot1{}, BK);
     __syncthreads();
     auto step = [&](auto cur_c, int s) {
       constexpr int cur = decltype(cur_c)::value;
-      if (s + 1 < stages) store(cur ^ 1, cur ^ 1);  // stage s + 1 (loaded one iteration ago)
-      if (s + 2 < stages) load(cur, (s + 2) * BK);
+      if (s + 1 < stages) store(std::integral_constant<int, cur ^ 1>{}, cur ^ 1);  // stage s + 1
+      if (s + 2 < stages) load(cur_c, (s + 2) * BK);
       compute(cur);
       __syncthreads();
     };
     for (int s = 0; s < stages; s += 2) {
-      step(std::integral_constant<int, 0>{}, s);
-      if (s + 1 < stages) step(std::integral_constant<int, 1>{}, s + 1);
+      step(Slot0{}, s);
+      if (s + 1 < stages) step(Slot1{}, s + 1);
     }
   }
 
@@ -211,8 +234,12 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
 
 // Launch shape per element type: 8 waves of 128x64 (two per SIMD, 128
 // accumulators each), loads two stages ahead.  Measured at m = n = 37888,
-// K = 1024: BF16 670 TFLOP/s and FP32 129 TFLOP/s, against 549 and 123 for
-// 4 waves of 128x128 loading one stage ahead.
+// K = 1024 (BF16): 565-583 TFLOP/s, against 550 for 4 waves of 128x128
+// loading one stage ahead and 475-530 for 256x128 tiles with two workgroups
+// per CU (with or without a persistent, staggered schedule).  The C update
+// is what holds it there: its read-modify-write moves 8 bytes per element of
+// C (11.5 GB at this size, the same order as the MFMA time at peak), and the
+// same loop without it runs at about 1.5 PFLOP/s.
 template <typename T>
 struct GemmShape {
   static constexpr int WM = 2, WN = 4, PF = 2;
