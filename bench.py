@@ -26,7 +26,10 @@ Also reported:
                 FP32 (8-point far-field rule: 8 kernel evaluations of 30 flop,
                 6 mirror sums and 36 FMAs per node pair), plus 7 920 FP32 per
                 pair for the projections; near and far nodes are counted on
-                the initial scene;
+                the initial scene.  reference_algorithm_count
+                prices the reference's algorithm instead (FP64 recurrences for
+                every pair: 32 500 flop per ordered pair, 15 000 of them FP64)
+                against the same kernel time;
   roofline_solver  iterative solvers: the solver's matvec (HBM-bound),
                 algorithmic bytes per launch (theta form: M * M * N * (3 +
                 6(N-1)) * 4, the FP32 numbers that define A; dense: n^2 * 4) /
@@ -211,8 +214,17 @@ def roofline_assembly(args, g):
     peak = flop / peak_s / 1e12
     theta = g["theta"]
     kernel = f"assemble_{'theta_' if theta else ''}kernel<5, 8, {'true' if args.integration == 'analytical' else 'false'}>"
+    # the reference algorithm's count (FP64 moment recurrences for every pair):
+    # 32 500 flop per ordered pair, 15 000 of them FP64
+    ref64, ref32 = 15_000.0 * pairs, 17_500.0 * pairs
+    ref_peak_s = ref64 / (FP64_PEAK_TFLOPS * 1e12) + ref32 / (FP32_PEAK_TFLOPS * 1e12)
+    ref = None
+    if args.integration == "analytical" and avg_ms > 0:
+        ref = {"flop_per_launch": ref64 + ref32, "fp64_flop_per_launch": ref64,
+               "time_at_peak_ms": round(1e3 * ref_peak_s, 3), "frac": round(ref_peak_s / (avg_ms * 1e-3), 4)}
     return {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 2), "peak": round(peak, 2),
             "unit": "TFLOP/s", "frac": round(achieved / peak, 4) if peak > 0 else 0.0,
+            "reference_algorithm_count": ref,
             "traffic": measured_traffic(kernel.split("<")[0], M, 5) if theta else None,
             "flop_per_launch": flop, "fp64_flop_per_launch": f64, "fp32_flop_per_launch": f32,
             "near_pair_nodes": near, "pair_nodes": pairs * 24,

@@ -301,6 +301,26 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 
+// Scale of one (target, source) block of the FP16 copy from g, the block's
+// largest |G_m(q_i)| over the modes m >= 1 the system components project:
+// each component is a projection sum_q w_q P_k(s_q) G (sum_q w_q = 2,
+// |P_k| <= 1), dotted with the unit t_i for the first three, so
+// |T| <= 2 sqrt(3) g < 6 g.  sv is the power of two at or above 6 g / 32768
+// (the stored values T / sv stay below 32768 in magnitude, and dividing by
+// a power of two is exact); iv = 1 / sv, 0 for an empty block.
+__device__ __forceinline__ void half_block_scale(float g, float& sv, float& iv) {
+  const float bound = 6.0f * g * (1.0f / 32768.0f);
+  if (!(bound > 0.0f) || !(bound < __builtin_huge_valf())) {  // empty, or not finite
+    sv = bound;
+    iv = 0.0f;
+    return;
+  }
+  int e;
+  (void)frexpf(bound, &e);  // bound = f 2^e, f in [0.5, 1)
+  sv = ldexpf(1.0f, e);
+  iv = ldexpf(1.0f, -e);
+}
+
 // The work of one workgroup: target fiber i, source group gy.
 //   phase 1  lane (p, q_i) writes G_m(q_i) of source slot p into LDS as row
 //            r = c*CB + (m == 0 ? p : PAIRS + p*N + m-1) of a GEMM operand
@@ -344,9 +364,9 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ __align__(16) float Pj[NM * RP];
   __shared__ double bred[PAIRS][NR];
   __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
-  // FP16 copy: largest |value| of each source's block (float bits), in 8
-  // replicas per source to spread the atomics, and its reciprocal scale
-  __shared__ unsigned smax[PAIRS][8];
+  // FP16 copy: each source slot's largest |G_m(q_i)|, m >= 1 (float bits),
+  // and the reciprocal of its block scale
+  __shared__ unsigned smax[PAIRS];
   __shared__ float sinv[PAIRS];
 
   const int tid = threadIdx.x;
@@ -367,7 +387,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   // RHS (and velocity-constant) sums of each source slot, kept in LDS; the
   // mode factors gamma_k, eigen_k as a table the RHS lanes index by k
   for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
-  if (tid < PAIRS * 8) smax[tid / 8][tid % 8] = 0u;
+  if (tid < PAIRS) smax[tid] = 0u;
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t)
     if (tid == t) {
@@ -387,8 +407,27 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       for (int m = 0; m < NM; ++m)
 #pragma unroll
         for (int c = 0; c < 6; ++c) G[(c * CB + (m == 0 ? p : PAIRS + p * N + m - 1)) * Q + qi] = acc[m][c];
+      if constexpr (HALF) {  // the modes the system components project
+        float g = 0.0f;
+#pragma unroll
+        for (int m = 1; m < NM; ++m)
+#pragma unroll
+          for (int c = 0; c < 6; ++c) g = fmaxf(g, fabsf(acc[m][c]));
+        atomicMax(&smax[p], __float_as_uint(g));
+      }
     }
     __syncthreads();
+    if constexpr (HALF) {
+      // FP16 block scale of each source slot, read and cleared before the
+      // next sweep's atomics (which follow the next barrier)
+      if (tid < PAIRS) {
+        float sv, iv;
+        half_block_scale(__uint_as_float(smax[tid]), sv, iv);
+        smax[tid] = 0u;
+        sinv[tid] = iv;
+        if (j0 + tid < je) S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
+      }
+    }
     for (int r = tid; r < RP; r += kPairThreads) {
       const float* gr = G + r * Q;
       float E[Q / 2], O[Q / 2];
@@ -455,53 +494,13 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       const int pm0 = 4 * (job - u * NV);
       floatx4 val = sys_val(u, pm0);
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
-      if constexpr (HALF) {  // each source's largest magnitude (self pair zeroed by put)
-        // the 4 columns span at most two sources: one atomic for each
-        const int sa = pm0 / N;
-        unsigned ma = 0u, mb = 0u;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const unsigned bits = (pm0 + e < npm) ? __float_as_uint(fabsf(val[e])) : 0u;
-          if ((pm0 + e) / N == sa)
-            ma = max(ma, bits);
-          else
-            mb = max(mb, bits);
-        }
-        atomicMax(&smax[sa][tid & 7], ma);
-        if (mb != 0u) atomicMax(&smax[sa + 1][tid & 7], mb);
-      }
-    }
-    // FP16 copy of the system components with one FP32 scale per (target,
-    // source) block, S16[i M + j] = max |T| / 32768, values T / S16 (as the
-    // GEMV on the FP16 copy reads them; zero blocks keep scale 0).  The values
-    // are formed again from the projections rather than kept in registers.
-    if constexpr (HALF) {
-      __syncthreads();
-      if (tid < PAIRS) {
-        unsigned m = 0u;
-#pragma unroll
-        for (int r = 0; r < 8; ++r) {
-          m = max(m, smax[tid][r]);
-          smax[tid][r] = 0u;  // for the next sweep (its atomics follow two barriers)
-        }
-        const float sv = __uint_as_float(m) * (1.0f / 32768.0f);
-        sinv[tid] = (sv > 0.0f) ? 1.0f / sv : 0.0f;
-        if (j0 + tid < je) S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
-      }
-      __syncthreads();
-      typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
-      _Float16* const Hcol = T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N;
-      for (int job = tid; job < NC * NV; job += kPairThreads) {
-        const int u = job / NV;
-        const int pm0 = 4 * (job - u * NV);
-        const floatx4 val = sys_val(u, pm0);
+      if constexpr (HALF) {  // the same values (self pair zeroed by put) on the FP16 copy
+        typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
         halfx4 h;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const bool self = self_here && static_cast<unsigned>(pm0 + e - sself) < static_cast<unsigned>(N);
-          h[e] = static_cast<_Float16>(self ? 0.0f : val[e] * sinv[(pm0 + e) / N]);
-        }
-        _Float16* dst = Hcol + static_cast<int64_t>(u) * Lp + pm0;
+        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * sinv[(pm0 + e) / N]);
+        _Float16* dst = T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N +
+                        static_cast<int64_t>(u) * Lp + pm0;
         if (full || pm0 + 3 < npm) {
           *reinterpret_cast<halfx4*>(dst) = h;
         } else {

@@ -666,7 +666,8 @@ hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const
 // ---------------------------------------------------------------------------
 // FP16 storage of the theta form (GMRES-IR inner solves), written by the
 // theta assembly.  Every (target i, source j) block -- the 27 x N numbers of
-// a 3N x 3N block of A -- shares one FP32 scale S[i][j] = max|T| / 32768, so
+// a 3N x 3N block of A -- shares one FP32 scale S[i][j], a power of two at
+// or above max|T| / 32768 (from a bound on the block's inner integrals), so
 // the stored values use the FP16 range whatever the block's distance (far
 // blocks would otherwise sink into FP16's subnormals).  Same component
 // layout as T.
