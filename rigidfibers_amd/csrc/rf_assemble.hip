@@ -301,24 +301,23 @@ __global__ __launch_bounds__(kPairThreads) void assemble_kernel(int M, int64_t l
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
 
-// Scale of one (target, source) block of the FP16 copy from g, the block's
-// largest |G_m(q_i)| over the modes m >= 1 the system components project:
-// each component is a projection sum_q w_q P_k(s_q) G (sum_q w_q = 2,
-// |P_k| <= 1), dotted with the unit t_i for the first three, so
-// |T| <= 2 sqrt(3) g < 6 g.  sv is the power of two at or above 6 g / 32768
-// (the stored values T / sv stay below 32768 in magnitude, and dividing by
-// a power of two is exact); iv = 1 / sv, 0 for an empty block.
-__device__ __forceinline__ void half_block_scale(float g, float& sv, float& iv) {
-  const float bound = 6.0f * g * (1.0f / 32768.0f);
-  if (!(bound > 0.0f) || !(bound < __builtin_huge_valf())) {  // empty, or not finite
-    sv = bound;
+// Scale of one (target, source) block of the FP16 copy from the bits of g,
+// the largest |projection| of the block's rows (modes m >= 1, projections
+// k >= 1): the first three components dot three of them with the unit t_i,
+// the others are projections themselves, so |T| <= sqrt(3) g < 2 g.  With g
+// = 1.f 2^(E - 127), sv = 2^(E - 140) > 2 g / 32768, so the stored values
+// T / sv stay below 32768 in magnitude (and dividing by a power of two is
+// exact); iv = 1 / sv.  An empty (or vanishing) block gets 0, 0; a
+// non-finite g is passed on as the scale.
+__device__ __forceinline__ void half_block_scale(unsigned gbits, float& sv, float& iv) {
+  const unsigned E = (gbits >> 23) & 0xffu;
+  if (E >= 14u && E < 255u) {
+    sv = __uint_as_float((E - 13u) << 23);
+    iv = __uint_as_float((267u - E) << 23);
+  } else {
+    sv = (E == 255u) ? __uint_as_float(gbits) : 0.0f;
     iv = 0.0f;
-    return;
   }
-  int e;
-  (void)frexpf(bound, &e);  // bound = f 2^e, f in [0.5, 1)
-  sv = ldexpf(1.0f, e);
-  iv = ldexpf(1.0f, -e);
 }
 
 // The work of one workgroup: target fiber i, source group gy.
@@ -364,10 +363,9 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ __align__(16) float Pj[NM * RP];
   __shared__ double bred[PAIRS][NR];
   __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
-  // FP16 copy: each source slot's largest |G_m(q_i)|, m >= 1 (float bits),
-  // and the reciprocal of its block scale
-  __shared__ unsigned smax[PAIRS];
-  __shared__ float sinv[PAIRS];
+  // FP16 copy: each source slot's largest |projection| (float bits), one
+  // buffer per sweep parity
+  __shared__ unsigned smax[2][PAIRS];
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -387,7 +385,7 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   // RHS (and velocity-constant) sums of each source slot, kept in LDS; the
   // mode factors gamma_k, eigen_k as a table the RHS lanes index by k
   for (int r = tid; r < PAIRS * NR; r += kPairThreads) bred[r / NR][r % NR] = 0.0;
-  if (tid < PAIRS) smax[tid] = 0u;
+  if (tid < 2 * PAIRS) smax[tid / PAIRS][tid % PAIRS] = 0u;
 #pragma unroll
   for (int t = 0; t < kMaxTerms; ++t)
     if (tid == t) {
@@ -407,26 +405,14 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       for (int m = 0; m < NM; ++m)
 #pragma unroll
         for (int c = 0; c < 6; ++c) G[(c * CB + (m == 0 ? p : PAIRS + p * N + m - 1)) * Q + qi] = acc[m][c];
-      if constexpr (HALF) {  // the modes the system components project
-        float g = 0.0f;
-#pragma unroll
-        for (int m = 1; m < NM; ++m)
-#pragma unroll
-          for (int c = 0; c < 6; ++c) g = fmaxf(g, fabsf(acc[m][c]));
-        atomicMax(&smax[p], __float_as_uint(g));
-      }
     }
     __syncthreads();
+    // FP16 copy: this sweep's block maxima go to smax[par]; the other buffer,
+    // last read in the previous sweep's phase 3 (before the barrier above), is
+    // cleared for the next sweep
+    const int par = ((j0 - jb) / PAIRS) & 1;
     if constexpr (HALF) {
-      // FP16 block scale of each source slot, read and cleared before the
-      // next sweep's atomics (which follow the next barrier)
-      if (tid < PAIRS) {
-        float sv, iv;
-        half_block_scale(__uint_as_float(smax[tid]), sv, iv);
-        smax[tid] = 0u;
-        sinv[tid] = iv;
-        if (j0 + tid < je) S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
-      }
+      if (tid < PAIRS) smax[par ^ 1][tid] = 0u;
     }
     for (int r = tid; r < RP; r += kPairThreads) {
       const float* gr = G + r * Q;
@@ -437,12 +423,18 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         E[h] = x + y;
         O[h] = x - y;
       }
+      float g = 0.0f;
 #pragma unroll
       for (int kk = 0; kk < NM; ++kk) {
         float acc = 0.0f;
 #pragma unroll
         for (int h = 0; h < Q / 2; ++h) acc = fmaf(quad.W[kk][h], (kk & 1) ? O[h] : E[h], acc);
         Pj[kk * RP + r] = acc;
+        if (kk > 0) g = fmaxf(g, fabsf(acc));
+      }
+      if constexpr (HALF) {  // rows of the modes m >= 1 feed the system components
+        const int e = r % CB;
+        if (e >= PAIRS) atomicMax(&smax[par][(e - PAIRS) / N], __float_as_uint(g));
       }
     }
     __syncthreads();
@@ -489,6 +481,13 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       }
       return val;
     };
+    if constexpr (HALF) {
+      if (tid < PAIRS && j0 + tid < je) {
+        float sv, iv;
+        half_block_scale(smax[par][tid], sv, iv);
+        S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
+      }
+    }
     for (int job = tid; job < NC * NV; job += kPairThreads) {
       const int u = job / NV;
       const int pm0 = 4 * (job - u * NV);
@@ -496,9 +495,15 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
       if constexpr (HALF) {  // the same values (self pair zeroed by put) on the FP16 copy
         typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+        // the 4 columns span at most two sources: e < split lie in source sa
+        const int sa = pm0 / N;
+        const int split = (sa + 1) * N - pm0;
+        float sv, iva, ivb;
+        half_block_scale(smax[par][sa], sv, iva);
+        half_block_scale(smax[par][min(sa + 1, PAIRS - 1)], sv, ivb);
         halfx4 h;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * sinv[(pm0 + e) / N]);
+        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * (e < split ? iva : ivb));
         _Float16* dst = T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N +
                         static_cast<int64_t>(u) * Lp + pm0;
         if (full || pm0 + 3 < npm) {
