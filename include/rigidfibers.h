@@ -85,7 +85,9 @@ typedef enum rf_initial_guess {
  * as the 3 + 6(N-1) numbers that determine a block column (Q0 and the
  * symmetric projections Theta, fortran/assemble_matrix.incl:59-137), 40% fewer
  * bytes for N = 5; the matrix-vector product rebuilds the blocks on the fly.
- * AUTO: THETA for the iterative solvers, DENSE for the direct ones.  Dumps
+ * AUTO: THETA.  The direct solvers expand the theta form into their factor
+ * storage and refine with the theta matrix-vector product; with DENSE they
+ * copy the dense matrix and refine with the dense product.  Dumps
  * (rf_get_system) always return the dense matrix. */
 typedef enum rf_matrix_format { RF_MATRIX_AUTO = 0, RF_MATRIX_DENSE = 1, RF_MATRIX_THETA = 2 } rf_matrix_format;
 

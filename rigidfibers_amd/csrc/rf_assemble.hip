@@ -940,6 +940,85 @@ hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t
 }
 #undef RF_TH_CASE
 
+// The whole dense A from the theta form, for the LU's factor storage
+// (leading dimension lda).  Lane = entry e (source j = e / N, mode k_j =
+// e % N) of 64 consecutive entries, so each of the 27 numbers of (i, e) is
+// one 256-byte load across the wave; each wave then walks a run of targets i
+// and writes the 3N x 3 block those numbers define, columns 3 e + b and rows
+// 3N i + r, with the arithmetic of expand_theta_kernel (the identity for
+// i == j).  A lane's 3N floats of a column are contiguous and the next
+// target continues them, so the stores meet in L2.
+template <int N>
+__global__ __launch_bounds__(256) void expand_theta_dense_kernel(int M, int64_t Lp, const float* __restrict__ T,
+                                                                 const double* __restrict__ ori, PhysConsts pc,
+                                                                 int tchunk, float* __restrict__ A, int64_t lda) {
+  constexpr int NC = 3 + 6 * (N - 1);
+  constexpr int ROWS = 3 * N;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+  if (e >= static_cast<int64_t>(M) * N) return;  // no barrier follows
+  const int j = static_cast<int>(e / N);
+  const int kj = static_cast<int>(e - static_cast<int64_t>(j) * N);
+  const int i1 = min(M, static_cast<int>(blockIdx.y + 1) * tchunk);
+  for (int i = static_cast<int>(blockIdx.y) * tchunk + wave; i < i1; i += 4) {
+    float* const blk = A + 3 * e * lda + static_cast<int64_t>(ROWS) * i;  // column 3 e, row 3N i
+    if (i == j) {
+      for (int b = 0; b < 3; ++b)
+        for (int r = 0; r < ROWS; ++r) blk[b * lda + r] = (r == 3 * kj + b) ? 1.0f : 0.0f;
+      continue;
+    }
+    const float* Ti = T + static_cast<int64_t>(i) * NC * Lp + e;
+    float v[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = Ti[c * Lp];
+    const float t[3] = {static_cast<float>(ori[3 * i]), static_cast<float>(ori[3 * i + 1]),
+                        static_cast<float>(ori[3 * i + 2])};
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      float* col = blk + b * lda;
+      const float q0 = v[b];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) col[a] = pc.D1 * t[a] * q0;
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        const float* Tk = v + 3 + 6 * (k - 1);
+        float qb = 0.0f;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) qb += Tk[sym_index(a, b)] * t[a];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) col[3 * k + a] = pc.gamma[k] * (Tk[sym_index(a, b)] - pc.eigen[k] * t[a] * qb);
+      }
+    }
+  }
+}
+
+hipError_t launch_expand_theta_dense(int N, int M, int64_t Lp, const float* T, const double* ori,
+                                     const PhysConsts& pc, float* A, int64_t lda, hipStream_t s) {
+  if (M < 1 || lda < static_cast<int64_t>(3) * N * M) return hipErrorInvalidValue;
+  constexpr int kTargets = 64;  // targets per workgroup, 16 per wave
+  const dim3 grid(static_cast<unsigned>((static_cast<int64_t>(M) * N + 63) / 64),
+                  static_cast<unsigned>((M + kTargets - 1) / kTargets));
+  switch (N) {
+#define RF_EXD(NN)                                                                                           \
+  case NN:                                                                                                   \
+    hipLaunchKernelGGL(expand_theta_dense_kernel<NN>, grid, dim3(256), 0, s, M, Lp, T, ori, pc, kTargets, A, \
+                       lda);                                                                                 \
+    break;
+    RF_EXD(1)
+    RF_EXD(2)
+    RF_EXD(3)
+    RF_EXD(4)
+    RF_EXD(5)
+    RF_EXD(6)
+    RF_EXD(7)
+    RF_EXD(8)
+#undef RF_EXD
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
                                float* A, hipStream_t s, int j0, int jcount) {
   if (jcount < 0) jcount = M - j0;

@@ -565,11 +565,24 @@ rf_status solve_bicgstab(rf_simulation* s) {
 rf_status solve_direct(rf_simulation* s) {
   const int mode = s->cfg.solver;
   rf::LuResult r{};
+  rf::LuSystem sys;
+  const int64_t n = s->n;
+  if (s->theta) {
+    sys.fill = [s](float* F, int64_t lda, hipStream_t st) {
+      return rf::launch_expand_theta_dense(s->N, s->M, s->Lp, s->T, s->ori[s->tori], s->pc, F, lda, st);
+    };
+  } else {
+    sys.fill = [s, n](float* F, int64_t lda, hipStream_t st) {
+      return hipMemcpy2DAsync(F, lda * sizeof(float), s->A, s->lda * sizeof(float), n * sizeof(float), n,
+                              hipMemcpyDeviceToDevice, st);
+    };
+  }
+  sys.residual = [s](const double* x, double* r, hipStream_t) { return sim_gemv(s, x, s->gpart, r, s->b, 1); };
   hipError_t e = rf::lu_solve(s->lu, mode == RF_SOLVER_DIRECT ? rf::LuMode::kFP32
                                      : mode == RF_SOLVER_DIRECT_MIXED_BF16 ? rf::LuMode::kBF16
                                                                           : rf::LuMode::kFP16,
-                              s->A, s->lda, s->n, s->b, s->x, s->gpart, s->plan, s->work, s->dpart, s->hdev,
-                              s->cfg.refinement_tolerance, s->cfg.refinement_max_iterations, s->stream, &r);
+                              sys, s->n, s->b, s->x, s->work, s->dpart, s->hdev, s->cfg.refinement_tolerance,
+                              s->cfg.refinement_max_iterations, s->stream, &r);
   if (e != hipSuccess) return fail(RF_ERR_DEVICE, std::string("LU: ") + hipGetErrorString(e));
   s->lu_last = r;
   s->stats.iterations = r.refinement_iterations;
@@ -740,8 +753,9 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
       cfg->solver == RF_SOLVER_GMRES || cfg->solver == RF_SOLVER_BICGSTAB || cfg->solver == RF_SOLVER_GMRES_IR;
   if (cfg->solver == RF_SOLVER_GMRES_IR && cfg->matrix_format == RF_MATRIX_DENSE)
     return cleanup(fail(RF_ERR_UNSUPPORTED, "GMRES-IR needs the theta form"));
-  s->theta = (cfg->matrix_format == RF_MATRIX_THETA) || (cfg->matrix_format == RF_MATRIX_AUTO && iterative);
-  if (s->theta && !iterative) return cleanup(fail(RF_ERR_UNSUPPORTED, "the direct solvers need the dense matrix"));
+  // The theta form unless the dense one is asked for: the direct solvers
+  // expand it into their factor storage and refine with the theta GEMV.
+  s->theta = cfg->matrix_format != RF_MATRIX_DENSE;
   if (s->theta) {
     const int NC = 3 + 6 * (s->N - 1);
     s->Lp = rf::theta_stride(M, s->N);
@@ -761,7 +775,7 @@ rf_status rf_create(const rf_config* cfg, const double* positions, const double*
   if (st == RF_OK) st = s->alloc(&s->x, static_cast<size_t>(n));
   if (st == RF_OK && cfg->initial_guess == RF_GUESS_EXTRAPOLATED) st = s->alloc(&s->xprev, static_cast<size_t>(n));
   if (st == RF_OK) st = s->alloc(&s->bpart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
-  if (s->theta && !cfg->separate_residual) {
+  if (s->theta && iterative && !cfg->separate_residual) {
     if (st == RF_OK) st = s->alloc(&s->apart, static_cast<size_t>(rf::max_pair_groups(M)) * n);
     if (st == RF_OK) st = s->alloc(&s->r0, static_cast<size_t>(n));
   }

@@ -55,6 +55,10 @@ hipError_t launch_velocities_stored(int N, int M, int64_t Lp, const float* Tv, c
 // column at A[0]).
 hipError_t launch_expand_theta(int N, int M, int64_t Lp, const float* T, const double* ori, const PhysConsts& pc,
                                float* A, hipStream_t s, int j0 = 0, int jcount = -1);
+// The whole dense A (n x n, leading dimension lda, the identity diagonal
+// blocks included) from the theta form, as the LU's factor storage takes it.
+hipError_t launch_expand_theta_dense(int N, int M, int64_t Lp, const float* T, const double* ori,
+                                     const PhysConsts& pc, float* A, int64_t lda, hipStream_t s);
 // The theta GEMV on the FP16 copy (rf_linalg.hip).
 hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const float* S, const double* ori,
                         const double* x, double* y, const double* b, int mode, const PhysConsts& pc, hipStream_t s,

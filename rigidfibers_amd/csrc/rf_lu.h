@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
+
 #include "rf_internal.h"
 
 namespace rf {
@@ -70,11 +72,20 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
 int64_t lu_workspace_bytes(const LuWorkspace& ws);
 void lu_workspace_destroy(LuWorkspace* ws);
 
-// Factors a copy of A (FP32 column-major, lda) and solves A x = b with FP64
-// iterative refinement of the residual computed from the FP32 A.
-// gpart/plan: GEMV workspace; work: >= 2n doubles; dpart/hdev: dot workspace.
-hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const float* A, int64_t lda, int64_t n, const double* b,
-                    double* x, double* gpart, const GemvPlan& plan, double* work, double* dpart, double* hdev,
-                    double tol, int maxit, hipStream_t s, LuResult* result);
+// The system an LU solve factors and refines against: fill writes the FP32 A
+// (column-major, leading dimension lda) into the factor storage F;
+// residual forms r = b - A x with FP64 sums of the same FP32 A.  Both are
+// enqueued on the stream they are given.
+struct LuSystem {
+  std::function<hipError_t(float* F, int64_t lda, hipStream_t s)> fill;
+  std::function<hipError_t(const double* x, double* r, hipStream_t s)> residual;
+};
+
+// Factors A (from sys.fill) and solves A x = b with FP64 iterative
+// refinement of the residual from sys.residual.  work: >= 2n doubles;
+// dpart/hdev: dot workspace.
+hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const LuSystem& sys, int64_t n, const double* b, double* x,
+                    double* work, double* dpart, double* hdev, double tol, int maxit, hipStream_t s,
+                    LuResult* result);
 
 }  // namespace rf

@@ -766,15 +766,13 @@ void lu_workspace_destroy(LuWorkspace* ws) {
   *ws = LuWorkspace{};
 }
 
-hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const float* A, int64_t lda, int64_t n, const double* b,
-                    double* x, double* gpart, const GemvPlan& plan, double* work, double* dpart, double* hdev,
-                    double tol, int maxit, hipStream_t s, LuResult* result) {
+hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const LuSystem& sys, int64_t n, const double* b, double* x,
+                    double* work, double* dpart, double* hdev, double tol, int maxit, hipStream_t s,
+                    LuResult* result) {
   hipError_t e;
   *result = LuResult{};
-  if (mode != ws.mode || n != ws.n || lda != ws.lda) return hipErrorInvalidValue;
-  if ((e = hipMemcpy2DAsync(ws.F, lda * sizeof(float), A, lda * sizeof(float), n * sizeof(float), n,
-                            hipMemcpyDeviceToDevice, s)) != hipSuccess)
-    return e;
+  if (mode != ws.mode || n != ws.n || !sys.fill || !sys.residual) return hipErrorInvalidValue;
+  if ((e = sys.fill(ws.F, ws.lda, s)) != hipSuccess) return e;
   if ((e = hipMemsetAsync(ws.info, 0, sizeof(int), s)) != hipSuccess) return e;
   if ((e = lu_factor(ws, mode, s, result)) != hipSuccess) return e;
   int info = 0;
@@ -798,7 +796,7 @@ hipError_t lu_solve(LuWorkspace& ws, LuMode mode, const float* A, int64_t lda, i
   if ((e = lu_apply_inverse(ws, x, s)) != hipSuccess) return e;
   double res = 0.0;
   for (int it = 0;; ++it) {
-    if ((e = gemv(plan, A, x, gpart, r, b, 1, s)) != hipSuccess) return e;
+    if ((e = sys.residual(x, r, s)) != hipSuccess) return e;
     if ((e = mdot(n, 1, r, n, r, dpart, hdev, s)) != hipSuccess) return e;
     double rr = 0.0;
     if ((e = hipMemcpyAsync(&rr, hdev, sizeof(double), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;

@@ -500,9 +500,9 @@ def test_cube_m100_ten_steps(oracle64):
 
 @pytest.mark.parametrize("analytical", [False, True])
 def test_cube_m2000_gmres_and_lu_trajectories(analytical):
-    """M = 2000 in a cube of side 40 (n = 30 000), 20 steps: GMRES to 1e-8 on
-    the theta form and the FP32 LU with FP64 refinement on the dense matrix
-    follow the same trajectory."""
+    """M = 2000 in a cube of side 40 (n = 30 000), 20 steps: GMRES to 1e-8 and
+    the FP32 LU with FP64 refinement (its factors from the expanded theta
+    form) follow the same trajectory."""
     from rigidfibers_amd.scenes import cube
 
     M = 2000
@@ -521,6 +521,43 @@ def test_cube_m2000_gmres_and_lu_trajectories(analytical):
     ep, eo = rel_err(out["gmres"][0], out["direct"][0]), rel_err(out["gmres"][1], out["direct"][1])
     print(f"cube M=2000 {integration}, 20 steps, GMRES vs LU: pos {ep:.1e} ori {eo:.1e}")
     assert ep < 1e-7 and eo < 1e-6
+
+
+@pytest.mark.parametrize("solver", ["direct", "mixed-bf16"])
+@pytest.mark.parametrize("integration", ["numerical", "analytical"])
+def test_direct_solvers_on_theta_and_dense_forms(solver, integration):
+    """The direct solvers factor the expanded theta form (the default) or a
+    copy of the dense matrix: the two systems differ by the FP32 rounding of
+    the two assembly kernels, the dumps agree to that, each solve meets its
+    own system's refinement tolerance (seen through the FP32 dump), and the
+    states after 3 steps agree."""
+    M = 150
+    pos, ori = random_scene(M, side=7.0, seed=31, min_dist=0.4)
+    params = ref_config(validate=False)
+    params.NUMBER_OF_FIBERS = M
+    out = {}
+    for fmt in ("auto", "dense"):
+        with _sim(params, pos, ori, solver=solver, integration=integration, matrix_format=fmt,
+                  refinement_tolerance=1e-10) as sim:
+            sim.assemble_system()
+            sim.solve_system()
+            st = sim.solver_stats()
+            A, b, x = sim.system(with_matrix=True)
+            res = np.linalg.norm(b - A.astype(np.float64) @ x) / np.linalg.norm(b)
+            # the dense dump is the system itself; the theta form's dump rounds
+            # the blocks its matrix-vector product forms in FP64 to FP32
+            assert st["converged"] and res <= (1e-9 if fmt == "dense" else 1e-7), (fmt, st, res)
+            sim.update_velocities()
+            sim.update_fibers(True)
+            sim.advance()
+            for t in range(1, 3):
+                sim.step(t)
+            out[fmt] = (A, x, sim.state())
+    eA = rel_err(out["auto"][0], out["dense"][0])
+    ex = rel_err(out["auto"][1], out["dense"][1])
+    ep, eo = rel_err(out["auto"][2][0], out["dense"][2][0]), rel_err(out["auto"][2][1], out["dense"][2][1])
+    print(f"{solver} {integration} theta vs dense: A {eA:.1e} x {ex:.1e} pos {ep:.1e} ori {eo:.1e}")
+    assert eA < 1e-6 and ex < 1e-5 and ep < 1e-7 and eo < 1e-6
 
 
 @pytest.mark.parametrize("solver", ["gmres", "bicgstab", "gmres-ir"])
@@ -658,7 +695,7 @@ def test_gmres_restart_extremes(restart):
     params.NUMBER_OF_FIBERS = M
     params.GMRES_RESTART = restart
     params.GMRES_TOLERANCE = 1e-10
-    with _sim(params, pos, ori, solver="mixed-bf16", refinement_tolerance=1e-12) as sim:
+    with _sim(params, pos, ori, solver="mixed-bf16", matrix_format="dense", refinement_tolerance=1e-12) as sim:
         sim.step(0)
         _, _, x_ref = sim.system()
     with _sim(params, pos, ori, solver="gmres", matrix_format="dense", confirm_residual=True) as sim:
@@ -714,7 +751,7 @@ def test_direct_solve_without_cross_block_pivoting(oracle64, solver, d, max_swee
     params.GMRES_MAX_ITERATIONS = 2000
     A_ref, b_ref = oracle64.assemble(make_params(M), pos, ori, False)
     x_ref = oracle64.solve_direct(A_ref, b_ref)
-    with _sim(params, pos, ori, solver=solver, refinement_tolerance=1e-10,
+    with _sim(params, pos, ori, solver=solver, matrix_format="dense", refinement_tolerance=1e-10,
               refinement_max_iterations=max_sweeps) as sim:
         code = sim.step(0)
         A, b, x = sim.system(with_matrix=True)
