@@ -941,68 +941,91 @@ hipError_t launch_assemble_theta(int N, int NQI, bool analytical, int M, int64_t
 #undef RF_TH_CASE
 
 // The whole dense A from the theta form, for the LU's factor storage
-// (leading dimension lda).  Lane = entry e (source j = e / N, mode k_j =
-// e % N) of 64 consecutive entries, so each of the 27 numbers of (i, e) is
-// one 256-byte load across the wave; each wave then walks a run of targets i
-// and writes the 3N x 3 block those numbers define, columns 3 e + b and rows
-// 3N i + r, with the arithmetic of expand_theta_kernel (the identity for
-// i == j).  A lane's 3N floats of a column are contiguous and the next
-// target continues them, so the stores meet in L2.
+// (leading dimension lda), with the arithmetic of expand_theta_kernel (the
+// identity for i == j).  A workgroup takes TI = 256 / 3N targets x 32 entries
+// (entry e: source j = e / N, mode k_j = e % N): it stages their 3 + 6(N-1)
+// numbers in LDS, one 128-byte row per (target, component), then writes the
+// tile's 96 columns 3 e + b, each a run of TI x 3N contiguous rows 3N i + r,
+// one row per lane.
+template <int N>
+constexpr int exp_targets() { return 256 / (3 * N); }
+constexpr int kExpEntries = 32;
+constexpr int kExpRow = kExpEntries + 4;  // padded LDS row (16-byte aligned, banks spread)
+
 template <int N>
 __global__ __launch_bounds__(256) void expand_theta_dense_kernel(int M, int64_t Lp, const float* __restrict__ T,
                                                                  const double* __restrict__ ori, PhysConsts pc,
-                                                                 int tchunk, float* __restrict__ A, int64_t lda) {
+                                                                 float* __restrict__ A, int64_t lda) {
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int ROWS = 3 * N;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t e = static_cast<int64_t>(blockIdx.x) * 64 + lane;
-  if (e >= static_cast<int64_t>(M) * N) return;  // no barrier follows
-  const int j = static_cast<int>(e / N);
-  const int kj = static_cast<int>(e - static_cast<int64_t>(j) * N);
-  const int i1 = min(M, static_cast<int>(blockIdx.y + 1) * tchunk);
-  for (int i = static_cast<int>(blockIdx.y) * tchunk + wave; i < i1; i += 4) {
-    float* const blk = A + 3 * e * lda + static_cast<int64_t>(ROWS) * i;  // column 3 e, row 3N i
-    if (i == j) {
-      for (int b = 0; b < 3; ++b)
-        for (int r = 0; r < ROWS; ++r) blk[b * lda + r] = (r == 3 * kj + b) ? 1.0f : 0.0f;
-      continue;
+  constexpr int TI = exp_targets<N>();
+  constexpr int TROWS = TI * ROWS;  // rows of a column in the tile, <= 256
+  __shared__ __align__(16) float Ts[TI * NC * kExpRow];
+  const int tid = threadIdx.x;
+  const int64_t L = static_cast<int64_t>(M) * N;
+  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kExpEntries;
+  const int i0 = static_cast<int>(blockIdx.y) * TI;
+  // stage: row (ti, c) holds T[i0 + ti][c][e0 .. e0 + 31] (within the padded
+  // row of Lp >= e0 + 32 numbers)
+  for (int q = tid; q < TI * NC * (kExpEntries / 4); q += 256) {
+    const int row = q / (kExpEntries / 4), part = q - row * (kExpEntries / 4);
+    const int ti = row / NC, c = row - ti * NC;
+    if (i0 + ti < M) {
+      const float4 v = *reinterpret_cast<const float4*>(T + (static_cast<int64_t>(i0 + ti) * NC + c) * Lp + e0 +
+                                                        4 * part);
+      *reinterpret_cast<float4*>(&Ts[row * kExpRow + 4 * part]) = v;
     }
-    const float* Ti = T + static_cast<int64_t>(i) * NC * Lp + e;
-    float v[NC];
+  }
+  __syncthreads();
+  if (tid >= TROWS) return;  // no barrier follows
+  const int ti = tid / ROWS, r = tid - ti * ROWS;
+  const int i = i0 + ti;
+  if (i >= M) return;
+  const int k = r / 3, a = r - 3 * k;
+  const float t[3] = {static_cast<float>(ori[3 * i]), static_cast<float>(ori[3 * i + 1]),
+                      static_cast<float>(ori[3 * i + 2])};
+  float gk = 0.0f, ek = 0.0f;
 #pragma unroll
-    for (int c = 0; c < NC; ++c) v[c] = Ti[c * Lp];
-    const float t[3] = {static_cast<float>(ori[3 * i]), static_cast<float>(ori[3 * i + 1]),
-                        static_cast<float>(ori[3 * i + 2])};
+  for (int kk = 1; kk < N; ++kk)
+    if (k == kk) {
+      gk = pc.gamma[kk];
+      ek = pc.eigen[kk];
+    }
+  const float* Tr = Ts + ti * NC * kExpRow;  // this target's rows
+  const int cbase = (k == 0) ? 0 : 3 + 6 * (k - 1);
+  const int ecount = static_cast<int>(min(static_cast<int64_t>(kExpEntries), L - e0));
+  float* const out = A + static_cast<int64_t>(ROWS) * i0 + tid;
+  for (int ee = 0; ee < ecount; ++ee) {
+    const int64_t e = e0 + ee;
+    const int j = static_cast<int>(e / N), kj = static_cast<int>(e - static_cast<int64_t>(j) * N);
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      float* col = blk + b * lda;
-      const float q0 = v[b];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) col[a] = pc.D1 * t[a] * q0;
-#pragma unroll
-      for (int k = 1; k < N; ++k) {
-        const float* Tk = v + 3 + 6 * (k - 1);
+      float v;
+      if (i == j) {
+        v = (r == 3 * kj + b) ? 1.0f : 0.0f;
+      } else if (k == 0) {
+        v = pc.D1 * t[a] * Tr[b * kExpRow + ee];
+      } else {
         float qb = 0.0f;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) qb += Tk[sym_index(a, b)] * t[a];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) col[3 * k + a] = pc.gamma[k] * (Tk[sym_index(a, b)] - pc.eigen[k] * t[a] * qb);
+        for (int a2 = 0; a2 < 3; ++a2) qb += Tr[(cbase + sym_index(a2, b)) * kExpRow + ee] * t[a2];
+        v = gk * (Tr[(cbase + sym_index(a, b)) * kExpRow + ee] - ek * t[a] * qb);
       }
+      out[(3 * e + b) * lda] = v;
     }
   }
 }
 
 hipError_t launch_expand_theta_dense(int N, int M, int64_t Lp, const float* T, const double* ori,
                                      const PhysConsts& pc, float* A, int64_t lda, hipStream_t s) {
-  if (M < 1 || lda < static_cast<int64_t>(3) * N * M) return hipErrorInvalidValue;
-  constexpr int kTargets = 64;  // targets per workgroup, 16 per wave
-  const dim3 grid(static_cast<unsigned>((static_cast<int64_t>(M) * N + 63) / 64),
-                  static_cast<unsigned>((M + kTargets - 1) / kTargets));
+  if (M < 1 || lda < static_cast<int64_t>(3) * N * M || Lp % kExpEntries != 0) return hipErrorInvalidValue;
+  const unsigned ex = static_cast<unsigned>((static_cast<int64_t>(M) * N + kExpEntries - 1) / kExpEntries);
   switch (N) {
-#define RF_EXD(NN)                                                                                           \
-  case NN:                                                                                                   \
-    hipLaunchKernelGGL(expand_theta_dense_kernel<NN>, grid, dim3(256), 0, s, M, Lp, T, ori, pc, kTargets, A, \
-                       lda);                                                                                 \
+#define RF_EXD(NN)                                                                                          \
+  case NN:                                                                                                  \
+    hipLaunchKernelGGL(expand_theta_dense_kernel<NN>,                                                      \
+                       dim3(ex, static_cast<unsigned>((M + exp_targets<NN>() - 1) / exp_targets<NN>())),   \
+                       dim3(256), 0, s, M, Lp, T, ori, pc, A, lda);                                         \
     break;
     RF_EXD(1)
     RF_EXD(2)
