@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
       float* cc = C + col * ldc;
       float w = 1.0f;
       if (!STORE && sa) {  // exact: powers of two
-        const int64_t rb = (m0 + wm + 32 * ta) >> 5;
+        const int64_t rb = min(m0 + wm + 32 * ta, static_cast<int64_t>(m) - 1) >> 5;  // rows past m: unused
         w = 1.0f / (sa[rb] * sb[col >> 5]);
       }
 #pragma unroll

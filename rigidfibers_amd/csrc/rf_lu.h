@@ -24,16 +24,22 @@ struct LuWorkspace {
   int* perm = nullptr;     // perm[k0 + c] = original row of factored row k0 + c (block-local)
   int* info = nullptr;     // device: first zero pivot (1-based) or 0
   double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
-  // packs of kLuPanel values per row, K-contiguous, in the mode's element type
-  void* At = nullptr;      // L21 of the current panel: row r - KE = L[r][K0 .. KE), per-32-row scales sA
-  void* At2 = nullptr;     // the same for the next panel, formed while the current one's trailing update
-  float* sA = nullptr;     // runs (look-ahead)
-  float* sA2 = nullptr;
-  void* Bt = nullptr;      // U12: row c - KE = U[K0 .. KE)[c], per-32-column scales sB
+  // packs, K-contiguous, in the mode's element type (per-32-row / per-32-column
+  // scales in the mixed modes); panels are taken in pairs a, b (rf_lu.hip)
+  void* At = nullptr;      // [L21_a L21_b], rows right of the pair: 2 kLuPanel per row, scales sA
+  float* sA = nullptr;
+  void* Bt = nullptr;      // [U12_a; U12_b], columns right of the pair: 2 kLuPanel per column, scales sB
   float* sB = nullptr;
-  void* Xm = nullptr;      // A12 before its solve (pack of columns), main stream
-  void* Ya = nullptr;      // A21 before its solve (pack of rows), aux stream
-  void* Tinv = nullptr;    // kLuPanel^2: the forward operator of the panel's diagonal block, row-major
+  void* At2 = nullptr;     // L21_a, rows below panel a: kLuPanel per row, scales sA2
+  float* sA2 = nullptr;
+  void* Ua = nullptr;      // U12_a over the columns right of the pair, scales sUa
+  float* sUa = nullptr;
+  void* Bb = nullptr;      // U12_a over panel b's columns (at most kLuPanel), scales sBb
+  float* sBb = nullptr;
+  void* Xm = nullptr;      // A12 before its solve (pack of columns)
+  void* Ya = nullptr;      // A21 before its solve (pack of rows)
+  void* Tinv = nullptr;    // kLuPanel^2: the forward operator of panel a's diagonal block, row-major
+  void* Tinv2 = nullptr;   // the same for panel b
   void* Uinv = nullptr;    // kLuPanel^2: U_D^-1 by columns
   hipStream_t aux = nullptr;  // the look-ahead's stream (default priority)
   hipEvent_t next_cols = nullptr, next_done = nullptr;

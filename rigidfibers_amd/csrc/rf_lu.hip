@@ -4,8 +4,9 @@
 // Reference: magma_sgesv_gpu (src/simulation.cu:403-418) and LAPACK sgesv
 // (fortran/solve_system.incl:3): LU with row pivoting, then L y = P b, U x = y.
 //
-// Factorisation of a copy F of A (A itself is kept for the residuals), in
-// outer panels of kLuPanel = W columns.  Panel [K0, KE):
+// Factorisation of F, the FP32 system written into the factor storage (the
+// residuals come from the system's own form), in outer panels of
+// kLuPanel = W columns.  Panel [K0, KE):
 //   1. its W x W diagonal block D is factored in place, 64-column blocks at a
 //      time: diag (one wavefront, partial pivoting among the block's rows,
 //      the row order kept as a permutation perm[k0 + c] = source row of row
@@ -16,13 +17,15 @@
 //      applies them), by running the panel solves on the unit vectors;
 //   3. the panel's rows below D, L21 = A21 Uinv, and its rows of U for the
 //      trailing columns, U12 = Tfwd A12, each one MFMA GEMM with K = W from
-//      packed copies of A21 / A12 (written to F);
-//   4. the trailing update A22 -= L21 U12 (K = W) on MFMA from the packs of
-//      L21 and U12 in the mode's element type, each 32-row block of L21 and
-//      32-column block of U12 scaled by a power of two that puts its largest
-//      magnitude in [2^14, 2^15) (the update divides the scales out exactly).
-// A one-panel look-ahead overlaps steps 1-3 of panel p + 1 (aux stream) with
-// the trailing update of panel p on the columns right of panel p + 1.
+//      packed copies of A21 / A12 (written to F).
+// The panels are taken in pairs: the second panel's columns are updated by
+// the first (K = W), and the trailing matrix right of the pair once by both,
+// A22 -= [L21_a L21_b] [U12_a; U12_b] (K = 2W), on MFMA from packs in the
+// mode's element type, each 32-row block of the L21 pack and 32-column block
+// of the U12 pack scaled by a power of two that puts its largest magnitude in
+// [2^14, 2^15) (the update divides the scales out exactly).  A one-pair
+// look-ahead overlaps the next pair's panels (aux stream) with the trailing
+// update of the columns right of that pair.
 //
 // Pivoting is restricted to the 64-row diagonal blocks (the system's identity-
 // dominated diagonal blocks make that pivot choice safe; the refinement below
@@ -271,15 +274,16 @@ __device__ __forceinline__ void store8(T* dst, const float* v) {
   }
 }
 
-// Packs of W = kLuPanel values per row, K-contiguous, in the GEMM's element
-// type; one workgroup (256 threads) per 32 rows of the pack.  With scale
-// != nullptr each 32-row block is multiplied by block_scale(its largest
-// magnitude), which is stored in scale[block].
-//   rows of F:    pack row q = F[r0 + q, k0 .. k0 + W)   (strided in F: LDS transpose)
-//   columns of F: pack row q = F[k0 .. k0 + W, c0 + q]   (contiguous in F)
+// Packs of kw values per row (kw a multiple of 64: one or two panels),
+// K-contiguous, in the GEMM's element type; one workgroup (256 threads) per
+// 32 rows of the pack.  With scale != nullptr each 32-row block is multiplied
+// by block_scale(its largest magnitude over all kw values), which is stored in
+// scale[block].
+//   rows of F:    pack row q = F[r0 + q, k0 .. k0 + kw)   (strided in F: LDS transpose)
+//   columns of F: pack row q = F[k0 .. k0 + kw, c0 + q]   (contiguous in F)
 template <typename T>
 __global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const float* __restrict__ F, int64_t r0,
-                                                           int64_t r1, int64_t k0, T* __restrict__ dst,
+                                                           int64_t r1, int64_t k0, int kw, T* __restrict__ dst,
                                                            float* __restrict__ scale) {
   __shared__ float tile[32][64 + 1];
   __shared__ float red[8];
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const fl
   if (scale) {
     float m = 0.0f;
     if (live)
-      for (int k = grp; k < kLuPanel; k += 8) m = fmaxf(m, fabsf(F[(k0 + k) * lda + r]));
+      for (int k = grp; k < kw; k += 8) m = fmaxf(m, fabsf(F[(k0 + k) * lda + r]));
     for (int o = 16; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 32));
     if (lane32 == 0) red[grp] = m;
     __syncthreads();
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const fl
     if (tid == 0) scale[blockIdx.x] = s;
   }
   const int q = tid >> 3, part = tid & 7;  // writer: pack row q, 8 values at 8 part
-  for (int kc = 0; kc < kLuPanel; kc += 64) {
+  for (int kc = 0; kc < kw; kc += 64) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -313,14 +317,14 @@ __global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const fl
       float v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) v[i] = tile[q][part * 8 + i] * s;
-      store8<T>(dst + (rb - r0 + q) * kLuPanel + kc + part * 8, v);
+      store8<T>(dst + (rb - r0 + q) * kw + kc + part * 8, v);
     }
   }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void lu_pack_cols_kernel(int64_t lda, const float* __restrict__ F, int64_t c0,
-                                                           int64_t c1, int64_t k0, T* __restrict__ dst,
+                                                           int64_t c1, int64_t k0, int kw, T* __restrict__ dst,
                                                            float* __restrict__ scale) {
   __shared__ float red[8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -332,7 +336,7 @@ __global__ __launch_bounds__(256) void lu_pack_cols_kernel(int64_t lda, const fl
       const int64_t c = cb + q;
       if (c >= c1) break;
       const float4* col = reinterpret_cast<const float4*>(F + c * lda + k0);
-      for (int k = lane; k < kLuPanel / 4; k += 64) {
+      for (int k = lane; k < kw / 4; k += 64) {
         const float4 v = col[k];
         m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
       }
@@ -348,11 +352,11 @@ __global__ __launch_bounds__(256) void lu_pack_cols_kernel(int64_t lda, const fl
     const int64_t c = cb + q;
     if (c >= c1) break;
     const float* col = F + c * lda + k0;
-    for (int k = lane * 8; k < kLuPanel; k += 512) {
+    for (int k = lane * 8; k < kw; k += 512) {
       const float4 a = *reinterpret_cast<const float4*>(col + k);
       const float4 b = *reinterpret_cast<const float4*>(col + k + 4);
       const float v[8] = {a.x * s, a.y * s, a.z * s, a.w * s, b.x * s, b.y * s, b.z * s, b.w * s};
-      store8<T>(dst + (c - c0) * kLuPanel + k, v);
+      store8<T>(dst + (c - c0) * kw + k, v);
     }
   }
 }
@@ -588,30 +592,57 @@ static_assert(kInvCols == 8, "the inverse kernel stores 8 values per thread");
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
 
 // Blocked right-looking LU with pivoting inside the 64-wide diagonal blocks,
-// panels of W = kLuPanel columns and a one-panel look-ahead: the trailing
-// update of panel p first updates the next panel's columns, then the aux
-// stream factors that panel (its diagonal block, inverses and L21) while the
-// main stream updates the remaining columns.  The two streams touch disjoint
-// columns of F; the packs of L21 alternate between two buffers.
+// panels of W = kLuPanel columns taken in pairs, so that the trailing matrix
+// is updated once per pair, with K = 2W (half the C traffic per flop of one
+// update per panel).  Pair [K0, KB), panels a = [K0, KA) and b = [KA, KB):
+//   1. factor a (diagonal blocks, inverses, L21_a into F);
+//   2. U12_a over b's columns, then b's columns -= L21_a U12_a (K = W);
+//   3. factor b;
+// and, when columns remain right of the pair (main stream):
+//   4. U12_a over the trailing columns; b's rows of them -= L21_a U12_a (K = W);
+//   5. U12_b over the trailing columns;
+//   6. the trailing update A22 -= [L21_a L21_b] [U12_a; U12_b] (K = 2W), from
+//      packs of F's columns [K0, KB) and rows [K0, KB).
+// Look-ahead: the update first covers the next pair's columns; the aux stream
+// then runs steps 1-3 of that pair while the main stream updates the rest.
+// The two streams touch disjoint columns of F and disjoint buffers (steps 1-3
+// use Y, X, La, Bb and Ti[], steps 4-6 of the same pair finish before the
+// next pair's 1-3 start).
 template <typename T>
 static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
-  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
+  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel, D = 2 * kLuPanel;
   const bool scaled = mode != LuMode::kFP32;
   float* F = ws.F;
-  T* Ab[2] = {static_cast<T*>(ws.At), static_cast<T*>(ws.At2)};
-  float* sA[2] = {ws.sA, ws.sA2};
-  T* Bp = static_cast<T*>(ws.Bt);
-  T* X = static_cast<T*>(ws.Xm);   // A12 copy (main stream)
-  T* Y = static_cast<T*>(ws.Ya);   // A21 copy (aux stream)
-  T* Ti = static_cast<T*>(ws.Tinv);
+  T* Lj = static_cast<T*>(ws.At);   // [L21_a L21_b] rows right of the pair, D per row
+  T* Uj = static_cast<T*>(ws.Bt);   // [U12_a; U12_b] columns right of the pair, D per column
+  T* La = static_cast<T*>(ws.At2);  // L21_a, rows below panel a, W per row
+  T* Ua = static_cast<T*>(ws.Ua);   // U12_a over the trailing columns, W per column
+  T* Bb = static_cast<T*>(ws.Bb);   // U12_a over panel b's columns, W per column
+  T* X = static_cast<T*>(ws.Xm);    // A12 before its solve (pack of columns)
+  T* Y = static_cast<T*>(ws.Ya);    // A21 before its solve (pack of rows)
+  T* Ti[2] = {static_cast<T*>(ws.Tinv), static_cast<T*>(ws.Tinv2)};  // forward operators of a and b
   T* Ui = static_cast<T*>(ws.Uinv);
   auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
+  auto sc = [&](float* p) { return scaled ? p : nullptr; };
   hipError_t e = hipSuccess;
   int nev = 0;
-  // panel [K0, KE) on stream st: diagonal block, block inverses and, when
-  // columns remain, the inverses of D's factors, L21 into F and its pack
-  auto lower_part = [&](int64_t K0, int slot, hipStream_t st) -> hipError_t {
-    const int64_t KE = std::min<int64_t>(n, K0 + W);
+  // a GEMM on the main stream, timed and its flops counted
+  auto timed_sub = [&](int64_t m, int64_t nc, int64_t K, const T* A, const T* B, float* C, const float* sa,
+                       const float* sb) -> hipError_t {
+    const bool timed = nev < LuWorkspace::kEvents;
+    if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
+    hipError_t ee = gemm_packed(mode, false, m, nc, K, A, K, B, K, C, lda, s, sa, sb);
+    if (ee != hipSuccess) return ee;
+    if (timed) {
+      (void)hipEventRecord(ws.ev[2 * nev + 1], s);
+      ++nev;
+      res->gemm_flop += 2.0 * static_cast<double>(m) * static_cast<double>(nc) * static_cast<double>(K);
+    }
+    return hipSuccess;
+  };
+  // panel [K0, KE) on stream st: diagonal blocks, block inverses and, when
+  // rows remain below, the forward operator of D into Ti[slot] and L21 into F
+  auto factor_panel = [&](int64_t K0, int64_t KE, int slot, hipStream_t st) -> hipError_t {
     for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
       const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
       const int64_t k1 = k0 + kb;
@@ -628,56 +659,72 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
                        K0 / kLuNb, ws.Dinv);
     if (KE >= n) return hipGetLastError();
     hipLaunchKernelGGL((lu_panel_inverse_kernel<T, true>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
-                       ws.Dinv, K0, Ti);
+                       ws.Dinv, K0, Ti[slot]);
     hipLaunchKernelGGL((lu_panel_inverse_kernel<T, false>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
                        ws.Dinv, K0, Ui);
-    // L21 = A21 Uinv
-    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0, Y,
-                       nullptr);
+    // L21 = A21 U_D^-1
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
+                       static_cast<int>(W), Y, nullptr);
     hipError_t ee = gemm_packed(mode, true, n - KE, W, W, Y, W, Ui, W, at(KE, K0), lda, st);
     if (ee != hipSuccess) return ee;
-    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
-                       Ab[slot], scaled ? sA[slot] : nullptr);
     return hipGetLastError();
   };
-  // the panel's rows of U for the trailing columns, U12 = Tfwd A12, and its pack
-  auto upper_part = [&](int64_t K0) -> hipError_t {
-    const int64_t KE = K0 + W;
-    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, s, lda, F, KE, n, K0, X,
-                       nullptr);
-    hipError_t ee = gemm_packed(mode, true, W, n - KE, W, Ti, W, X, W, at(K0, KE), lda, s);
+  // U12 of the full panel [K0, K0 + W) over columns [c0, c1): Ti[slot] A12
+  auto form_u12 = [&](int64_t K0, int64_t c0, int64_t c1, int slot, hipStream_t st) -> hipError_t {
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(c1 - c0, 32)), dim3(256), 0, st, lda, F, c0, c1, K0,
+                       static_cast<int>(W), X, nullptr);
+    hipError_t ee = gemm_packed(mode, true, W, c1 - c0, W, Ti[slot], W, X, W, at(K0, c0), lda, st);
     if (ee != hipSuccess) return ee;
-    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, s, lda, F, KE, n, K0, Bp,
-                       scaled ? ws.sB : nullptr);
     return hipGetLastError();
   };
-  // C[KE:n, c0:c1] -= L[KE:n, K0:KE] U[K0:KE, c0:c1], K = W (timed)
-  auto trailing = [&](int64_t KE, int64_t c0, int64_t c1, int slot) -> hipError_t {
-    const bool timed = nev < LuWorkspace::kEvents;
-    if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-    hipError_t ee = gemm_packed(mode, false, n - KE, c1 - c0, W, Ab[slot], W, Bp + (c0 - KE) * W, W, at(KE, c0),
-                                lda, s, scaled ? sA[slot] : nullptr, scaled ? ws.sB + (c0 - KE) / 32 : nullptr);
+  // steps 1-3 of the pair at K0 on stream st
+  auto pair_panels = [&](int64_t K0, hipStream_t st) -> hipError_t {
+    const int64_t KA = std::min<int64_t>(n, K0 + W), KB = std::min<int64_t>(n, K0 + D);
+    hipError_t ee = factor_panel(K0, KA, 0, st);
+    if (ee != hipSuccess || KA >= n) return ee;
+    if ((ee = form_u12(K0, KA, KB, 0, st)) != hipSuccess) return ee;
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KA, 32)), dim3(256), 0, st, lda, F, KA, n, K0,
+                       static_cast<int>(W), La, sc(ws.sA2));
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(KB - KA, 32)), dim3(256), 0, st, lda, F, KA, KB, K0,
+                       static_cast<int>(W), Bb, sc(ws.sBb));
+    ee = gemm_packed(mode, false, n - KA, KB - KA, W, La, W, Bb, W, at(KA, KA), lda, st, sc(ws.sA2), sc(ws.sBb));
     if (ee != hipSuccess) return ee;
-    if (timed) {
-      (void)hipEventRecord(ws.ev[2 * nev + 1], s);
-      ++nev;
-      res->gemm_flop += 2.0 * static_cast<double>(n - KE) * static_cast<double>(c1 - c0) * static_cast<double>(W);
-    }
-    return hipSuccess;
+    return factor_panel(KA, KB, 1, st);
   };
-  if ((e = lower_part(0, 0, s)) != hipSuccess) return e;
-  for (int64_t K0 = 0, p = 0; K0 + W < n; K0 += W, ++p) {
-    const int64_t KE = K0 + W;
-    if ((e = upper_part(K0)) != hipSuccess) return e;
-    // trailing update: the next panel's columns, then (look-ahead) that panel
-    // on the aux stream while the main stream updates the rest
-    const int64_t NE = std::min<int64_t>(n, KE + W);
-    if ((e = trailing(KE, KE, NE, p & 1)) != hipSuccess) return e;
+  // steps 4-6 of the pair at K0 (main stream), up to the packs of the update
+  auto pair_prepare = [&](int64_t K0) -> hipError_t {
+    const int64_t KA = K0 + W, KB = K0 + D;
+    hipError_t ee = form_u12(K0, KB, n, 0, s);
+    if (ee != hipSuccess) return ee;
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
+                       static_cast<int>(W), Ua, sc(ws.sUa));
+    // b's rows of the trailing columns: the first W rows of La are b's rows
+    if ((ee = timed_sub(W, n - KB, W, La, Ua, at(KA, KB), sc(ws.sA2), sc(ws.sUa))) != hipSuccess) return ee;
+    if ((ee = form_u12(KA, KB, n, 1, s)) != hipSuccess) return ee;
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
+                       static_cast<int>(D), Lj, sc(ws.sA));
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
+                       static_cast<int>(D), Uj, sc(ws.sB));
+    return hipGetLastError();
+  };
+  // A22[KB:n, c0:c1] -= Lj Uj[c0:c1], K = 2W
+  auto trailing = [&](int64_t KB, int64_t c0, int64_t c1) -> hipError_t {
+    return timed_sub(n - KB, c1 - c0, D, Lj, Uj + (c0 - KB) * D, at(KB, c0), sc(ws.sA),
+                     scaled ? ws.sB + (c0 - KB) / 32 : nullptr);
+  };
+  if ((e = pair_panels(0, s)) != hipSuccess) return e;
+  for (int64_t K0 = 0; K0 + D < n; K0 += D) {
+    const int64_t KB = K0 + D;
+    if ((e = pair_prepare(K0)) != hipSuccess) return e;
+    // the next pair's columns, then (look-ahead) that pair on the aux stream
+    // while the main stream updates the rest
+    const int64_t NE = std::min<int64_t>(n, KB + D);
+    if ((e = trailing(KB, KB, NE)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_cols, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(ws.aux, ws.next_cols, 0)) != hipSuccess) return e;
-    if ((e = lower_part(KE, (p + 1) & 1, ws.aux)) != hipSuccess) return e;
+    if ((e = pair_panels(KB, ws.aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
-    if (NE < n && (e = trailing(KE, NE, n, p & 1)) != hipSuccess) return e;
+    if (NE < n && (e = trailing(KB, NE, n)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
   res->gemm_launches = nev;
@@ -731,11 +778,13 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   if ((e = hipMalloc(&ws->perm, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->info, sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
-  for (void** p : {&ws->At, &ws->At2, &ws->Bt, &ws->Xm, &ws->Ya})
+  for (void** p : {&ws->At, &ws->Bt})  // two panels wide
+    if ((e = hipMalloc(p, 2 * pack)) != hipSuccess) return e;
+  for (void** p : {&ws->At2, &ws->Ua, &ws->Xm, &ws->Ya})
     if ((e = hipMalloc(p, pack)) != hipSuccess) return e;
-  for (void** p : {&ws->Tinv, &ws->Uinv})
+  for (void** p : {&ws->Bb, &ws->Tinv, &ws->Tinv2, &ws->Uinv})
     if ((e = hipMalloc(p, inv)) != hipSuccess) return e;
-  for (float** p : {&ws->sA, &ws->sA2, &ws->sB})
+  for (float** p : {&ws->sA, &ws->sB, &ws->sA2, &ws->sUa, &ws->sBb})
     if ((e = hipMalloc(p, scales)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&ws->next_cols, hipEventDisableTiming)) != hipSuccess) return e;
@@ -744,7 +793,7 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n) + 5 * pack + 2 * inv + 3 * scales +
+                                   sizeof(double) * 3 * static_cast<size_t>(n) + 8 * pack + 4 * inv + 5 * scales +
                                    dinv);
   return hipSuccess;
 }
@@ -754,8 +803,9 @@ int64_t lu_workspace_bytes(const LuWorkspace& ws) { return ws.bytes; }
 void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->aux) (void)hipStreamSynchronize(ws->aux);
   for (void* p : {static_cast<void*>(ws->F), static_cast<void*>(ws->perm), static_cast<void*>(ws->info),
-                  static_cast<void*>(ws->vec), ws->At, ws->At2, ws->Bt, ws->Xm, ws->Ya, ws->Tinv, ws->Uinv,
-                  static_cast<void*>(ws->sA), static_cast<void*>(ws->sA2), static_cast<void*>(ws->sB),
+                  static_cast<void*>(ws->vec), ws->At, ws->At2, ws->Bt, ws->Ua, ws->Bb, ws->Xm, ws->Ya, ws->Tinv,
+                  ws->Tinv2, ws->Uinv, static_cast<void*>(ws->sA), static_cast<void*>(ws->sA2),
+                  static_cast<void*>(ws->sB), static_cast<void*>(ws->sUa), static_cast<void*>(ws->sBb),
                   static_cast<void*>(ws->Dinv)})
     if (p) (void)hipFree(p);
   if (ws->next_cols) (void)hipEventDestroy(ws->next_cols);
