@@ -136,14 +136,16 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restr
   if (lane == 0 && zero_col != 0 && info[0] == 0) info[0] = static_cast<int>(k0) + zero_col;
 }
 
-// l21: rows [r0, r1), lane = row: x U11 = a, written to F.
+// l21: rows [r0, r1), lane = row: x U11 = a, written to F.  Column-oriented
+// substitution: once x[j] is final, its multiple of U's row j leaves every
+// later x[c] (independent FMAs rather than one dependent chain per entry).
 __global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
                                                      int64_t k0) {
-  __shared__ float Ut[kLuNb][kLuNb];  // Ut[c][j] = U[j][c]
+  __shared__ __attribute__((aligned(16))) float Ur[kLuNb][kLuNb + 4];  // Ur[j][c] = U[j][c], 16-byte rows
   __shared__ float rd[kLuNb];
   for (int e = threadIdx.x; e < kLuNb * kLuNb; e += 256) {
     const int c = e >> 6, j = e & 63;
-    Ut[c][j] = F[(k0 + c) * lda + k0 + j];
+    Ur[j][c] = F[(k0 + c) * lda + k0 + j];
   }
   if (threadIdx.x < kLuNb) {
     const float d = F[(k0 + threadIdx.x) * lda + k0 + threadIdx.x];
@@ -156,11 +158,11 @@ __global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int
 #pragma unroll
   for (int c = 0; c < kLuNb; ++c) x[c] = F[(k0 + c) * lda + r];
 #pragma unroll
-  for (int c = 0; c < kLuNb; ++c) {
-    float v = x[c];
+  for (int j = 0; j < kLuNb; ++j) {
+    const float xj = x[j] * rd[j];
+    x[j] = xj;
 #pragma unroll
-    for (int j = 0; j < c; ++j) v = fmaf(-x[j], Ut[c][j], v);
-    x[c] = v * rd[c];
+    for (int c = j + 1; c < kLuNb; ++c) x[c] = fmaf(-xj, Ur[j][c], x[c]);
   }
 #pragma unroll
   for (int c = 0; c < kLuNb; ++c) F[(k0 + c) * lda + r] = x[c];
@@ -172,11 +174,11 @@ __global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int
 __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
                                                     int64_t k0, const int* __restrict__ perm) {
   __shared__ float S[kLuNb][kLuNb + 1];   // [column][row]
-  __shared__ __attribute__((aligned(16))) float Lr[kLuNb][kLuNb + 4];  // Lr[r][j] = L[r][j], 16-byte rows
+  __shared__ __attribute__((aligned(16))) float Lc[kLuNb][kLuNb + 4];  // Lc[j][r] = L[r][j], 16-byte rows
   const int lane = threadIdx.x;
   const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * kLuNb;
 #pragma unroll 8
-  for (int j = 0; j < kLuNb; ++j) Lr[lane][j] = F[(k0 + j) * lda + k0 + lane];
+  for (int j = 0; j < kLuNb; ++j) Lc[j][lane] = F[(k0 + j) * lda + k0 + lane];
   const int src = min(max(perm[k0 + lane] - static_cast<int>(k0), 0), kLuNb - 1);
 #pragma unroll 8
   for (int cc = 0; cc < kLuNb; ++cc) {
@@ -184,15 +186,16 @@ __global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int6
     S[cc][lane] = (col < c1) ? F[col * lda + k0 + src] : 0.0f;
   }
   __syncthreads();
+  // column-oriented forward substitution: x[j] final, then its multiple of
+  // L's column j leaves every later x[r]
   float x[kLuNb];
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) x[r] = S[lane][r];
 #pragma unroll
-  for (int r = 1; r < kLuNb; ++r) {
-    float v = x[r];
+  for (int j = 0; j < kLuNb - 1; ++j) {
+    const float xj = x[j];
 #pragma unroll
-    for (int j = 0; j < r; ++j) v = fmaf(-Lr[r][j], x[j], v);
-    x[r] = v;
+    for (int r = j + 1; r < kLuNb; ++r) x[r] = fmaf(-Lc[j][r], xj, x[r]);
   }
 #pragma unroll
   for (int r = 0; r < kLuNb; ++r) S[lane][r] = x[r];
