@@ -526,17 +526,18 @@ __global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const f
 //        row-major: out[r W + k] = Tfwd[r][k] (the A operand of U12 = Tfwd A12);
 //   BWD: Uinv = U_D^-1, written by columns: out[c W + r] = Uinv[r][c] (the
 //        B operand of L21 = A21 Uinv).
+// Both come from one launch: workgroups [0, W / kInvCols) form Tfwd, the
+// rest Uinv (the two sweeps are independent).
 constexpr int kInvCols = 8;
 template <typename T, bool FWD>
-__global__ __launch_bounds__(256) void lu_panel_inverse_kernel(int64_t lda, const float* __restrict__ F,
-                                                               const int* __restrict__ perm,
-                                                               const double* __restrict__ Dinv, int64_t K0,
-                                                               T* __restrict__ out) {
+__device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __restrict__ F,
+                                                    const int* __restrict__ perm, const double* __restrict__ Dinv,
+                                                    int64_t K0, int blk, T* __restrict__ out,
+                                                    float (*__restrict__ ys)[kLuPanel],
+                                                    float (*__restrict__ tb)[kLuNb]) {
   constexpr int W = kLuPanel;
-  __shared__ __attribute__((aligned(16))) float ys[kInvCols][W];
-  __shared__ float tb[kInvCols][kLuNb];
   const int tid = threadIdx.x;
-  const int j0 = blockIdx.x * kInvCols;
+  const int j0 = blk * kInvCols;
   for (int q = tid; q < kInvCols * W; q += 256) {
     const int c = q / W, r = q % W;
     ys[c][r] = (r == j0 + c) ? 1.0f : 0.0f;
@@ -591,6 +592,20 @@ __global__ __launch_bounds__(256) void lu_panel_inverse_kernel(int64_t lda, cons
   }
 }
 static_assert(kInvCols == 8, "the inverse kernel stores 8 values per thread");
+
+template <typename T>
+__global__ __launch_bounds__(256) void lu_panel_inverses_kernel(int64_t lda, const float* __restrict__ F,
+                                                                const int* __restrict__ perm,
+                                                                const double* __restrict__ Dinv, int64_t K0,
+                                                                T* __restrict__ tfwd, T* __restrict__ uinv) {
+  constexpr int G = kLuPanel / kInvCols;  // workgroups per inverse
+  __shared__ __attribute__((aligned(16))) float ys[kInvCols][kLuPanel];  // the workgroup's vectors
+  __shared__ float tb[kInvCols][kLuNb];
+  if (static_cast<int>(blockIdx.x) < G)
+    panel_inverse_block<T, true>(lda, F, perm, Dinv, K0, blockIdx.x, tfwd, ys, tb);
+  else
+    panel_inverse_block<T, false>(lda, F, perm, Dinv, K0, blockIdx.x - G, uinv, ys, tb);
+}
 
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
 
@@ -661,10 +676,8 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(KE - K0, kLuNb)), dim3(64), 0, st, n, lda, F,
                        K0 / kLuNb, ws.Dinv);
     if (KE >= n) return hipGetLastError();
-    hipLaunchKernelGGL((lu_panel_inverse_kernel<T, true>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
-                       ws.Dinv, K0, Ti[slot]);
-    hipLaunchKernelGGL((lu_panel_inverse_kernel<T, false>), dim3(W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
-                       ws.Dinv, K0, Ui);
+    hipLaunchKernelGGL((lu_panel_inverses_kernel<T>), dim3(2 * W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
+                       ws.Dinv, K0, Ti[slot], Ui);
     // L21 = A21 U_D^-1
     hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
                        static_cast<int>(W), Y, nullptr);
