@@ -136,74 +136,67 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restr
   if (lane == 0 && zero_col != 0 && info[0] == 0) info[0] = static_cast<int>(k0) + zero_col;
 }
 
-// l21: rows [r0, r1), lane = row: x U11 = a, written to F.  Column-oriented
-// substitution: once x[j] is final, its multiple of U's row j leaves every
-// later x[c] (independent FMAs rather than one dependent chain per entry).
-__global__ __launch_bounds__(256) void lu_l21_kernel(int64_t r0, int64_t r1, int64_t lda, float* __restrict__ F,
-                                                     int64_t k0) {
-  __shared__ __attribute__((aligned(16))) float Ur[kLuNb][kLuNb + 4];  // Ur[j][c] = U[j][c], 16-byte rows
+// l21 and u12 of the 64-column block at k0, one launch: workgroups
+// [0, nl) solve x U11 = a for 256 rows of [k1, e) each (lane = row; the
+// block's columns of F), the rest L11 y = P a for 256 columns of [k1, e)
+// each (lane = column; the block's rows of F gathered by its permutation,
+// L11 unit lower).  The two parts write disjoint parts of F and read only the
+// factored block besides their own entries.  Column-oriented substitution:
+// once an entry is final, its multiple of U's row (L's column) leaves every
+// later entry (independent FMAs rather than one dependent chain per entry).
+__global__ __launch_bounds__(256) void lu_l21_u12_kernel(int64_t k1, int64_t e, int64_t lda, float* __restrict__ F,
+                                                         int64_t k0, const int* __restrict__ perm, int nl) {
+  // l21: Mt[j][c] = U[j][c];  u12: Mt[j][r] = L[r][j]  (16-byte rows)
+  __shared__ __attribute__((aligned(16))) float Mt[kLuNb][kLuNb + 4];
   __shared__ float rd[kLuNb];
-  for (int e = threadIdx.x; e < kLuNb * kLuNb; e += 256) {
-    const int c = e >> 6, j = e & 63;
-    Ur[j][c] = F[(k0 + c) * lda + k0 + j];
+  __shared__ int src[kLuNb];
+  const int tid = threadIdx.x;
+  const bool lpart = static_cast<int>(blockIdx.x) < nl;
+  for (int q = tid; q < kLuNb * kLuNb; q += 256) {
+    const int hi = q >> 6, lo = q & 63;
+    if (lpart)
+      Mt[lo][hi] = F[(k0 + hi) * lda + k0 + lo];  // U[lo][hi]
+    else
+      Mt[hi][lo] = F[(k0 + hi) * lda + k0 + lo];  // L[lo][hi]
   }
-  if (threadIdx.x < kLuNb) {
-    const float d = F[(k0 + threadIdx.x) * lda + k0 + threadIdx.x];
-    rd[threadIdx.x] = (d != 0.0f) ? 1.0f / d : 0.0f;
+  if (tid < kLuNb) {
+    if (lpart) {
+      const float d = F[(k0 + tid) * lda + k0 + tid];
+      rd[tid] = (d != 0.0f) ? 1.0f / d : 0.0f;
+    } else {
+      src[tid] = min(max(perm[k0 + tid] - static_cast<int>(k0), 0), kLuNb - 1);
+    }
   }
   __syncthreads();
-  const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (r >= r1) return;
   float x[kLuNb];
+  if (lpart) {
+    const int64_t r = k1 + static_cast<int64_t>(blockIdx.x) * 256 + tid;
+    if (r >= e) return;  // no barrier follows
 #pragma unroll
-  for (int c = 0; c < kLuNb; ++c) x[c] = F[(k0 + c) * lda + r];
+    for (int c = 0; c < kLuNb; ++c) x[c] = F[(k0 + c) * lda + r];
 #pragma unroll
-  for (int j = 0; j < kLuNb; ++j) {
-    const float xj = x[j] * rd[j];
-    x[j] = xj;
+    for (int j = 0; j < kLuNb; ++j) {
+      const float xj = x[j] * rd[j];
+      x[j] = xj;
 #pragma unroll
-    for (int c = j + 1; c < kLuNb; ++c) x[c] = fmaf(-xj, Ur[j][c], x[c]);
-  }
+      for (int c = j + 1; c < kLuNb; ++c) x[c] = fmaf(-xj, Mt[j][c], x[c]);
+    }
 #pragma unroll
-  for (int c = 0; c < kLuNb; ++c) F[(k0 + c) * lda + r] = x[c];
-}
-
-// u12: columns [c0, c1), 64 per workgroup (lane = column for the solve): rows
-// k0.. of the block gathered by its permutation, L11 y = a (unit lower),
-// written to F.
-__global__ __launch_bounds__(64) void lu_u12_kernel(int64_t c0, int64_t c1, int64_t lda, float* __restrict__ F,
-                                                    int64_t k0, const int* __restrict__ perm) {
-  __shared__ float S[kLuNb][kLuNb + 1];   // [column][row]
-  __shared__ __attribute__((aligned(16))) float Lc[kLuNb][kLuNb + 4];  // Lc[j][r] = L[r][j], 16-byte rows
-  const int lane = threadIdx.x;
-  const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * kLuNb;
-#pragma unroll 8
-  for (int j = 0; j < kLuNb; ++j) Lc[j][lane] = F[(k0 + j) * lda + k0 + lane];
-  const int src = min(max(perm[k0 + lane] - static_cast<int>(k0), 0), kLuNb - 1);
-#pragma unroll 8
-  for (int cc = 0; cc < kLuNb; ++cc) {
-    const int64_t col = cb + cc;
-    S[cc][lane] = (col < c1) ? F[col * lda + k0 + src] : 0.0f;
-  }
-  __syncthreads();
-  // column-oriented forward substitution: x[j] final, then its multiple of
-  // L's column j leaves every later x[r]
-  float x[kLuNb];
+    for (int c = 0; c < kLuNb; ++c) F[(k0 + c) * lda + r] = x[c];
+  } else {
+    const int64_t c = k1 + static_cast<int64_t>(blockIdx.x - nl) * 256 + tid;
+    if (c >= e) return;  // no barrier follows
+    float* col = F + c * lda + k0;
 #pragma unroll
-  for (int r = 0; r < kLuNb; ++r) x[r] = S[lane][r];
+    for (int r = 0; r < kLuNb; ++r) x[r] = col[src[r]];
 #pragma unroll
-  for (int j = 0; j < kLuNb - 1; ++j) {
-    const float xj = x[j];
+    for (int j = 0; j < kLuNb - 1; ++j) {
+      const float xj = x[j];
 #pragma unroll
-    for (int r = j + 1; r < kLuNb; ++r) x[r] = fmaf(-Lc[j][r], xj, x[r]);
-  }
+      for (int r = j + 1; r < kLuNb; ++r) x[r] = fmaf(-Mt[j][r], xj, x[r]);
+    }
 #pragma unroll
-  for (int r = 0; r < kLuNb; ++r) S[lane][r] = x[r];
-  __syncthreads();
-#pragma unroll 8
-  for (int cc = 0; cc < kLuNb; ++cc) {
-    const int64_t col = cb + cc;
-    if (col < c1) F[col * lda + k0 + lane] = S[cc][lane];
+    for (int r = 0; r < kLuNb; ++r) col[r] = x[r];
   }
 }
 
@@ -666,9 +659,8 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
       const int64_t k1 = k0 + kb;
       hipLaunchKernelGGL(lu_diag_kernel, dim3(1), dim3(64), 0, st, lda, F, k0, kb, ws.perm, ws.info);
       if (k1 < KE) {  // a full block (only the last block of the matrix is short)
-        hipLaunchKernelGGL(lu_l21_kernel, dim3(grid_of(KE - k1, 256)), dim3(256), 0, st, k1, KE, lda, F, k0);
-        hipLaunchKernelGGL(lu_u12_kernel, dim3(grid_of(KE - k1, kLuNb)), dim3(64), 0, st, k1, KE, lda, F, k0,
-                           ws.perm);
+        const int nl = static_cast<int>(grid_of(KE - k1, 256));
+        hipLaunchKernelGGL(lu_l21_u12_kernel, dim3(2 * nl), dim3(256), 0, st, k1, KE, lda, F, k0, ws.perm, nl);
         hipLaunchKernelGGL(lu_dupdate_kernel, dim3(grid_of(KE - k1, kLuNb), grid_of(KE - k1, kLuNb)), dim3(256), 0,
                            st, lda, F, k0, k1, KE);
       }
