@@ -15,7 +15,12 @@ namespace rf {
 enum class LuMode { kFP32, kBF16, kFP16 };
 
 constexpr int kLuNb = 64;       // diagonal block: pivoting happens inside it
-constexpr int kLuPanel = 1024;  // outer panel: the K of the trailing MFMA update
+constexpr int kLuPanel = 1024;  // outer panel
+// Panels per trailing update (its K = kLuGroup kLuPanel).  Measured at
+// n = 75 000 (BF16): 2 -> 555 ms per step (trailing GEMM 710 TFLOP/s),
+// 4 -> 564 ms (767 TFLOP/s; the longer in-group chains on the look-ahead
+// stream cost more than the faster update saves).
+constexpr int kLuGroup = 2;
 
 struct LuWorkspace {
   LuMode mode = LuMode::kFP32;
@@ -25,21 +30,20 @@ struct LuWorkspace {
   int* info = nullptr;     // device: first zero pivot (1-based) or 0
   double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
   // packs, K-contiguous, in the mode's element type (per-32-row / per-32-column
-  // scales in the mixed modes); panels are taken in pairs a, b (rf_lu.hip)
-  void* At = nullptr;      // [L21_a L21_b], rows right of the pair: 2 kLuPanel per row, scales sA
+  // scales in the mixed modes); panels are taken in groups of kLuGroup (rf_lu.hip)
+  void* At = nullptr;      // L21 of the group's panels, rows right of the group: kLuGroup kLuPanel per row
   float* sA = nullptr;
-  void* Bt = nullptr;      // [U12_a; U12_b], columns right of the pair: 2 kLuPanel per column, scales sB
+  void* Bt = nullptr;      // U12 of the group's panels, columns right of the group: kLuGroup kLuPanel per column
   float* sB = nullptr;
-  void* At2 = nullptr;     // L21_a, rows below panel a: kLuPanel per row, scales sA2
-  float* sA2 = nullptr;
-  void* Ua = nullptr;      // U12_a over the columns right of the pair, scales sUa
-  float* sUa = nullptr;
-  void* Bb = nullptr;      // U12_a over panel b's columns (at most kLuPanel), scales sBb
-  float* sBb = nullptr;
+  // the group's own updates (at most kLuGroup - 1 panels deep): rows and
+  // columns packs, one pair for the panels' stream (0), one for the main (1)
+  void* Pr[2] = {};        // rows: n (0) / kLuPanel (1) of them
+  void* Pc[2] = {};        // columns: kLuPanel (0) / n (1) of them
+  float* sPr[2] = {};
+  float* sPc[2] = {};
   void* Xm = nullptr;      // A12 before its solve (pack of columns)
   void* Ya = nullptr;      // A21 before its solve (pack of rows)
-  void* Tinv = nullptr;    // kLuPanel^2: the forward operator of panel a's diagonal block, row-major
-  void* Tinv2 = nullptr;   // the same for panel b
+  void* Tinv = nullptr;    // kLuGroup x kLuPanel^2: the forward operators of the group's diagonal blocks
   void* Uinv = nullptr;    // kLuPanel^2: U_D^-1 by columns
   hipStream_t aux = nullptr;  // the look-ahead's stream (default priority)
   hipEvent_t next_cols = nullptr, next_done = nullptr;

@@ -18,14 +18,15 @@
 //   3. the panel's rows below D, L21 = A21 Uinv, and its rows of U for the
 //      trailing columns, U12 = Tfwd A12, each one MFMA GEMM with K = W from
 //      packed copies of A21 / A12 (written to F).
-// The panels are taken in pairs: the second panel's columns are updated by
-// the first (K = W), and the trailing matrix right of the pair once by both,
-// A22 -= [L21_a L21_b] [U12_a; U12_b] (K = 2W), on MFMA from packs in the
-// mode's element type, each 32-row block of the L21 pack and 32-column block
-// of the U12 pack scaled by a power of two that puts its largest magnitude in
-// [2^14, 2^15) (the update divides the scales out exactly).  A one-pair
-// look-ahead overlaps the next pair's panels (aux stream) with the trailing
-// update of the columns right of that pair.
+// The panels are taken in groups of kLuGroup: within a group each panel's
+// columns are updated by the group's earlier panels (left-looking), and the
+// trailing matrix right of the group once by all of them, A22 -= L21 U12
+// with K = kLuGroup W, on MFMA from packs in the mode's element type, each
+// 32-row block of the L21 pack and 32-column block of the U12 pack scaled by
+// a power of two that puts its largest magnitude in [2^14, 2^15) (the update
+// divides the scales out exactly).  A one-group look-ahead overlaps the next
+// group's panels (aux stream) with the trailing update of the columns right
+// of that group.
 //
 // Pivoting is restricted to the 64-row diagonal blocks (the system's identity-
 // dominated diagonal blocks make that pivot choice safe; the refinement below
@@ -603,46 +604,47 @@ __global__ __launch_bounds__(256) void lu_panel_inverses_kernel(int64_t lda, con
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
 
 // Blocked right-looking LU with pivoting inside the 64-wide diagonal blocks,
-// panels of W = kLuPanel columns taken in pairs, so that the trailing matrix
-// is updated once per pair, with K = 2W (half the C traffic per flop of one
-// update per panel).  Pair [K0, KB), panels a = [K0, KA) and b = [KA, KB):
-//   1. factor a (diagonal blocks, inverses, L21_a into F);
-//   2. U12_a over b's columns, then b's columns -= L21_a U12_a (K = W);
-//   3. factor b;
-// and, when columns remain right of the pair (main stream):
-//   4. U12_a over the trailing columns; b's rows of them -= L21_a U12_a (K = W);
-//   5. U12_b over the trailing columns;
-//   6. the trailing update A22 -= [L21_a L21_b] [U12_a; U12_b] (K = 2W), from
-//      packs of F's columns [K0, KB) and rows [K0, KB).
-// Look-ahead: the update first covers the next pair's columns; the aux stream
-// then runs steps 1-3 of that pair while the main stream updates the rest.
-// The two streams touch disjoint columns of F and disjoint buffers (steps 1-3
-// use Y, X, La, Bb and Ti[], steps 4-6 of the same pair finish before the
-// next pair's 1-3 start).
+// panels of W = kLuPanel columns taken in groups of G = kLuGroup, so that the
+// trailing matrix is updated once per group, with K = G W (the C traffic per
+// flop of one update per panel, divided by G).  Group [K0, KG), panels
+// i = 0 .. G-1 at K_i = K0 + i W, each brought up to date left-looking:
+//   for each earlier panel j: its rows of panel i's columns -= the panels
+//   before j (K = j W), then U12_j over panel i's columns; panel i's columns
+//   below -= the panels before i (K = i W); factor panel i (diagonal blocks,
+//   inverses, L21 into F).
+// When columns remain right of the group (main stream): for each panel j, its
+// rows of the trailing columns -= the panels before j, then U12_j over them;
+// then the trailing update A22 -= L21 U12 with K = G W, from packs of F's
+// columns [K0, KG) and rows [K0, KG).
+// Look-ahead: the update first covers the next group's columns; the aux
+// stream then brings that group's panels up to date while the main stream
+// updates the rest.  The two streams touch disjoint columns of F and disjoint
+// buffers (the group's panels use Y, X, Ti[], Ui and the update packs of slot
+// 0; the main stream's preparation of the same group, which uses X, Ti[] and
+// slot 1, finishes before the next group's panels start).
 template <typename T>
 static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
-  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel, D = 2 * kLuPanel;
+  constexpr int G = kLuGroup;
+  const int64_t n = ws.n, lda = ws.lda, W = kLuPanel, GW = G * kLuPanel;
   const bool scaled = mode != LuMode::kFP32;
   float* F = ws.F;
-  T* Lj = static_cast<T*>(ws.At);   // [L21_a L21_b] rows right of the pair, D per row
-  T* Uj = static_cast<T*>(ws.Bt);   // [U12_a; U12_b] columns right of the pair, D per column
-  T* La = static_cast<T*>(ws.At2);  // L21_a, rows below panel a, W per row
-  T* Ua = static_cast<T*>(ws.Ua);   // U12_a over the trailing columns, W per column
-  T* Bb = static_cast<T*>(ws.Bb);   // U12_a over panel b's columns, W per column
-  T* X = static_cast<T*>(ws.Xm);    // A12 before its solve (pack of columns)
-  T* Y = static_cast<T*>(ws.Ya);    // A21 before its solve (pack of rows)
-  T* Ti[2] = {static_cast<T*>(ws.Tinv), static_cast<T*>(ws.Tinv2)};  // forward operators of a and b
+  T* Lj = static_cast<T*>(ws.At);
+  T* Uj = static_cast<T*>(ws.Bt);
+  T* X = static_cast<T*>(ws.Xm);
+  T* Y = static_cast<T*>(ws.Ya);
   T* Ui = static_cast<T*>(ws.Uinv);
+  auto Ti = [&](int slot) { return static_cast<T*>(ws.Tinv) + static_cast<int64_t>(slot) * W * W; };
   auto at = [&](int64_t r, int64_t c) { return F + c * lda + r; };
   auto sc = [&](float* p) { return scaled ? p : nullptr; };
   hipError_t e = hipSuccess;
   int nev = 0;
-  // a GEMM on the main stream, timed and its flops counted
-  auto timed_sub = [&](int64_t m, int64_t nc, int64_t K, const T* A, const T* B, float* C, const float* sa,
-                       const float* sb) -> hipError_t {
-    const bool timed = nev < LuWorkspace::kEvents;
+  // C -= A B on stream st; the updates of the columns right of a group (main
+  // stream, `count`) are timed and their flops counted
+  auto sub = [&](int64_t m, int64_t nc, int64_t K, const T* A, const T* B, float* C, const float* sa,
+                 const float* sb, hipStream_t st, bool count) -> hipError_t {
+    const bool timed = count && nev < LuWorkspace::kEvents;
     if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-    hipError_t ee = gemm_packed(mode, false, m, nc, K, A, K, B, K, C, lda, s, sa, sb);
+    hipError_t ee = gemm_packed(mode, false, m, nc, K, A, K, B, K, C, lda, st, sa, sb);
     if (ee != hipSuccess) return ee;
     if (timed) {
       (void)hipEventRecord(ws.ev[2 * nev + 1], s);
@@ -651,8 +653,19 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     }
     return hipSuccess;
   };
+  // F[r0:r1, c0:c1] -= F[r0:r1, K0:K0+kw] F[K0:K0+kw, c0:c1] from the packs of slot ps
+  auto update = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1, int64_t K0, int64_t kw, int ps,
+                    hipStream_t st, bool count) -> hipError_t {
+    T* Pr = static_cast<T*>(ws.Pr[ps]);
+    T* Pc = static_cast<T*>(ws.Pc[ps]);
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(r1 - r0, 32)), dim3(256), 0, st, lda, F, r0, r1, K0,
+                       static_cast<int>(kw), Pr, sc(ws.sPr[ps]));
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(c1 - c0, 32)), dim3(256), 0, st, lda, F, c0, c1, K0,
+                       static_cast<int>(kw), Pc, sc(ws.sPc[ps]));
+    return sub(r1 - r0, c1 - c0, kw, Pr, Pc, at(r0, c0), sc(ws.sPr[ps]), sc(ws.sPc[ps]), st, count);
+  };
   // panel [K0, KE) on stream st: diagonal blocks, block inverses and, when
-  // rows remain below, the forward operator of D into Ti[slot] and L21 into F
+  // rows remain below, the forward operator of D into Ti(slot) and L21 into F
   auto factor_panel = [&](int64_t K0, int64_t KE, int slot, hipStream_t st) -> hipError_t {
     for (int64_t k0 = K0; k0 < KE; k0 += kLuNb) {
       const int kb = static_cast<int>(std::min<int64_t>(kLuNb, KE - k0));
@@ -669,7 +682,7 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
                        K0 / kLuNb, ws.Dinv);
     if (KE >= n) return hipGetLastError();
     hipLaunchKernelGGL((lu_panel_inverses_kernel<T>), dim3(2 * W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
-                       ws.Dinv, K0, Ti[slot], Ui);
+                       ws.Dinv, K0, Ti(slot), Ui);
     // L21 = A21 U_D^-1
     hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
                        static_cast<int>(W), Y, nullptr);
@@ -677,62 +690,65 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     if (ee != hipSuccess) return ee;
     return hipGetLastError();
   };
-  // U12 of the full panel [K0, K0 + W) over columns [c0, c1): Ti[slot] A12
+  // U12 of the full panel [K0, K0 + W) over columns [c0, c1): Ti(slot) A12
   auto form_u12 = [&](int64_t K0, int64_t c0, int64_t c1, int slot, hipStream_t st) -> hipError_t {
     hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(c1 - c0, 32)), dim3(256), 0, st, lda, F, c0, c1, K0,
                        static_cast<int>(W), X, nullptr);
-    hipError_t ee = gemm_packed(mode, true, W, c1 - c0, W, Ti[slot], W, X, W, at(K0, c0), lda, st);
+    hipError_t ee = gemm_packed(mode, true, W, c1 - c0, W, Ti(slot), W, X, W, at(K0, c0), lda, st);
     if (ee != hipSuccess) return ee;
     return hipGetLastError();
   };
-  // steps 1-3 of the pair at K0 on stream st
-  auto pair_panels = [&](int64_t K0, hipStream_t st) -> hipError_t {
-    const int64_t KA = std::min<int64_t>(n, K0 + W), KB = std::min<int64_t>(n, K0 + D);
-    hipError_t ee = factor_panel(K0, KA, 0, st);
-    if (ee != hipSuccess || KA >= n) return ee;
-    if ((ee = form_u12(K0, KA, KB, 0, st)) != hipSuccess) return ee;
-    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KA, 32)), dim3(256), 0, st, lda, F, KA, n, K0,
-                       static_cast<int>(W), La, sc(ws.sA2));
-    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(KB - KA, 32)), dim3(256), 0, st, lda, F, KA, KB, K0,
-                       static_cast<int>(W), Bb, sc(ws.sBb));
-    ee = gemm_packed(mode, false, n - KA, KB - KA, W, La, W, Bb, W, at(KA, KA), lda, st, sc(ws.sA2), sc(ws.sBb));
-    if (ee != hipSuccess) return ee;
-    return factor_panel(KA, KB, 1, st);
+  // the panels of the group at K0, brought up to date and factored, on st
+  auto group_panels = [&](int64_t K0, hipStream_t st) -> hipError_t {
+    hipError_t ee = hipSuccess;
+    for (int i = 0; i < G; ++i) {
+      const int64_t Ki = K0 + i * W;
+      if (Ki >= n) break;
+      const int64_t KEi = std::min<int64_t>(n, Ki + W);
+      for (int j = 0; j < i; ++j) {  // rows of panel j over panel i's columns, then U12_j there
+        const int64_t Kj = K0 + j * W;
+        if (j > 0 && (ee = update(Kj, Kj + W, Ki, KEi, K0, j * W, 0, st, false)) != hipSuccess) return ee;
+        if ((ee = form_u12(Kj, Ki, KEi, j, st)) != hipSuccess) return ee;
+      }
+      if (i > 0 && (ee = update(Ki, n, Ki, KEi, K0, i * W, 0, st, false)) != hipSuccess) return ee;
+      if ((ee = factor_panel(Ki, KEi, i, st)) != hipSuccess) return ee;
+    }
+    return ee;
   };
-  // steps 4-6 of the pair at K0 (main stream), up to the packs of the update
-  auto pair_prepare = [&](int64_t K0) -> hipError_t {
-    const int64_t KA = K0 + W, KB = K0 + D;
-    hipError_t ee = form_u12(K0, KB, n, 0, s);
-    if (ee != hipSuccess) return ee;
-    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
-                       static_cast<int>(W), Ua, sc(ws.sUa));
-    // b's rows of the trailing columns: the first W rows of La are b's rows
-    if ((ee = timed_sub(W, n - KB, W, La, Ua, at(KA, KB), sc(ws.sA2), sc(ws.sUa))) != hipSuccess) return ee;
-    if ((ee = form_u12(KA, KB, n, 1, s)) != hipSuccess) return ee;
-    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
-                       static_cast<int>(D), Lj, sc(ws.sA));
-    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KB, 32)), dim3(256), 0, s, lda, F, KB, n, K0,
-                       static_cast<int>(D), Uj, sc(ws.sB));
+  // the group's rows of the columns right of it, and the packs of the update
+  // (main stream); the group's panels are all full here
+  auto group_prepare = [&](int64_t K0) -> hipError_t {
+    const int64_t KG = K0 + GW;
+    hipError_t ee = hipSuccess;
+    for (int j = 0; j < G; ++j) {
+      const int64_t Kj = K0 + j * W;
+      if (j > 0 && (ee = update(Kj, Kj + W, KG, n, K0, j * W, 1, s, true)) != hipSuccess) return ee;
+      if ((ee = form_u12(Kj, KG, n, j, s)) != hipSuccess) return ee;
+    }
+    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KG, 32)), dim3(256), 0, s, lda, F, KG, n, K0,
+                       static_cast<int>(GW), Lj, sc(ws.sA));
+    hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KG, 32)), dim3(256), 0, s, lda, F, KG, n, K0,
+                       static_cast<int>(GW), Uj, sc(ws.sB));
     return hipGetLastError();
   };
-  // A22[KB:n, c0:c1] -= Lj Uj[c0:c1], K = 2W
-  auto trailing = [&](int64_t KB, int64_t c0, int64_t c1) -> hipError_t {
-    return timed_sub(n - KB, c1 - c0, D, Lj, Uj + (c0 - KB) * D, at(KB, c0), sc(ws.sA),
-                     scaled ? ws.sB + (c0 - KB) / 32 : nullptr);
+  // A22[KG:n, c0:c1] -= L21 U12[c0:c1], K = G W
+  auto trailing = [&](int64_t KG, int64_t c0, int64_t c1) -> hipError_t {
+    return sub(n - KG, c1 - c0, GW, Lj, Uj + (c0 - KG) * GW, at(KG, c0), sc(ws.sA),
+               scaled ? ws.sB + (c0 - KG) / 32 : nullptr, s, true);
   };
-  if ((e = pair_panels(0, s)) != hipSuccess) return e;
-  for (int64_t K0 = 0; K0 + D < n; K0 += D) {
-    const int64_t KB = K0 + D;
-    if ((e = pair_prepare(K0)) != hipSuccess) return e;
-    // the next pair's columns, then (look-ahead) that pair on the aux stream
-    // while the main stream updates the rest
-    const int64_t NE = std::min<int64_t>(n, KB + D);
-    if ((e = trailing(KB, KB, NE)) != hipSuccess) return e;
+  if ((e = group_panels(0, s)) != hipSuccess) return e;
+  for (int64_t K0 = 0; K0 + GW < n; K0 += GW) {
+    const int64_t KG = K0 + GW;
+    if ((e = group_prepare(K0)) != hipSuccess) return e;
+    // the next group's columns, then (look-ahead) that group's panels on the
+    // aux stream while the main stream updates the rest
+    const int64_t NE = std::min<int64_t>(n, KG + GW);
+    if ((e = trailing(KG, KG, NE)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_cols, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(ws.aux, ws.next_cols, 0)) != hipSuccess) return e;
-    if ((e = pair_panels(KB, ws.aux)) != hipSuccess) return e;
+    if ((e = group_panels(KG, ws.aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
-    if (NE < n && (e = trailing(KB, NE, n)) != hipSuccess) return e;
+    if (NE < n && (e = trailing(KG, NE, n)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
   res->gemm_launches = nev;
@@ -786,13 +802,20 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   if ((e = hipMalloc(&ws->perm, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->info, sizeof(int))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
-  for (void** p : {&ws->At, &ws->Bt})  // two panels wide
-    if ((e = hipMalloc(p, 2 * pack)) != hipSuccess) return e;
-  for (void** p : {&ws->At2, &ws->Ua, &ws->Xm, &ws->Ya})
+  constexpr int G = kLuGroup;
+  const size_t wide = esz * static_cast<size_t>(n) * (G - 1) * kLuPanel;     // n rows, (G - 1) panels deep
+  const size_t narrow = esz * static_cast<size_t>(kLuPanel) * (G - 1) * kLuPanel;
+  for (void** p : {&ws->At, &ws->Bt})  // the group's panels
+    if ((e = hipMalloc(p, G * pack)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Pr[0], wide)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Pc[0], narrow)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Pr[1], narrow)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Pc[1], wide)) != hipSuccess) return e;
+  for (void** p : {&ws->Xm, &ws->Ya})
     if ((e = hipMalloc(p, pack)) != hipSuccess) return e;
-  for (void** p : {&ws->Bb, &ws->Tinv, &ws->Tinv2, &ws->Uinv})
-    if ((e = hipMalloc(p, inv)) != hipSuccess) return e;
-  for (float** p : {&ws->sA, &ws->sB, &ws->sA2, &ws->sUa, &ws->sBb})
+  if ((e = hipMalloc(&ws->Tinv, G * inv)) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->Uinv, inv)) != hipSuccess) return e;
+  for (float** p : {&ws->sA, &ws->sB, &ws->sPr[0], &ws->sPr[1], &ws->sPc[0], &ws->sPc[1]})
     if ((e = hipMalloc(p, scales)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&ws->next_cols, hipEventDisableTiming)) != hipSuccess) return e;
@@ -801,8 +824,8 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n) + 8 * pack + 4 * inv + 5 * scales +
-                                   dinv);
+                                   sizeof(double) * 3 * static_cast<size_t>(n) + (2 * G + 2) * pack + 2 * wide +
+                                   2 * narrow + (G + 1) * inv + 6 * scales + dinv);
   return hipSuccess;
 }
 
@@ -811,10 +834,10 @@ int64_t lu_workspace_bytes(const LuWorkspace& ws) { return ws.bytes; }
 void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->aux) (void)hipStreamSynchronize(ws->aux);
   for (void* p : {static_cast<void*>(ws->F), static_cast<void*>(ws->perm), static_cast<void*>(ws->info),
-                  static_cast<void*>(ws->vec), ws->At, ws->At2, ws->Bt, ws->Ua, ws->Bb, ws->Xm, ws->Ya, ws->Tinv,
-                  ws->Tinv2, ws->Uinv, static_cast<void*>(ws->sA), static_cast<void*>(ws->sA2),
-                  static_cast<void*>(ws->sB), static_cast<void*>(ws->sUa), static_cast<void*>(ws->sBb),
-                  static_cast<void*>(ws->Dinv)})
+                  static_cast<void*>(ws->vec), ws->At, ws->Bt, ws->Pr[0], ws->Pr[1], ws->Pc[0], ws->Pc[1], ws->Xm,
+                  ws->Ya, ws->Tinv, ws->Uinv, static_cast<void*>(ws->sA), static_cast<void*>(ws->sB),
+                  static_cast<void*>(ws->sPr[0]), static_cast<void*>(ws->sPr[1]), static_cast<void*>(ws->sPc[0]),
+                  static_cast<void*>(ws->sPc[1]), static_cast<void*>(ws->Dinv)})
     if (p) (void)hipFree(p);
   if (ws->next_cols) (void)hipEventDestroy(ws->next_cols);
   if (ws->next_done) (void)hipEventDestroy(ws->next_done);

@@ -292,13 +292,13 @@ def test_solvers_agree_with_oracle(oracle64, solver, M):
 
 def test_lu_modes_agree_across_panels():
     """n = 4500 (five 1024-column panels, the last one ragged, taken in
-    pairs): the FP32 LU and the BF16 / FP16 LUs with per-block-scaled packs
-    give the same refined solution.  A pair with columns right of it updates
-    its second panel's rows of them (K = 1024), then the trailing matrix once
-    (K = 2048), split in two -- the next pair's columns first, so that the
-    next pair is factored meanwhile, then the rest -- except where the next
-    pair reaches the end: 5 timed launches.  No solve needs the GMRES
-    fallback."""
+    groups of two): the FP32 LU and the BF16 / FP16 LUs with
+    per-block-scaled packs give the same refined solution.  A group with
+    columns right of it updates its second panel's rows of them (K = 1024),
+    then the trailing matrix once (K = 2048), split in two -- the next
+    group's columns first, so that the next group is factored meanwhile,
+    then the rest -- except where the next group reaches the end: 5 timed
+    launches.  No solve needs the GMRES fallback."""
     M = 300
     pos, ori = random_scene(M, side=5.0 * (M / 60) ** (1 / 3), seed=8, min_dist=0.4)
     params = ref_config()
@@ -313,9 +313,9 @@ def test_lu_modes_agree_across_panels():
         assert st["converged"] and st["relative_residual"] <= 1e-11
         assert tm["direct_fallback"] == 0
         out[solver] = (x, tm["gemm_launches"])
-    n, W = 15 * M, 1024
-    D = 2 * W
-    launches = sum(2 + (K0 + 2 * D < n) for K0 in range(0, n, D) if K0 + D < n)
+    n, W, G = 15 * M, 1024, 2
+    GW = G * W
+    launches = sum((G - 1) + 1 + (K0 + 2 * GW < n) for K0 in range(0, n, GW) if K0 + GW < n)
     assert launches == 5
     for solver, (x, nl) in out.items():
         assert nl == launches
