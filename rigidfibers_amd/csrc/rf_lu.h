@@ -28,7 +28,7 @@ struct LuWorkspace {
   float* F = nullptr;      // factors (FP32, column-major, lda); the assembled A is kept intact
   int* perm = nullptr;     // perm[k0 + c] = original row of factored row k0 + c (block-local)
   int* info = nullptr;     // device: first zero pivot (1-based) or 0
-  double* vec = nullptr;   // 3 x n scratch vectors for the solves and refinement
+  double* vec = nullptr;   // scratch vector of the solves (n doubles)
   // packs, K-contiguous, in the mode's element type (per-32-row / per-32-column
   // scales in the mixed modes); panels are taken in groups of kLuGroup (rf_lu.hip)
   void* At = nullptr;      // L21 of the group's panels, rows right of the group: kLuGroup kLuPanel per row
@@ -45,6 +45,10 @@ struct LuWorkspace {
   void* Ya = nullptr;      // A21 before its solve (pack of rows)
   void* Tinv = nullptr;    // kLuGroup x kLuPanel^2: the forward operators of the group's diagonal blocks
   void* Uinv = nullptr;    // kLuPanel^2: U_D^-1 by columns
+  // per panel with rows below it, FP32, row-major kLuPanel^2: the forward
+  // operator and U_D^-1 again, for the solves
+  float* Tsol = nullptr;
+  float* Usol = nullptr;
   hipStream_t aux = nullptr;  // the look-ahead's stream (default priority)
   hipEvent_t next_cols = nullptr, next_done = nullptr;
   double* Dinv = nullptr;  // per diagonal block b: L_bb^-1 then U_bb^-1, 64 x 64 FP64 each, [j][c]

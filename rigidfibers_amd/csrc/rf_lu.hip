@@ -491,27 +491,108 @@ __global__ __launch_bounds__(kSolveThreads) void lu_bwd_panel_kernel(int64_t lda
   for (int r = tid; r < len; r += kSolveThreads) y[K0 + r] = ys[r];
 }
 
-// y[r0:r1] -= F[r0:r1, c0:c0+w] y[c0:c0+w]  (ranges disjoint), lane = row.
-__global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const float* __restrict__ F, int64_t c0,
-                                                            int w, int64_t r0, int64_t r1, double* __restrict__ y) {
-  __shared__ double xs[kLuPanel];
-  for (int c = threadIdx.x; c < w; c += 256) xs[c] = y[c0 + c];
+// t[r] = sum_k M[r W + k] x[k], r < W, for one of a panel's FP32 solve
+// inverses M (row-major W x W): one wavefront per row (16-byte loads along the
+// row, FP64 sums, reduced across the wavefront), kBgRows rows per workgroup.
+constexpr int kBgRows = 8;
+__global__ __launch_bounds__(256) void lu_block_gemv_kernel(const float* __restrict__ M,
+                                                            const double* __restrict__ x, double* __restrict__ t) {
+  constexpr int W = kLuPanel, Q = W / 256;  // float4 loads per lane and row
+  constexpr int R = kBgRows / 4;            // rows per wavefront
+  __shared__ double xs[W];
+  for (int k = threadIdx.x; k < W; k += 256) xs[k] = x[k];
   __syncthreads();
-  const int64_t r = r0 + static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (r >= r1) return;
-  const float* p = F + c0 * lda + r;
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  int c = 0;
-  for (; c + 16 <= w; c += 16) {
-    float v[16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rw = blockIdx.x * kBgRows + wave * R;
+  float4 v[R][Q];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) v[u] = p[(c + u) * lda];
+  for (int i = 0; i < R; ++i)
 #pragma unroll
-    for (int u = 0; u < 16; ++u) s[u & 3] = fma(static_cast<double>(v[u]), xs[c + u], s[u & 3]);
+    for (int q = 0; q < Q; ++q)
+      v[i][q] = reinterpret_cast<const float4*>(M + static_cast<int64_t>(rw + i) * W)[lane + 64 * q];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double* xk = xs + 4 * (lane + 64 * q);
+      s0 = fma(static_cast<double>(v[i][q].x), xk[0], s0);
+      s1 = fma(static_cast<double>(v[i][q].y), xk[1], s1);
+      s0 = fma(static_cast<double>(v[i][q].z), xk[2], s0);
+      s1 = fma(static_cast<double>(v[i][q].w), xk[3], s1);
+    }
+    double sum = s0 + s1;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    if (lane == 0) t[rw + i] = sum;
   }
-  for (; c < w; ++c) s[0] = fma(static_cast<double>(p[c * lda]), xs[c], s[0]);
-  y[r] -= (s[0] + s[1]) + (s[2] + s[3]);
 }
+static_assert(kLuPanel % kBgRows == 0 && kBgRows % 4 == 0 && kLuPanel % 256 == 0, "block GEMV tiling");
+
+// y[r0:r1] -= F[r0:r1, c0:c0+w] x[0:w] (w <= W; the rows disjoint from the
+// columns' block; r0 and lda multiples of 4).  Workgroup = kSgRows rows x all
+// w columns: wavefront v takes columns [v W / kSgWaves, (v + 1) W / kSgWaves),
+// lane 4 consecutive rows (one 16-byte load per column), FP64 sums; the
+// wavefronts' partial sums are added in LDS in a fixed order.  With
+// copy != nullptr, workgroup 0 also writes x to copy[0:w] (the solved block
+// back into the vector).
+constexpr int kSgWaves = 16, kSgRows = 256;
+__global__ __launch_bounds__(64 * kSgWaves) void lu_solve_gemv_kernel(int64_t lda, const float* __restrict__ F,
+                                                                       int64_t c0, int w, int64_t r0, int64_t r1,
+                                                                       const double* __restrict__ x,
+                                                                       double* __restrict__ y,
+                                                                       double* __restrict__ copy) {
+  constexpr int CW = kLuPanel / kSgWaves;  // columns per wavefront
+  constexpr int U = 16;                    // loads in flight per lane
+  __shared__ double xs[kLuPanel];
+  __shared__ double part[kSgWaves][kSgRows];
+  const int tid = threadIdx.x;
+  for (int c = tid; c < w; c += 64 * kSgWaves) xs[c] = x[c];
+  if (copy != nullptr && blockIdx.x == 0)
+    for (int c = tid; c < w; c += 64 * kSgWaves) copy[c] = x[c];
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int64_t rt = r0 + static_cast<int64_t>(blockIdx.x) * kSgRows;  // the workgroup's first row
+  const int64_t rb = rt + 4 * lane;                                     // the lane's first row
+  const int cb = wave * CW, ce = min(w, cb + CW);
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (rb + 4 <= r1) {
+    const float* p = F + c0 * lda + rb;
+    int c = cb;
+    for (; c + U <= ce; c += U) {
+      float4 v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const float4*>(p + (c + u) * lda);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double xc = xs[c + u];
+        s[0] = fma(static_cast<double>(v[u].x), xc, s[0]);
+        s[1] = fma(static_cast<double>(v[u].y), xc, s[1]);
+        s[2] = fma(static_cast<double>(v[u].z), xc, s[2]);
+        s[3] = fma(static_cast<double>(v[u].w), xc, s[3]);
+      }
+    }
+    for (; c < ce; ++c) {
+      const float4 v = *reinterpret_cast<const float4*>(p + c * lda);
+      s[0] = fma(static_cast<double>(v.x), xs[c], s[0]);
+      s[1] = fma(static_cast<double>(v.y), xs[c], s[1]);
+      s[2] = fma(static_cast<double>(v.z), xs[c], s[2]);
+      s[3] = fma(static_cast<double>(v.w), xs[c], s[3]);
+    }
+  } else if (rb < r1) {
+    for (int i = 0; i < 4 && rb + i < r1; ++i)
+      for (int c = cb; c < ce; ++c) s[i] = fma(static_cast<double>(F[(c0 + c) * lda + rb + i]), xs[c], s[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) part[wave][4 * lane + i] = s[i];
+  __syncthreads();
+  if (tid < kSgRows && rt + tid < r1) {
+    double acc = part[0][tid];
+#pragma unroll
+    for (int v = 1; v < kSgWaves; ++v) acc += part[v][tid];
+    y[rt + tid] -= acc;
+  }
+}
+static_assert(kSgRows == 4 * 64 && kLuPanel % kSgWaves == 0 && 64 * kSgWaves >= kSgRows, "solve GEMV tiling");
 
 // The inverses of a factored W x W diagonal block D = F[K0:K0+W, K0:K0+W]:
 // the panel solves above applied to unit vectors, kInvCols of them per
@@ -521,13 +602,14 @@ __global__ __launch_bounds__(256) void lu_panel_gemv_kernel(int64_t lda, const f
 //   BWD: Uinv = U_D^-1, written by columns: out[c W + r] = Uinv[r][c] (the
 //        B operand of L21 = A21 Uinv).
 // Both come from one launch: workgroups [0, W / kInvCols) form Tfwd, the
-// rest Uinv (the two sweeps are independent).
+// rest Uinv (the two sweeps are independent).  Each is also written in FP32,
+// row-major (sol[r W + k]), for the solves (lu_block_gemv_kernel).
 constexpr int kInvCols = 8;
 template <typename T, bool FWD>
 __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __restrict__ F,
                                                     const int* __restrict__ perm, const double* __restrict__ Dinv,
                                                     int64_t K0, int blk, T* __restrict__ out,
-                                                    float (*__restrict__ ys)[kLuPanel],
+                                                    float* __restrict__ sol, float (*__restrict__ ys)[kLuPanel],
                                                     float (*__restrict__ tb)[kLuNb]) {
   constexpr int W = kLuPanel;
   const int tid = threadIdx.x;
@@ -571,14 +653,14 @@ __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __
     }
     __syncthreads();
   }
-  if (FWD) {
-    for (int r = tid; r < W; r += 256) {
-      float v[kInvCols];
+  for (int r = tid; r < W; r += 256) {
+    float v[kInvCols];
 #pragma unroll
-      for (int c = 0; c < kInvCols; ++c) v[c] = ys[c][r];
-      store8<T>(out + static_cast<int64_t>(r) * W + j0, v);
-    }
-  } else {
+    for (int c = 0; c < kInvCols; ++c) v[c] = ys[c][r];
+    if (FWD) store8<T>(out + static_cast<int64_t>(r) * W + j0, v);
+    store8<float>(sol + static_cast<int64_t>(r) * W + j0, v);
+  }
+  if (!FWD) {
     for (int q = tid; q < kInvCols * W / 8; q += 256) {
       const int c = q / (W / 8), r = (q % (W / 8)) * 8;
       store8<T>(out + static_cast<int64_t>(j0 + c) * W + r, &ys[c][r]);
@@ -591,14 +673,15 @@ template <typename T>
 __global__ __launch_bounds__(256) void lu_panel_inverses_kernel(int64_t lda, const float* __restrict__ F,
                                                                 const int* __restrict__ perm,
                                                                 const double* __restrict__ Dinv, int64_t K0,
-                                                                T* __restrict__ tfwd, T* __restrict__ uinv) {
+                                                                T* __restrict__ tfwd, T* __restrict__ uinv,
+                                                                float* __restrict__ tsol, float* __restrict__ usol) {
   constexpr int G = kLuPanel / kInvCols;  // workgroups per inverse
   __shared__ __attribute__((aligned(16))) float ys[kInvCols][kLuPanel];  // the workgroup's vectors
   __shared__ float tb[kInvCols][kLuNb];
   if (static_cast<int>(blockIdx.x) < G)
-    panel_inverse_block<T, true>(lda, F, perm, Dinv, K0, blockIdx.x, tfwd, ys, tb);
+    panel_inverse_block<T, true>(lda, F, perm, Dinv, K0, blockIdx.x, tfwd, tsol, ys, tb);
   else
-    panel_inverse_block<T, false>(lda, F, perm, Dinv, K0, blockIdx.x - G, uinv, ys, tb);
+    panel_inverse_block<T, false>(lda, F, perm, Dinv, K0, blockIdx.x - G, uinv, usol, ys, tb);
 }
 
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
@@ -682,7 +765,7 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
                        K0 / kLuNb, ws.Dinv);
     if (KE >= n) return hipGetLastError();
     hipLaunchKernelGGL((lu_panel_inverses_kernel<T>), dim3(2 * W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
-                       ws.Dinv, K0, Ti(slot), Ui);
+                       ws.Dinv, K0, Ti(slot), Ui, ws.Tsol + (K0 / W) * W * W, ws.Usol + (K0 / W) * W * W);
     // L21 = A21 U_D^-1
     hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KE, 32)), dim3(256), 0, st, lda, F, KE, n, K0,
                        static_cast<int>(W), Y, nullptr);
@@ -767,28 +850,38 @@ static hipError_t lu_factor(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResul
   return hipErrorInvalidValue;
 }
 
-// y <- (LU)^-1 P y
+// y <- (LU)^-1 P y.  Panels with rows below them (their FP32 solve inverses
+// stored): the block's forward operator (or U_D^-1) times y's block into the
+// scratch t, then the rows outside the block -= the panel times t, the same
+// launch writing t back into y.  The last panel (no inverses): the
+// block-by-block panel kernels, in place.
 static hipError_t lu_apply_inverse(LuWorkspace& ws, double* y, hipStream_t s) {
   const int64_t n = ws.n, lda = ws.lda, W = kLuPanel;
-  for (int64_t K0 = 0; K0 < n; K0 += W) {
-    const int64_t KE = std::min<int64_t>(n, K0 + W);
-    hipLaunchKernelGGL(lu_fwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.perm, ws.Dinv, K0, KE,
-                       y);
-    if (KE < n)
-      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(grid_of(n - KE, 256)), dim3(256), 0, s, lda, ws.F, K0,
-                         static_cast<int>(KE - K0), KE, n, y);
+  const int64_t KL = ((n - 1) / W) * W;  // the last panel
+  double* t = ws.vec;
+  auto gemv = [&](int64_t c0, int w, int64_t r0, int64_t r1, const double* xv, double* copy) {
+    const unsigned grid = std::max(1u, grid_of(r1 - r0, kSgRows));
+    hipLaunchKernelGGL(lu_solve_gemv_kernel, dim3(grid), dim3(64 * kSgWaves), 0, s, lda, ws.F, c0, w, r0, r1, xv,
+                       y, copy);
+  };
+  for (int64_t K0 = 0; K0 < KL; K0 += W) {
+    hipLaunchKernelGGL(lu_block_gemv_kernel, dim3(W / kBgRows), dim3(256), 0, s, ws.Tsol + (K0 / W) * W * W, y + K0,
+                       t);
+    gemv(K0, static_cast<int>(W), K0 + W, n, t, y + K0);
   }
-  for (int64_t K0 = ((n - 1) / W) * W; K0 >= 0; K0 -= W) {
-    const int64_t KE = std::min<int64_t>(n, K0 + W);
-    hipLaunchKernelGGL(lu_bwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.Dinv, K0, KE, y);
-    if (K0 > 0)
-      hipLaunchKernelGGL(lu_panel_gemv_kernel, dim3(grid_of(K0, 256)), dim3(256), 0, s, lda, ws.F, K0,
-                         static_cast<int>(KE - K0), int64_t(0), K0, y);
+  hipLaunchKernelGGL(lu_fwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.perm, ws.Dinv, KL, n, y);
+  hipLaunchKernelGGL(lu_bwd_panel_kernel, dim3(1), dim3(kSolveThreads), 0, s, lda, ws.F, ws.Dinv, KL, n, y);
+  if (KL > 0) gemv(KL, static_cast<int>(n - KL), 0, KL, y + KL, nullptr);
+  for (int64_t K0 = KL - W; K0 >= 0; K0 -= W) {
+    hipLaunchKernelGGL(lu_block_gemv_kernel, dim3(W / kBgRows), dim3(256), 0, s, ws.Usol + (K0 / W) * W * W, y + K0,
+                       t);
+    gemv(K0, static_cast<int>(W), 0, K0, t, y + K0);
   }
   return hipGetLastError();
 }
 
 hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws) {
+  if (n < 1 || lda < n || lda % 4 != 0) return hipErrorInvalidValue;  // the solves load 4 rows at a time
   ws->mode = mode;
   ws->n = n;
   ws->lda = lda;
@@ -801,7 +894,7 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   if ((e = hipMalloc(&ws->F, sizeof(float) * static_cast<size_t>(lda) * n)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->perm, sizeof(int) * static_cast<size_t>(n))) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->info, sizeof(int))) != hipSuccess) return e;
-  if ((e = hipMalloc(&ws->vec, sizeof(double) * 3 * static_cast<size_t>(n))) != hipSuccess) return e;
+  if ((e = hipMalloc(&ws->vec, sizeof(double) * static_cast<size_t>(n))) != hipSuccess) return e;
   constexpr int G = kLuGroup;
   const size_t wide = esz * static_cast<size_t>(n) * (G - 1) * kLuPanel;     // n rows, (G - 1) panels deep
   const size_t narrow = esz * static_cast<size_t>(kLuPanel) * (G - 1) * kLuPanel;
@@ -815,6 +908,10 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
     if ((e = hipMalloc(p, pack)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->Tinv, G * inv)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->Uinv, inv)) != hipSuccess) return e;
+  const size_t sol = sizeof(float) * static_cast<size_t>((n - 1) / kLuPanel) * kLuPanel * kLuPanel;
+  if (sol > 0)
+    for (float** p : {&ws->Tsol, &ws->Usol})
+      if ((e = hipMalloc(p, sol)) != hipSuccess) return e;
   for (float** p : {&ws->sA, &ws->sB, &ws->sPr[0], &ws->sPr[1], &ws->sPc[0], &ws->sPc[1]})
     if ((e = hipMalloc(p, scales)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
@@ -824,8 +921,8 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * 3 * static_cast<size_t>(n) + (2 * G + 2) * pack + 2 * wide +
-                                   2 * narrow + (G + 1) * inv + 6 * scales + dinv);
+                                   sizeof(double) * static_cast<size_t>(n) + (2 * G + 2) * pack + 2 * wide +
+                                   2 * narrow + (G + 1) * inv + 2 * sol + 6 * scales + dinv);
   return hipSuccess;
 }
 
@@ -837,7 +934,8 @@ void lu_workspace_destroy(LuWorkspace* ws) {
                   static_cast<void*>(ws->vec), ws->At, ws->Bt, ws->Pr[0], ws->Pr[1], ws->Pc[0], ws->Pc[1], ws->Xm,
                   ws->Ya, ws->Tinv, ws->Uinv, static_cast<void*>(ws->sA), static_cast<void*>(ws->sB),
                   static_cast<void*>(ws->sPr[0]), static_cast<void*>(ws->sPr[1]), static_cast<void*>(ws->sPc[0]),
-                  static_cast<void*>(ws->sPc[1]), static_cast<void*>(ws->Dinv)})
+                  static_cast<void*>(ws->sPc[1]), static_cast<void*>(ws->Dinv), static_cast<void*>(ws->Tsol),
+                  static_cast<void*>(ws->Usol)})
     if (p) (void)hipFree(p);
   if (ws->next_cols) (void)hipEventDestroy(ws->next_cols);
   if (ws->next_done) (void)hipEventDestroy(ws->next_done);
