@@ -364,8 +364,9 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ double bred[PAIRS][NR];
   __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
   // FP16 copy: each source slot's largest |projection| (float bits), one
-  // buffer per sweep parity
+  // buffer per sweep parity; the inverse scale of each (source, mode) column
   __shared__ unsigned smax[2][PAIRS];
+  __shared__ __align__(16) float ivcol[PN];
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -481,12 +482,17 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       }
       return val;
     };
+    // FP16 copy: the sweep's block scales (S16) and each column's inverse
+    // scale, once per sweep, so that the jobs below read four with one load
+    _Float16* const T16col = HALF ? T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N : nullptr;
     if constexpr (HALF) {
-      if (tid < PAIRS && j0 + tid < je) {
+      if (tid < PN) {
         float sv, iv;
-        half_block_scale(smax[par][tid], sv, iv);
-        S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
+        half_block_scale(smax[par][tid / N], sv, iv);
+        ivcol[tid] = iv;
+        if (tid % N == 0 && j0 + tid / N < je) S16[static_cast<int64_t>(i) * M + j0 + tid / N] = sv;
       }
+      __syncthreads();
     }
     for (int job = tid; job < NC * NV; job += kPairThreads) {
       const int u = job / NV;
@@ -495,17 +501,11 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
       if constexpr (HALF) {  // the same values (self pair zeroed by put) on the FP16 copy
         typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
-        // the 4 columns span at most two sources: e < split lie in source sa
-        const int sa = pm0 / N;
-        const int split = (sa + 1) * N - pm0;
-        float sv, iva, ivb;
-        half_block_scale(smax[par][sa], sv, iva);
-        half_block_scale(smax[par][min(sa + 1, PAIRS - 1)], sv, ivb);
+        const floatx4 iv = *reinterpret_cast<const floatx4*>(ivcol + pm0);
         halfx4 h;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * (e < split ? iva : ivb));
-        _Float16* dst = T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N +
-                        static_cast<int64_t>(u) * Lp + pm0;
+        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * iv[e]);
+        _Float16* dst = T16col + (static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0));
         if (full || pm0 + 3 < npm) {
           *reinterpret_cast<halfx4*>(dst) = h;
         } else {
