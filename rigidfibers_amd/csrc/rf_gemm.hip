@@ -187,7 +187,60 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
     }
   }
 
-  // C -= acc / (sa sb), or C = acc (STORE)
+  // C -= acc / (sa sb), or C = acc (STORE).  A whole tile with 16-byte C
+  // access (the same for every thread of the workgroup) reads half its
+  // wave's C blocks at a time, all loads in flight together, into the
+  // registers the K loop no longer needs; other tiles take the general path.
+  if (!STORE && vec4 && m0 + kTile <= m && n0 + kTile <= n) {
+    constexpr int TH = TM / 2;
+    static_assert(TM % 2 == 0, "two halves of the wave's row blocks");
+    // the wave's block scales first (uniform: scalar loads), so that no wait
+    // for them falls between the C loads
+    const int uw = __builtin_amdgcn_readfirstlane(wave);
+    const int64_t um = m0 + (uw % WM) * (kTile / WM), un = n0 + (uw / WM) * (kTile / WN);
+    float w[TM][TN];
+#pragma unroll
+    for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f;
+    if (sa) {
+      float ra[TM], rb[TN];
+#pragma unroll
+      for (int ta = 0; ta < TM; ++ta) ra[ta] = sa[(um + 32 * ta) >> 5];
+#pragma unroll
+      for (int tb = 0; tb < TN; ++tb) rb[tb] = sb[(un + 32 * tb) >> 5];
+#pragma unroll
+      for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f / (ra[ta] * rb[tb]);  // exact: powers of two
+    }
+    const float* cbase = C + (un + (lane & 31)) * ldc + um + 4 * (lane >> 5);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      v4f cv[TH][TN][4];
+#pragma unroll
+      for (int a = 0; a < TH; ++a)
+#pragma unroll
+        for (int tb = 0; tb < TN; ++tb)
+#pragma unroll
+          for (int g = 0; g < 4; ++g)
+            cv[a][tb][g] = *reinterpret_cast<const v4f*>(cbase + 32 * tb * ldc + 32 * (h * TH + a) + 8 * g);
+#pragma unroll
+      for (int a = 0; a < TH; ++a)
+#pragma unroll
+        for (int tb = 0; tb < TN; ++tb) {
+          const int ta = h * TH + a;
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            v4f c = cv[a][tb][g];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) c[i] = fmaf(-w[ta][tb], acc[ta][tb][4 * g + i], c[i]);
+            *reinterpret_cast<v4f*>(const_cast<float*>(cbase) + 32 * tb * ldc + 32 * ta + 8 * g) = c;
+          }
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int ta = 0; ta < TM; ++ta)
 #pragma unroll
@@ -239,7 +292,9 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
 // per CU (with or without a persistent, staggered schedule).  The C update
 // is what holds it there: its read-modify-write moves 8 bytes per element of
 // C (11.5 GB at this size, the same order as the MFMA time at peak), and the
-// same loop without it runs at about 1.5 PFLOP/s.
+// same loop without it runs at about 1.5 PFLOP/s.  The loop itself reads 24
+// 16-byte LDS fragments per 32 MFMAs and stores each stage once: per CU and
+// stage about as many LDS cycles as MFMA cycles.
 template <typename T>
 struct GemmShape {
   static constexpr int WM = 2, WN = 4, PF = 2;
