@@ -70,11 +70,12 @@ __device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
 }
 
 // Elimination step for column C (compile-time, so the row stays in registers).
-// The pivot row is broadcast through LDS (the pivot lane stores its row, every
-// lane reads it back); rows already chosen get the multiplier 0, so their
-// update fma(-0, p, x) leaves them exactly as they are (finite p).
+// The pivot row is broadcast from the pivot lane's registers (v_readlane: the
+// pivot lane is uniform over the wavefront); rows already chosen get the
+// multiplier 0, so their update fma(-0, p, x) leaves them exactly as they are
+// (finite p).
 template <int C>
-__device__ __forceinline__ void diag_step(DiagLane& d, int kb, int lane, float* prow) {
+__device__ __forceinline__ void diag_step(DiagLane& d, int kb, int lane) {
   if constexpr (C < kLuNb) {
     if (C >= kb) return;
     // pivot: largest |row[C]| among the rows not yet chosen.  The key orders
@@ -85,39 +86,27 @@ __device__ __forceinline__ void diag_step(DiagLane& d, int kb, int lane, float* 
     const unsigned bits = __builtin_bit_cast(unsigned, fabsf(a)) >> 6;
     const unsigned key = d.used ? 0u : (((bits + 1u) << 6) | static_cast<unsigned>(63 - lane));
     const int pr = 63 - static_cast<int>(wave_max_u32(key) & 63u);
-    constexpr int C4 = C & ~3;  // 16-byte aligned start of the broadcast
     if (lane == pr) {
       d.used = true;
       d.pos = C;
-#pragma unroll
-      for (int c = C4; c < kLuNb; c += 4)
-        *reinterpret_cast<float4*>(prow + c) = make_float4(d.row[c], d.row[c + 1], d.row[c + 2], d.row[c + 3]);
     }
-    __syncthreads();  // one wavefront: orders the stores before the reads
-    float p[kLuNb - C4];
+    float p[kLuNb - C];
 #pragma unroll
-    for (int c = C4; c < kLuNb; c += 4) {
-      const float4 v = *reinterpret_cast<const float4*>(prow + c);
-      p[c - C4] = v.x;
-      p[c - C4 + 1] = v.y;
-      p[c - C4 + 2] = v.z;
-      p[c - C4 + 3] = v.w;
-    }
-    __syncthreads();  // the next column's stores come after these reads
-    const float pv = p[C - C4];
+    for (int c = C; c < kLuNb; ++c)
+      p[c - C] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, d.row[c]), pr));
+    const float pv = p[0];
     if (pv == 0.0f && d.zero_col == 0) d.zero_col = C + 1;
     const float inv = (pv != 0.0f) ? 1.0f / pv : 0.0f;
     const float l = d.used ? 0.0f : d.row[C] * inv;
     if (!d.used) d.row[C] = l;
 #pragma unroll
-    for (int cc = C + 1; cc < kLuNb; ++cc) d.row[cc] = fmaf(-l, p[cc - C4], d.row[cc]);
-    diag_step<C + 1>(d, kb, lane, prow);
+    for (int cc = C + 1; cc < kLuNb; ++cc) d.row[cc] = fmaf(-l, p[cc - C], d.row[cc]);
+    diag_step<C + 1>(d, kb, lane);
   }
 }
 
 __global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restrict__ F, int64_t k0, int kb,
                                                      int* __restrict__ perm, int* __restrict__ info) {
-  __shared__ __attribute__((aligned(16))) float prow[kLuNb];
   const int lane = threadIdx.x;
   DiagLane d;
 #pragma unroll
@@ -125,7 +114,7 @@ __global__ __launch_bounds__(64) void lu_diag_kernel(int64_t lda, float* __restr
   d.used = lane >= kb;  // rows past the block never pivot
   d.pos = lane;
   d.zero_col = 0;
-  diag_step<0>(d, kb, lane, prow);
+  diag_step<0>(d, kb, lane);
   const int pos = d.pos, zero_col = d.zero_col;
   const float* row = d.row;
   if (lane < kb) {
