@@ -599,7 +599,7 @@ __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __
                                                     const int* __restrict__ perm, const double* __restrict__ Dinv,
                                                     int64_t K0, int blk, T* __restrict__ out,
                                                     float* __restrict__ sol, float (*__restrict__ ys)[kLuPanel],
-                                                    float (*__restrict__ tb)[kLuNb]) {
+                                                    float (*__restrict__ tb)[kLuNb], double* __restrict__ Ms) {
   constexpr int W = kLuPanel;
   const int tid = threadIdx.x;
   const int j0 = blk * kInvCols;
@@ -613,6 +613,13 @@ __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __
     const int b0 = (FWD ? bi : nb - 1 - bi) * kLuNb;
     const int64_t k0 = K0 + b0;
     const double* Minv = Dinv + (k0 / kLuNb) * 2 * kLuNb * kLuNb + (FWD ? 0 : kLuNb * kLuNb);
+    // the block inverse into LDS, all its loads in flight together
+    constexpr int kPer = kLuNb * kLuNb / 256;
+    double mv[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) mv[t] = Minv[t * 256 + tid];
+#pragma unroll
+    for (int t = 0; t < kPer; ++t) Ms[t * 256 + tid] = mv[t];
     for (int q = tid; q < kInvCols * kLuNb; q += 256) {
       const int c = q >> 6, i = q & 63;
       const int src = FWD ? min(max(perm[k0 + i] - static_cast<int>(k0), 0), kLuNb - 1) : i;
@@ -621,10 +628,10 @@ __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __
     __syncthreads();
     for (int q = tid; q < kInvCols * kLuNb; q += 256) {
       const int c = q >> 6, i = q & 63;
-      double acc = 0.0;
-#pragma unroll 8
-      for (int j = 0; j < kLuNb; ++j) acc = fma(Minv[j * kLuNb + i], static_cast<double>(tb[c][j]), acc);
-      ys[c][b0 + i] = static_cast<float>(acc);
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < kLuNb; ++j) acc[j & 3] = fma(Ms[j * kLuNb + i], static_cast<double>(tb[c][j]), acc[j & 3]);
+      ys[c][b0 + i] = static_cast<float>((acc[0] + acc[1]) + (acc[2] + acc[3]));
     }
     __syncthreads();
     const int ra = FWD ? b0 + kLuNb : 0, rz = FWD ? W : b0;
@@ -667,10 +674,11 @@ __global__ __launch_bounds__(256) void lu_panel_inverses_kernel(int64_t lda, con
   constexpr int G = kLuPanel / kInvCols;  // workgroups per inverse
   __shared__ __attribute__((aligned(16))) float ys[kInvCols][kLuPanel];  // the workgroup's vectors
   __shared__ float tb[kInvCols][kLuNb];
+  __shared__ double Ms[kLuNb * kLuNb];  // the current diagonal block's inverse
   if (static_cast<int>(blockIdx.x) < G)
-    panel_inverse_block<T, true>(lda, F, perm, Dinv, K0, blockIdx.x, tfwd, tsol, ys, tb);
+    panel_inverse_block<T, true>(lda, F, perm, Dinv, K0, blockIdx.x, tfwd, tsol, ys, tb, Ms);
   else
-    panel_inverse_block<T, false>(lda, F, perm, Dinv, K0, blockIdx.x - G, uinv, usol, ys, tb);
+    panel_inverse_block<T, false>(lda, F, perm, Dinv, K0, blockIdx.x - G, uinv, usol, ys, tb, Ms);
 }
 
 static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((len + per - 1) / per); }
