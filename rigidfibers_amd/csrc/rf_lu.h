@@ -17,10 +17,11 @@ enum class LuMode { kFP32, kBF16, kFP16 };
 constexpr int kLuNb = 64;       // diagonal block: pivoting happens inside it
 constexpr int kLuPanel = 1024;  // outer panel
 // Panels per trailing update (its K = kLuGroup kLuPanel).  Measured at
-// n = 75 000 (BF16): 2 -> 555 ms per step (trailing GEMM 710 TFLOP/s),
-// 4 -> 564 ms (767 TFLOP/s; the longer in-group chains on the look-ahead
-// stream cost more than the faster update saves).
-constexpr int kLuGroup = 2;
+// n = 75 000 in one run (BF16 / FP16 ms per step, trailing GEMM TFLOP/s):
+// 2 -> 495 / 456 (749 / 833), 3 -> 494 / 457 (787 / 878), 4 -> 508 / 470
+// (799 / 902: the longer in-group chains on the look-ahead stream cost more
+// than the faster update saves).
+constexpr int kLuGroup = 3;
 
 struct LuWorkspace {
   LuMode mode = LuMode::kFP32;

@@ -292,13 +292,12 @@ def test_solvers_agree_with_oracle(oracle64, solver, M):
 
 def test_lu_modes_agree_across_panels():
     """n = 4500 (five 1024-column panels, the last one ragged, taken in
-    groups of two): the FP32 LU and the BF16 / FP16 LUs with
-    per-block-scaled packs give the same refined solution.  A group with
-    columns right of it updates its second panel's rows of them (K = 1024),
-    then the trailing matrix once (K = 2048), split in two -- the next
-    group's columns first, so that the next group is factored meanwhile,
-    then the rest -- except where the next group reaches the end: 5 timed
-    launches.  No solve needs the GMRES fallback."""
+    groups of three): the FP32 LU and the BF16 / FP16 LUs with
+    per-block-scaled packs give the same refined solution.  The first group
+    updates its second and third panels' rows of the columns right of it
+    (K = 1024, 2048), then the trailing matrix once (K = 3072); the next
+    group reaches the end, so that update is not split: 3 timed launches.
+    No solve needs the GMRES fallback."""
     M = 300
     pos, ori = random_scene(M, side=5.0 * (M / 60) ** (1 / 3), seed=8, min_dist=0.4)
     params = ref_config()
@@ -313,10 +312,10 @@ def test_lu_modes_agree_across_panels():
         assert st["converged"] and st["relative_residual"] <= 1e-11
         assert tm["direct_fallback"] == 0
         out[solver] = (x, tm["gemm_launches"])
-    n, W, G = 15 * M, 1024, 2
+    n, W, G = 15 * M, 1024, 3
     GW = G * W
     launches = sum((G - 1) + 1 + (K0 + 2 * GW < n) for K0 in range(0, n, GW) if K0 + GW < n)
-    assert launches == 5
+    assert launches == 3
     for solver, (x, nl) in out.items():
         assert nl == launches
         e = rel_err(x, out["direct"][0])
