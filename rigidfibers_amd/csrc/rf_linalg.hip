@@ -673,8 +673,14 @@ hipError_t launch_dots_partial_batch(int nvb, int kmax, const SusDev* sus, const
 // layout as T.
 
 // y = A16 x (mode 0) or b - A16 x (mode 1): the theta GEMV on the FP16 copy.
-// Each lane takes two consecutive entries (one 4-byte load per component, a
-// wave reads 256 contiguous bytes) and folds the block scale into x.
+// A workgroup takes kG16Targets consecutive targets, so that the x entries it
+// streams serve all of them (alone, x's 24 bytes per entry were as many bytes
+// as the entry's 27 FP16 values: 6.38 -> 6.79 TB/s at M = 5000 in a copy of
+// the kernel); each lane takes two consecutive entries (one 4-byte load per
+// component, a wave reads 256 contiguous bytes) and folds each target's block
+// scale into x.  Each target's sums see the same operations in the same order
+// as with one target per workgroup.
+constexpr int kG16Targets = 2;
 template <int N>
 __global__ __launch_bounds__(256) void gemv_theta16_kernel(int M, int64_t Lp, const _Float16* __restrict__ T16,
                                                            const float* __restrict__ S,
@@ -683,74 +689,97 @@ __global__ __launch_bounds__(256) void gemv_theta16_kernel(int M, int64_t Lp, co
                                                            const double* __restrict__ b, int mode, PhysConsts pc,
                                                            const int* __restrict__ skip) {
   typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+  constexpr int TT = kG16Targets;
   if (skip != nullptr && *skip) return;
   constexpr int NC = 3 + 6 * (N - 1);
   constexpr int NACC = 1 + 3 * (N - 1);
-  __shared__ double red[256 / 64][NACC];
-  const int i = blockIdx.x;
-  const _Float16* Hi = T16 + static_cast<int64_t>(i) * NC * Lp;
-  const float* Si = S + static_cast<int64_t>(i) * M;
+  __shared__ double red[256 / 64][TT][NACC];
+  const int i0 = TT * static_cast<int>(blockIdx.x);
+  const int nt = min(TT, M - i0);  // targets of this workgroup (uniform)
   const int L = M * N;
-  double acc[NACC];
+  double acc[TT][NACC];
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
-  auto entry = [&](int e, double hv[NC]) {
-    const double sj = static_cast<double>(Si[e / N]);
-    const double x0 = sj * x[3 * static_cast<int64_t>(e)], x1 = sj * x[3 * static_cast<int64_t>(e) + 1],
-                 x2 = sj * x[3 * static_cast<int64_t>(e) + 2];
-    acc[0] = fma(hv[0], x0, fma(hv[1], x1, fma(hv[2], x2, acc[0])));
+  for (int t = 0; t < TT; ++t)
+#pragma unroll
+    for (int a = 0; a < NACC; ++a) acc[t][a] = 0.0;
+  // entry of scaled x (xa, xb, xc) and FP16 values hv into target t's sums
+  auto entry = [&](double* ac, const double hv[NC], double xa, double xb, double xc) {
+    ac[0] = fma(hv[0], xa, fma(hv[1], xb, fma(hv[2], xc, ac[0])));
 #pragma unroll
     for (int k = 1; k < N; ++k) {
       const double* t = hv + 3 + 6 * (k - 1);  // xx yy zz xy xz yz
-      double* u = acc + 1 + 3 * (k - 1);
-      u[0] = fma(t[0], x0, fma(t[3], x1, fma(t[4], x2, u[0])));
-      u[1] = fma(t[3], x0, fma(t[1], x1, fma(t[5], x2, u[1])));
-      u[2] = fma(t[4], x0, fma(t[5], x1, fma(t[2], x2, u[2])));
+      double* u = ac + 1 + 3 * (k - 1);
+      u[0] = fma(t[0], xa, fma(t[3], xb, fma(t[4], xc, u[0])));
+      u[1] = fma(t[3], xa, fma(t[1], xb, fma(t[5], xc, u[1])));
+      u[2] = fma(t[4], xa, fma(t[5], xb, fma(t[2], xc, u[2])));
     }
   };
   const int Lpair = L & ~1;
   for (int e = 2 * threadIdx.x; e < Lpair; e += 512) {
-    double h0[NC], h1[NC];
+    double xv[6];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      const h2 v = __builtin_nontemporal_load(reinterpret_cast<const h2*>(Hi + c * Lp + e));
-      h0[c] = static_cast<double>(static_cast<float>(v.x));
-      h1[c] = static_cast<double>(static_cast<float>(v.y));
+    for (int q = 0; q < 6; ++q) xv[q] = x[3 * static_cast<int64_t>(e) + q];
+#pragma unroll
+    for (int t = 0; t < TT; ++t) {
+      if (t >= nt) break;
+      const _Float16* Hi = T16 + static_cast<int64_t>(i0 + t) * NC * Lp;
+      const float* Si = S + static_cast<int64_t>(i0 + t) * M;
+      h2 v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; ++c) v[c] = __builtin_nontemporal_load(reinterpret_cast<const h2*>(Hi + c * Lp + e));
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        double hv[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) hv[c] = static_cast<double>(static_cast<float>(v[c][q]));
+        const double sj = static_cast<double>(Si[(e + q) / N]);
+        entry(acc[t], hv, sj * xv[3 * q], sj * xv[3 * q + 1], sj * xv[3 * q + 2]);
+      }
     }
-    entry(e, h0);
-    entry(e + 1, h1);
   }
   if ((L & 1) && threadIdx.x == 0) {
-    double h0[NC];
+    const int e = L - 1;
+    for (int t = 0; t < nt; ++t) {
+      const _Float16* Hi = T16 + static_cast<int64_t>(i0 + t) * NC * Lp;
+      double hv[NC];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) h0[c] = static_cast<double>(static_cast<float>(Hi[c * Lp + L - 1]));
-    entry(L - 1, h0);
+      for (int c = 0; c < NC; ++c) hv[c] = static_cast<double>(static_cast<float>(Hi[c * Lp + e]));
+      const double sj = static_cast<double>(S[static_cast<int64_t>(i0 + t) * M + e / N]);
+#pragma unroll
+      for (int tt = 0; tt < TT; ++tt)
+        if (tt == t)
+          entry(acc[tt], hv, sj * x[3 * static_cast<int64_t>(e)], sj * x[3 * static_cast<int64_t>(e) + 1],
+                sj * x[3 * static_cast<int64_t>(e) + 2]);
+    }
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) {
-    double v = acc[a];
+  for (int t = 0; t < TT; ++t)
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wave][a] = v;
-  }
+    for (int a = 0; a < NACC; ++a) {
+      double v = acc[t][a];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+      if (lane == 0) red[wave][t][a] = v;
+    }
   __syncthreads();
-  if (threadIdx.x < 3 * N) {
-    const int r = threadIdx.x, k = r / 3, a = r - 3 * k;
-    const double t[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
+  if (static_cast<int>(threadIdx.x) < nt * 3 * N) {
+    const int t = threadIdx.x / (3 * N), r = threadIdx.x - t * 3 * N, k = r / 3, a = r - 3 * k;
+    const int i = i0 + t;
+    const double tv[3] = {ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]};
     double out;
     if (k == 0) {
-      const double s0 = red[0][0] + red[1][0] + red[2][0] + red[3][0];
-      out = static_cast<double>(pc.D1) * t[a] * s0;
+      const double s0 = red[0][t][0] + red[1][t][0] + red[2][t][0] + red[3][t][0];
+      out = static_cast<double>(pc.D1) * tv[a] * s0;
     } else {
       double U[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int idx = 1 + 3 * (k - 1) + c;
-        U[c] = red[0][idx] + red[1][idx] + red[2][idx] + red[3][idx];
+        U[c] = red[0][t][idx] + red[1][t][idx] + red[2][t][idx] + red[3][t][idx];
       }
-      const double tU = t[0] * U[0] + t[1] * U[1] + t[2] * U[2];
-      out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * t[a] * tU);
+      const double tU = tv[0] * U[0] + tv[1] * U[1] + tv[2] * U[2];
+      out = static_cast<double>(pc.gamma[k]) * (U[a] - static_cast<double>(pc.eigen[k]) * tv[a] * tU);
     }
     const int64_t row = static_cast<int64_t>(3 * N) * i + r;
     out += x[row];  // identity diagonal block
@@ -765,7 +794,8 @@ hipError_t gemv_theta16(int N, int M, int64_t Lp, const _Float16* T16, const flo
   switch (N) {
 #define RF_G16(NN)                                                                                            \
   case NN:                                                                                                    \
-    hipLaunchKernelGGL(gemv_theta16_kernel<NN>, dim3(M), dim3(256), 0, s, M, Lp, T16, S, ori, x, y, b, mode, \
+    hipLaunchKernelGGL(gemv_theta16_kernel<NN>, dim3((M + kG16Targets - 1) / kG16Targets), dim3(256), 0, s, M, Lp, \
+                       T16, S, ori, x, y, b, mode, \
                        pc, skip);                                                                             \
     break;
     RF_G16(1)
