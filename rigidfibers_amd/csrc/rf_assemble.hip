@@ -960,21 +960,35 @@ __global__ __launch_bounds__(256) void expand_theta_dense_kernel(int M, int64_t 
   constexpr int ROWS = 3 * N;
   constexpr int TI = exp_targets<N>();
   constexpr int TROWS = TI * ROWS;  // rows of a column in the tile, <= 256
-  __shared__ __align__(16) float Ts[TI * NC * kExpRow];
+  constexpr int kChunks = TI * NC * (kExpEntries / 4);  // 16-byte chunks of the tile
+  constexpr int kPer = (kChunks + 255) / 256;           // per thread
+  // the tile's rows, then rows for the last round's chunks past the tile
+  __shared__ __align__(16) float Ts[kPer * 256 / (kExpEntries / 4) * kExpRow];
   const int tid = threadIdx.x;
   const int64_t L = static_cast<int64_t>(M) * N;
   const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kExpEntries;
   const int i0 = static_cast<int>(blockIdx.y) * TI;
   // stage: row (ti, c) holds T[i0 + ti][c][e0 .. e0 + 31] (within the padded
-  // row of Lp >= e0 + 32 numbers)
-  for (int q = tid; q < TI * NC * (kExpEntries / 4); q += 256) {
-    const int row = q / (kExpEntries / 4), part = q - row * (kExpEntries / 4);
+  // row of Lp >= e0 + 32 numbers).  Every load of the thread is in flight
+  // before the LDS stores, which are unconditional (a guarded store would let
+  // the compiler sink each load to it): chunks past the tile, and rows of
+  // targets past M, load the tile's first chunk and store it where nothing
+  // reads it.
+  float4 sv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    int q = tid + 256 * u;
+    int row = q / (kExpEntries / 4);
+    if (q >= kChunks || i0 + row / NC >= M) q = 0, row = 0;
+    const int part = q - row * (kExpEntries / 4);
     const int ti = row / NC, c = row - ti * NC;
-    if (i0 + ti < M) {
-      const float4 v = *reinterpret_cast<const float4*>(T + (static_cast<int64_t>(i0 + ti) * NC + c) * Lp + e0 +
-                                                        4 * part);
-      *reinterpret_cast<float4*>(&Ts[row * kExpRow + 4 * part]) = v;
-    }
+    sv[u] = *reinterpret_cast<const float4*>(T + (static_cast<int64_t>(i0 + ti) * NC + c) * Lp + e0 + 4 * part);
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int q = tid + 256 * u;
+    const int row = q / (kExpEntries / 4), part = q - row * (kExpEntries / 4);
+    *reinterpret_cast<float4*>(&Ts[row * kExpRow + 4 * part]) = sv[u];
   }
   __syncthreads();
   if (tid >= TROWS) return;  // no barrier follows
