@@ -142,12 +142,22 @@ __global__ __launch_bounds__(256) void lu_l21_u12_kernel(int64_t k1, int64_t e, 
   __shared__ int src[kLuNb];
   const int tid = threadIdx.x;
   const bool lpart = static_cast<int>(blockIdx.x) < nl;
-  for (int q = tid; q < kLuNb * kLuNb; q += 256) {
-    const int hi = q >> 6, lo = q & 63;
-    if (lpart)
-      Mt[lo][hi] = F[(k0 + hi) * lda + k0 + lo];  // U[lo][hi]
-    else
-      Mt[hi][lo] = F[(k0 + hi) * lda + k0 + lo];  // L[lo][hi]
+  {  // the factored block, every load of the thread in flight together
+    constexpr int kPer = kLuNb * kLuNb / 256;
+    float bv[kPer];
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 256 * u;
+      bv[u] = F[(k0 + (q >> 6)) * lda + k0 + (q & 63)];
+    }
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int q = tid + 256 * u, hi = q >> 6, lo = q & 63;
+      if (lpart)
+        Mt[lo][hi] = bv[u];  // U[lo][hi]
+      else
+        Mt[hi][lo] = bv[u];  // L[lo][hi]
+    }
   }
   if (tid < kLuNb) {
     if (lpart) {
@@ -200,10 +210,21 @@ __global__ __launch_bounds__(256) void lu_dupdate_kernel(int64_t lda, float* __r
   const int tid = threadIdx.x;
   const int64_t r0 = k1 + static_cast<int64_t>(blockIdx.x) * kLuNb;
   const int64_t cc0 = k1 + static_cast<int64_t>(blockIdx.y) * kLuNb;
-  for (int q = tid; q < kLuNb * kLuNb; q += 256) {
-    const int k = q >> 6, i = q & 63;
-    Ls[k][i] = (r0 + i < e) ? F[(k0 + k) * lda + r0 + i] : 0.0f;
-    Us[i][k] = (cc0 + k < e) ? F[(cc0 + k) * lda + k0 + i] : 0.0f;  // q: column k, row i of U
+  // staging: every load of the thread in flight together (addresses clamped
+  // to the last row / column, the values past e replaced by zeros)
+  constexpr int kPer = kLuNb * kLuNb / 256;
+  float lv[kPer], uv[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int q = tid + 256 * u, k = q >> 6, i = q & 63;
+    lv[u] = F[(k0 + k) * lda + min(r0 + i, e - 1)];
+    uv[u] = F[min(cc0 + k, e - 1) * lda + k0 + i];  // column k, row i of U
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int q = tid + 256 * u, k = q >> 6, i = q & 63;
+    Ls[k][i] = (r0 + i < e) ? lv[u] : 0.0f;
+    Us[i][k] = (cc0 + k < e) ? uv[u] : 0.0f;
   }
   __syncthreads();
   const int tr = (tid & 15) * 4, tc = (tid >> 4) * 4;
@@ -220,6 +241,18 @@ __global__ __launch_bounds__(256) void lu_dupdate_kernel(int64_t lda, float* __r
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+  }
+  if (r0 + kLuNb <= e && cc0 + kLuNb <= e) {  // a whole tile (uniform): its 16 loads in flight together
+    float cv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) cv[j][i] = F[(cc0 + tc + j) * lda + r0 + tr + i];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) F[(cc0 + tc + j) * lda + r0 + tr + i] = cv[j][i] - acc[i][j];
+    return;
   }
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
