@@ -293,50 +293,61 @@ __device__ __forceinline__ void store8(T* dst, const float* v) {
   }
 }
 
-// Packs of kw values per row (kw a multiple of 64: one or two panels),
+// Packs of kw values per row (kw a multiple of kPackChunk: whole panels),
 // K-contiguous, in the GEMM's element type; one workgroup (256 threads) per
 // 32 rows of the pack.  With scale != nullptr each 32-row block is multiplied
 // by block_scale(its largest magnitude over all kw values), which is stored in
 // scale[block].
 //   rows of F:    pack row q = F[r0 + q, k0 .. k0 + kw)   (strided in F: LDS transpose)
 //   columns of F: pack row q = F[k0 .. k0 + kw, c0 + q]   (contiguous in F)
+// Both read with many loads of a thread in flight (unrolled loops, clamped
+// addresses instead of guarded loads).
+constexpr int kPackChunk = 256;  // columns of F per pass of the row pack
 template <typename T>
 __global__ __launch_bounds__(256) void lu_pack_rows_kernel(int64_t lda, const float* __restrict__ F, int64_t r0,
                                                            int64_t r1, int64_t k0, int kw, T* __restrict__ dst,
                                                            float* __restrict__ scale) {
-  __shared__ float tile[32][64 + 1];
+  constexpr int KC = kPackChunk, PER = KC / 8;  // per thread and pass: PER columns of its row
+  __shared__ float tile[32][KC + 1];
   __shared__ float red[8];
   const int tid = threadIdx.x, lane32 = tid & 31, grp = tid >> 5;
   const int64_t rb = r0 + static_cast<int64_t>(blockIdx.x) * 32;
-  const int64_t r = rb + lane32;
-  const bool live = r < r1;
+  const bool live = rb + lane32 < r1;
+  const float* const frow = F + min(rb + lane32, r1 - 1);  // rows past r1 read the last row, store zeros
   float s = 1.0f;
   if (scale) {
-    float m = 0.0f;
-    if (live)
-      for (int k = grp; k < kw; k += 8) m = fmaxf(m, fabsf(F[(k0 + k) * lda + r]));
-    for (int o = 16; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 32));
-    if (lane32 == 0) red[grp] = m;
+    float m[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = grp; k < kw; k += 64) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) m[u] = fmaxf(m[u], fabsf(frow[(k0 + k + 8 * u) * lda]));
+    }
+    float mm = live ? fmaxf(fmaxf(fmaxf(m[0], m[1]), fmaxf(m[2], m[3])), fmaxf(fmaxf(m[4], m[5]), fmaxf(m[6], m[7])))
+                    : 0.0f;
+    for (int o = 16; o > 0; o >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o, 32));
+    if (lane32 == 0) red[grp] = mm;
     __syncthreads();
-    m = red[0];
-    for (int g = 1; g < 8; ++g) m = fmaxf(m, red[g]);
-    s = block_scale(m);
+    mm = red[0];
+    for (int g = 1; g < 8; ++g) mm = fmaxf(mm, red[g]);
+    s = block_scale(mm);
     if (tid == 0) scale[blockIdx.x] = s;
   }
-  const int q = tid >> 3, part = tid & 7;  // writer: pack row q, 8 values at 8 part
-  for (int kc = 0; kc < kw; kc += 64) {
-    __syncthreads();
+  const int q = tid >> 3, part = tid & 7;  // writer: pack row q, 8 values at 64 c + 8 part
+  for (int kc = 0; kc < kw; kc += KC) {
+    float v[PER];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int k = grp * 8 + i;
-      tile[lane32][k] = live ? F[(k0 + kc + k) * lda + r] : 0.0f;
-    }
+    for (int i = 0; i < PER; ++i) v[i] = frow[(k0 + kc + grp * PER + i) * lda];
+    __syncthreads();  // the previous pass's reads of the tile are done
+#pragma unroll
+    for (int i = 0; i < PER; ++i) tile[lane32][grp * PER + i] = live ? v[i] : 0.0f;
     __syncthreads();
     if (rb + q < r1) {
-      float v[8];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) v[i] = tile[q][part * 8 + i] * s;
-      store8<T>(dst + (rb - r0 + q) * kw + kc + part * 8, v);
+      for (int c = 0; c < KC / 64; ++c) {
+        float w[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) w[i] = tile[q][64 * c + part * 8 + i] * s;
+        store8<T>(dst + (rb - r0 + q) * kw + kc + 64 * c + part * 8, w);
+      }
     }
   }
 }
@@ -350,35 +361,46 @@ __global__ __launch_bounds__(256) void lu_pack_cols_kernel(int64_t lda, const fl
   const int64_t cb = c0 + static_cast<int64_t>(blockIdx.x) * 32;
   float s = 1.0f;
   if (scale) {
-    float m = 0.0f;
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int q = wave; q < 32; q += 4) {
       const int64_t c = cb + q;
       if (c >= c1) break;
       const float4* col = reinterpret_cast<const float4*>(F + c * lda + k0);
-      for (int k = lane; k < kw / 4; k += 64) {
-        const float4 v = col[k];
-        m = fmaxf(m, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+      for (int k = lane; k < kw / 4; k += 256) {  // kw / 4 a multiple of 256
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float4 v = col[k + 64 * u];
+          m[u] = fmaxf(m[u], fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+        }
       }
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if (lane == 0) red[wave] = m;
+    float mm = fmaxf(fmaxf(m[0], m[1]), fmaxf(m[2], m[3]));
+    for (int o = 32; o > 0; o >>= 1) mm = fmaxf(mm, __shfl_xor(mm, o, 64));
+    if (lane == 0) red[wave] = mm;
     __syncthreads();
-    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    s = block_scale(m);
+    mm = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    s = block_scale(mm);
     if (tid == 0) scale[blockIdx.x] = s;
   }
-  for (int q = wave; q < 32; q += 4) {  // a wave per column: 64 lanes x 16 values
+  for (int q = wave; q < 32; q += 4) {  // a wave per column: 64 lanes x 8 values per 512
     const int64_t c = cb + q;
     if (c >= c1) break;
     const float* col = F + c * lda + k0;
-    for (int k = lane * 8; k < kw; k += 512) {
-      const float4 a = *reinterpret_cast<const float4*>(col + k);
-      const float4 b = *reinterpret_cast<const float4*>(col + k + 4);
-      const float v[8] = {a.x * s, a.y * s, a.z * s, a.w * s, b.x * s, b.y * s, b.z * s, b.w * s};
-      store8<T>(dst + (c - c0) * kw + k, v);
+    for (int k0l = lane * 8; k0l < kw; k0l += 4 * 512) {  // kw a multiple of 2048 or of 512
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0l + 512 * u;
+        if (k < kw) {
+          const float4 a = *reinterpret_cast<const float4*>(col + k);
+          const float4 b = *reinterpret_cast<const float4*>(col + k + 4);
+          const float v[8] = {a.x * s, a.y * s, a.z * s, a.w * s, b.x * s, b.y * s, b.z * s, b.w * s};
+          store8<T>(dst + (c - c0) * kw + k, v);
+        }
+      }
     }
   }
 }
+static_assert(kLuPanel % kPackChunk == 0 && kLuPanel % 1024 == 0, "packs take whole chunks");
 
 // ---------------------------------------------------------------------------
 // Solves with the factors (FP64 vectors, FP32 factors), by panels of
