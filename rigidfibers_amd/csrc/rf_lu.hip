@@ -693,11 +693,13 @@ __device__ __forceinline__ void panel_inverse_block(int64_t lda, const float* __
     for (int r = ra + tid; r < rz; r += 256) {
       const float* p = F + k0 * lda + K0 + r;
       float acc[kInvCols] = {};
-#pragma unroll 4
-      for (int j = 0; j < kLuNb; ++j) {
-        const float f = p[j * lda];
+      float f[kLuNb];  // the row's 64 values, all loads in flight together
 #pragma unroll
-        for (int c = 0; c < kInvCols; ++c) acc[c] = fmaf(f, ys[c][b0 + j], acc[c]);
+      for (int j = 0; j < kLuNb; ++j) f[j] = p[j * lda];
+#pragma unroll
+      for (int j = 0; j < kLuNb; ++j) {
+#pragma unroll
+        for (int c = 0; c < kInvCols; ++c) acc[c] = fmaf(f[j], ys[c][b0 + j], acc[c]);
       }
 #pragma unroll
       for (int c = 0; c < kInvCols; ++c) ys[c][r] -= acc[c];
