@@ -38,6 +38,17 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 
 constexpr int kTile = 256;
+// Tile order: with kGemmGroupRows > 0 the grid is one-dimensional and each of
+// the 8 XCDs (workgroup w runs on XCD w % 8) takes a contiguous run of the
+// tiles ordered in groups of kGemmGroupRows row tiles (rows fastest inside a
+// group, then columns), so that the workgroups resident on an XCD share a few
+// A and B panels (with K = 3072, each tile's A and B panels are 1.5 MB each:
+// row tiles fastest, the 32 workgroups on an XCD read 32 different A panels,
+// ~400 MB over the chip, more than the 256 MB MALL; groups of 4 rows keep
+// them in it); 0: a 2-D grid, row tiles fastest.  In the LU at n = 75 000
+// (BF16 / FP16 ms per step, trailing GEMM TFLOP/s): 0 -> 488 / 438 (743 /
+// 860), 4 -> 466 / 422 (790 / 897), 8 -> 468 / 425, 16 -> 480 / 439.
+constexpr int kGemmGroupRows = 4;
 constexpr int kStageBytes = 128;
 constexpr int kLdsRowBytes = kStageBytes + 16;
 
@@ -63,8 +74,21 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave % WM) * (kTile / WM), wn = (wave / WM) * (kTile / WN);
-  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * kTile;
-  const int64_t n0 = static_cast<int64_t>(blockIdx.y) * kTile;
+  int64_t m0, n0;
+  if constexpr (kGemmGroupRows > 0) {
+    const int mt = (m + kTile - 1) / kTile, nt = (n + kTile - 1) / kTile, tiles = mt * nt;
+    const int w = static_cast<int>(blockIdx.x), per = (tiles + 7) / 8;
+    const int v = (w & 7) * per + (w >> 3);  // this XCD's run of the grouped order
+    if (v >= tiles) return;                  // padding of the grid (the whole workgroup)
+    const int first = v / (kGemmGroupRows * nt) * kGemmGroupRows;
+    const int rows = min(kGemmGroupRows, mt - first);
+    const int within = v - first * nt;
+    m0 = static_cast<int64_t>(first + within % rows) * kTile;
+    n0 = static_cast<int64_t>(within / rows) * kTile;
+  } else {
+    m0 = static_cast<int64_t>(blockIdx.x) * kTile;
+    n0 = static_cast<int64_t>(blockIdx.y) * kTile;
+  }
   const int srow = tid >> 3, spart = tid & 7;
 
   v16f acc[TM][TN];
@@ -338,9 +362,9 @@ hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K,
       m > INT32_MAX || n > INT32_MAX)
     return hipErrorInvalidValue;
   const int vec4 = ((reinterpret_cast<uintptr_t>(C) & 15) == 0 && (ldc & 3) == 0) ? 1 : 0;
-  // row tiles fastest: the workgroups resident together share one B tile
-  // (measured faster than grouping tiles per XCD)
-  const dim3 grid(static_cast<unsigned>((m + kTile - 1) / kTile), static_cast<unsigned>((n + kTile - 1) / kTile));
+  // tile order: see kGemmGroupRows
+  const unsigned mt = static_cast<unsigned>((m + kTile - 1) / kTile), nt = static_cast<unsigned>((n + kTile - 1) / kTile);
+  const dim3 grid = kGemmGroupRows > 0 ? dim3((mt * nt + 7) / 8 * 8) : dim3(mt, nt);
   const int mi = static_cast<int>(m), ni = static_cast<int>(n), ki = static_cast<int>(K);
   if ((sa == nullptr) != (sb == nullptr)) return hipErrorInvalidValue;
   switch (mode) {
