@@ -32,8 +32,11 @@ struct LuWorkspace {
   double* vec = nullptr;   // scratch vector of the solves (n doubles)
   // packs, K-contiguous, in the mode's element type (per-32-row / per-32-column
   // scales in the mixed modes); panels are taken in groups of kLuGroup (rf_lu.hip)
-  void* At = nullptr;      // L21 of the group's panels, rows right of the group: kLuGroup kLuPanel per row
-  float* sA = nullptr;
+  // L21 of a group's panels, rows below the group: kLuGroup kLuPanel per row;
+  // two, by the group's parity (the look-ahead stream packs the next group's
+  // while the main stream's update reads the current one)
+  void* At[2] = {};
+  float* sA[2] = {};
   void* Bt = nullptr;      // U12 of the group's panels, columns right of the group: kLuGroup kLuPanel per column
   float* sB = nullptr;
   // the group's own updates (at most kLuGroup - 1 panels deep): rows and

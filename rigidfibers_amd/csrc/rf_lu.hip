@@ -754,18 +754,21 @@ static unsigned grid_of(int64_t len, int per) { return static_cast<unsigned>((le
 // then the trailing update A22 -= L21 U12 with K = G W, from packs of F's
 // columns [K0, KG) and rows [K0, KG).
 // Look-ahead: the update first covers the next group's columns; the aux
-// stream then brings that group's panels up to date while the main stream
-// updates the rest.  The two streams touch disjoint columns of F and disjoint
-// buffers (the group's panels use Y, X, Ti[], Ui and the update packs of slot
-// 0; the main stream's preparation of the same group, which uses X, Ti[] and
-// slot 1, finishes before the next group's panels start).
+// stream then brings that group's panels up to date, and packs their L21 for
+// the group's trailing update, while the main stream updates the rest.  The
+// two streams touch disjoint columns of F and disjoint buffers (the group's
+// panels use Y, X, Ti[], Ui, the update packs of slot 0 and the L21 pack of
+// the group's parity; the main stream's preparation of the same group, which
+// uses X, Ti[] and slot 1, finishes before the next group's panels start, and
+// its update reads the other parity's L21 pack).
 template <typename T>
 static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuResult* res) {
   constexpr int G = kLuGroup;
   const int64_t n = ws.n, lda = ws.lda, W = kLuPanel, GW = G * kLuPanel;
   const bool scaled = mode != LuMode::kFP32;
   float* F = ws.F;
-  T* Lj = static_cast<T*>(ws.At);
+  auto Lj = [&](int64_t K0) { return static_cast<T*>(ws.At[(K0 / GW) & 1]); };  // by the group's parity
+  auto sAj = [&](int64_t K0) { return ws.sA[(K0 / GW) & 1]; };
   T* Uj = static_cast<T*>(ws.Bt);
   T* X = static_cast<T*>(ws.Xm);
   T* Y = static_cast<T*>(ws.Ya);
@@ -850,10 +853,15 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
       if (i > 0 && (ee = update(Ki, n, Ki, KEi, K0, i * W, 0, st, false)) != hipSuccess) return ee;
       if ((ee = factor_panel(Ki, KEi, i, st)) != hipSuccess) return ee;
     }
-    return ee;
+    // the L21 pack of the trailing update, here (off the main stream for the
+    // look-ahead groups): the group's L21 rows below it are final
+    if (K0 + GW < n)
+      hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - K0 - GW, 32)), dim3(256), 0, st, lda, F, K0 + GW,
+                         n, K0, static_cast<int>(GW), Lj(K0), sc(sAj(K0)));
+    return hipGetLastError();
   };
-  // the group's rows of the columns right of it, and the packs of the update
-  // (main stream); the group's panels are all full here
+  // the group's rows of the columns right of it, and the U12 pack of the
+  // update (main stream); the group's panels are all full here
   auto group_prepare = [&](int64_t K0) -> hipError_t {
     const int64_t KG = K0 + GW;
     hipError_t ee = hipSuccess;
@@ -862,15 +870,13 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
       if (j > 0 && (ee = update(Kj, Kj + W, KG, n, K0, j * W, 1, s, true)) != hipSuccess) return ee;
       if ((ee = form_u12(Kj, KG, n, j, s)) != hipSuccess) return ee;
     }
-    hipLaunchKernelGGL((lu_pack_rows_kernel<T>), dim3(grid_of(n - KG, 32)), dim3(256), 0, s, lda, F, KG, n, K0,
-                       static_cast<int>(GW), Lj, sc(ws.sA));
     hipLaunchKernelGGL((lu_pack_cols_kernel<T>), dim3(grid_of(n - KG, 32)), dim3(256), 0, s, lda, F, KG, n, K0,
                        static_cast<int>(GW), Uj, sc(ws.sB));
     return hipGetLastError();
   };
   // A22[KG:n, c0:c1] -= L21 U12[c0:c1], K = G W
   auto trailing = [&](int64_t KG, int64_t c0, int64_t c1) -> hipError_t {
-    return sub(n - KG, c1 - c0, GW, Lj, Uj + (c0 - KG) * GW, at(KG, c0), sc(ws.sA),
+    return sub(n - KG, c1 - c0, GW, Lj(KG - GW), Uj + (c0 - KG) * GW, at(KG, c0), sc(sAj(KG - GW)),
                scaled ? ws.sB + (c0 - KG) / 32 : nullptr, s, true);
   };
   if ((e = group_panels(0, s)) != hipSuccess) return e;
@@ -952,7 +958,7 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   constexpr int G = kLuGroup;
   const size_t wide = esz * static_cast<size_t>(n) * (G - 1) * kLuPanel;     // n rows, (G - 1) panels deep
   const size_t narrow = esz * static_cast<size_t>(kLuPanel) * (G - 1) * kLuPanel;
-  for (void** p : {&ws->At, &ws->Bt})  // the group's panels
+  for (void** p : {&ws->At[0], &ws->At[1], &ws->Bt})  // the group's panels
     if ((e = hipMalloc(p, G * pack)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->Pr[0], wide)) != hipSuccess) return e;
   if ((e = hipMalloc(&ws->Pc[0], narrow)) != hipSuccess) return e;
@@ -966,7 +972,7 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   if (sol > 0)
     for (float** p : {&ws->Tsol, &ws->Usol})
       if ((e = hipMalloc(p, sol)) != hipSuccess) return e;
-  for (float** p : {&ws->sA, &ws->sB, &ws->sPr[0], &ws->sPr[1], &ws->sPc[0], &ws->sPc[1]})
+  for (float** p : {&ws->sA[0], &ws->sA[1], &ws->sB, &ws->sPr[0], &ws->sPr[1], &ws->sPc[0], &ws->sPc[1]})
     if ((e = hipMalloc(p, scales)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&ws->aux, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipEventCreateWithFlags(&ws->next_cols, hipEventDisableTiming)) != hipSuccess) return e;
@@ -975,8 +981,8 @@ hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace*
   for (auto& ev : ws->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess) return e;
   ws->bytes = static_cast<int64_t>(sizeof(float) * static_cast<size_t>(lda) * n + sizeof(int) * (n + 1) +
-                                   sizeof(double) * static_cast<size_t>(n) + (2 * G + 2) * pack + 2 * wide +
-                                   2 * narrow + (G + 1) * inv + 2 * sol + 6 * scales + dinv);
+                                   sizeof(double) * static_cast<size_t>(n) + (3 * G + 2) * pack + 2 * wide +
+                                   2 * narrow + (G + 1) * inv + 2 * sol + 7 * scales + dinv);
   return hipSuccess;
 }
 
@@ -985,8 +991,9 @@ int64_t lu_workspace_bytes(const LuWorkspace& ws) { return ws.bytes; }
 void lu_workspace_destroy(LuWorkspace* ws) {
   if (ws->aux) (void)hipStreamSynchronize(ws->aux);
   for (void* p : {static_cast<void*>(ws->F), static_cast<void*>(ws->perm), static_cast<void*>(ws->info),
-                  static_cast<void*>(ws->vec), ws->At, ws->Bt, ws->Pr[0], ws->Pr[1], ws->Pc[0], ws->Pc[1], ws->Xm,
-                  ws->Ya, ws->Tinv, ws->Uinv, static_cast<void*>(ws->sA), static_cast<void*>(ws->sB),
+                  static_cast<void*>(ws->vec), ws->At[0], ws->At[1], ws->Bt, ws->Pr[0], ws->Pr[1], ws->Pc[0],
+                  ws->Pc[1], ws->Xm, ws->Ya, ws->Tinv, ws->Uinv, static_cast<void*>(ws->sA[0]),
+                  static_cast<void*>(ws->sA[1]), static_cast<void*>(ws->sB),
                   static_cast<void*>(ws->sPr[0]), static_cast<void*>(ws->sPr[1]), static_cast<void*>(ws->sPc[0]),
                   static_cast<void*>(ws->sPc[1]), static_cast<void*>(ws->Dinv), static_cast<void*>(ws->Tsol),
                   static_cast<void*>(ws->Usol)})
