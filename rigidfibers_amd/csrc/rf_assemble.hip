@@ -966,8 +966,17 @@ __global__ __launch_bounds__(256) void expand_theta_dense_kernel(int M, int64_t 
   __shared__ __align__(16) float Ts[kPer * 256 / (kExpEntries / 4) * kExpRow];
   const int tid = threadIdx.x;
   const int64_t L = static_cast<int64_t>(M) * N;
-  const int64_t e0 = static_cast<int64_t>(blockIdx.x) * kExpEntries;
-  const int i0 = static_cast<int>(blockIdx.y) * TI;
+  // Tile order: workgroup w runs on XCD w % 8; each XCD takes a contiguous
+  // run of tiles, target groups fastest, so the column runs that continue
+  // each other (and share their first and last 128-byte lines) are written
+  // one after another through the same L2.
+  const int ny = (M + TI - 1) / TI;
+  const int tiles = static_cast<int>((L + kExpEntries - 1) / kExpEntries) * ny;
+  const int per = (tiles + 7) / 8;
+  const int v = (static_cast<int>(blockIdx.x) & 7) * per + (static_cast<int>(blockIdx.x) >> 3);
+  if (v >= tiles) return;  // uniform over the workgroup, before any barrier
+  const int64_t e0 = static_cast<int64_t>(v / ny) * kExpEntries;
+  const int i0 = (v % ny) * TI;
   // stage: row (ti, c) holds T[i0 + ti][c][e0 .. e0 + 31] (within the padded
   // row of Lp >= e0 + 32 numbers).  Every load of the thread is in flight
   // before the LDS stores, which are unconditional (a guarded store would let
@@ -1033,14 +1042,16 @@ __global__ __launch_bounds__(256) void expand_theta_dense_kernel(int M, int64_t 
 hipError_t launch_expand_theta_dense(int N, int M, int64_t Lp, const float* T, const double* ori,
                                      const PhysConsts& pc, float* A, int64_t lda, hipStream_t s) {
   if (M < 1 || lda < static_cast<int64_t>(3) * N * M || Lp % kExpEntries != 0) return hipErrorInvalidValue;
-  const unsigned ex = static_cast<unsigned>((static_cast<int64_t>(M) * N + kExpEntries - 1) / kExpEntries);
+  const int64_t ex = (static_cast<int64_t>(M) * N + kExpEntries - 1) / kExpEntries;
   switch (N) {
 #define RF_EXD(NN)                                                                                          \
-  case NN:                                                                                                  \
-    hipLaunchKernelGGL(expand_theta_dense_kernel<NN>,                                                      \
-                       dim3(ex, static_cast<unsigned>((M + exp_targets<NN>() - 1) / exp_targets<NN>())),   \
+  case NN: {                                                                                                \
+    const int64_t tiles = ex * ((M + exp_targets<NN>() - 1) / exp_targets<NN>());                          \
+    if (tiles > INT32_MAX - 8) return hipErrorInvalidValue;                                                 \
+    hipLaunchKernelGGL(expand_theta_dense_kernel<NN>, dim3(static_cast<unsigned>((tiles + 7) / 8 * 8)),     \
                        dim3(256), 0, s, M, Lp, T, ori, pc, A, lda);                                         \
-    break;
+    break;                                                                                                  \
+  }
     RF_EXD(1)
     RF_EXD(2)
     RF_EXD(3)

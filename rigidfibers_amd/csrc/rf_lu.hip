@@ -413,34 +413,40 @@ static_assert(kLuPanel % kPackChunk == 0 && kLuPanel % 1024 == 0, "packs take wh
 // inverse, FP64 substitution on the FP32 factors.  Stored transposed,
 // Dinv[b][0][c][r] = (L_bb^-1)[r][c] and Dinv[b][1][c][r] = (U_bb^-1)[r][c],
 // so that lane r of a solve reads row r of the inverse coalesced.
-__global__ __launch_bounds__(64) void lu_block_inverse_kernel(int64_t n, int64_t lda, const float* __restrict__ F,
-                                                              int64_t first_block, double* __restrict__ Dinv) {
+__global__ __launch_bounds__(128) void lu_block_inverse_kernel(int64_t n, int64_t lda, const float* __restrict__ F,
+                                                               int64_t first_block, double* __restrict__ Dinv) {
   __shared__ float blk[kLuNb][kLuNb + 1];  // [col][row]
-  const int lane = threadIdx.x;
+  // X[w][r][c]: wavefront w's solution (w = 0: L^-1, w = 1: U^-1), lane c's
+  // column (one-off kernel: LDS, not registers)
+  __shared__ double X[2][kLuNb][kLuNb];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t bidx = first_block + blockIdx.x;
   const int64_t k0 = bidx * kLuNb;
   const int kb = static_cast<int>(min(static_cast<int64_t>(kLuNb), n - k0));
 #pragma unroll 8
-  for (int c = 0; c < kLuNb; ++c)
+  for (int c = w; c < kLuNb; c += 2)
     blk[c][lane] = (c < kb && lane < kb) ? F[(k0 + c) * lda + k0 + lane] : (c == lane ? 1.0f : 0.0f);
   __syncthreads();
-  double* Li = Dinv + bidx * 2 * kLuNb * kLuNb;
-  double* Ui = Li + kLuNb * kLuNb;
-  __shared__ double X[kLuNb][kLuNb];  // X[r][c]: lane c's column (one-off kernel: LDS, not registers)
-  // L X = I (unit lower): x_r = delta_rc - sum_{j<r} L[r][j] x_j
-  for (int r = 0; r < kLuNb; ++r) {
-    double v = (r == lane) ? 1.0 : 0.0;
-    for (int j = 0; j < r; ++j) v = fma(-static_cast<double>(blk[j][r]), X[j][lane], v);
-    X[r][lane] = v;
+  double (*Xw)[kLuNb] = X[w];
+  double* out = Dinv + (bidx * 2 + w) * kLuNb * kLuNb;
+  if (w == 0) {
+    // L X = I (unit lower): x_r = delta_rc - sum_{j<r} L[r][j] x_j
+    for (int r = 0; r < kLuNb; ++r) {
+      double v = (r == lane) ? 1.0 : 0.0;
+#pragma unroll 8
+      for (int j = 0; j < r; ++j) v = fma(-static_cast<double>(blk[j][r]), Xw[j][lane], v);
+      Xw[r][lane] = v;
+    }
+  } else {
+    // U X = I: x_r = (delta_rc - sum_{j>r} U[r][j] x_j) / U[r][r]
+    for (int r = kLuNb - 1; r >= 0; --r) {
+      double v = (r == lane) ? 1.0 : 0.0;
+#pragma unroll 8
+      for (int j = r + 1; j < kLuNb; ++j) v = fma(-static_cast<double>(blk[j][r]), Xw[j][lane], v);
+      Xw[r][lane] = v / static_cast<double>(blk[r][r]);
+    }
   }
-  for (int r = 0; r < kLuNb; ++r) Li[lane * kLuNb + r] = X[r][lane];
-  // U X = I: x_r = (delta_rc - sum_{j>r} U[r][j] x_j) / U[r][r]
-  for (int r = kLuNb - 1; r >= 0; --r) {
-    double v = (r == lane) ? 1.0 : 0.0;
-    for (int j = r + 1; j < kLuNb; ++j) v = fma(-static_cast<double>(blk[j][r]), X[j][lane], v);
-    X[r][lane] = v / static_cast<double>(blk[r][r]);
-  }
-  for (int r = 0; r < kLuNb; ++r) Ui[lane * kLuNb + r] = X[r][lane];
+  for (int r = 0; r < kLuNb; ++r) out[lane * kLuNb + r] = Xw[r][lane];
 }
 
 // sum_c F[(k0 + c) lda + row] ys[c] over one block's columns (p = F + k0 lda +
@@ -818,7 +824,7 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
                            st, lda, F, k0, k1, KE);
       }
     }
-    hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(KE - K0, kLuNb)), dim3(64), 0, st, n, lda, F,
+    hipLaunchKernelGGL(lu_block_inverse_kernel, dim3(grid_of(KE - K0, kLuNb)), dim3(128), 0, st, n, lda, F,
                        K0 / kLuNb, ws.Dinv);
     if (KE >= n) return hipGetLastError();
     hipLaunchKernelGGL((lu_panel_inverses_kernel<T>), dim3(2 * W / kInvCols), dim3(256), 0, st, lda, F, ws.perm,
