@@ -364,9 +364,8 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
   __shared__ double bred[PAIRS][NR];
   __shared__ float modes[2 * kMaxTerms];  // gamma_k, then eigen_k
   // FP16 copy: each source slot's largest |projection| (float bits), one
-  // buffer per sweep parity; the inverse scale of each (source, mode) column
+  // buffer per sweep parity
   __shared__ unsigned smax[2][PAIRS];
-  __shared__ __align__(16) float ivcol[PN];
 
   const int tid = threadIdx.x;
   const int p = tid / Q;
@@ -482,38 +481,20 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
       }
       return val;
     };
-    // FP16 copy: the sweep's block scales (S16) and each column's inverse
-    // scale, once per sweep, so that the jobs below read four with one load
-    _Float16* const T16col = HALF ? T16 + static_cast<int64_t>(i) * NC * Lp + static_cast<int64_t>(j0) * N : nullptr;
+    // FP16 copy: the sweep's block scales (S16); the values are converted
+    // when the workgroup reads its rows back after the sweeps
     if constexpr (HALF) {
-      if (tid < PN) {
+      if (tid < PAIRS && j0 + tid < je) {
         float sv, iv;
-        half_block_scale(smax[par][tid / N], sv, iv);
-        ivcol[tid] = iv;
-        if (tid % N == 0 && j0 + tid / N < je) S16[static_cast<int64_t>(i) * M + j0 + tid / N] = sv;
+        half_block_scale(smax[par][tid], sv, iv);
+        S16[static_cast<int64_t>(i) * M + j0 + tid] = sv;
       }
-      __syncthreads();
     }
     for (int job = tid; job < NC * NV; job += kPairThreads) {
       const int u = job / NV;
       const int pm0 = 4 * (job - u * NV);
       floatx4 val = sys_val(u, pm0);
       put(Tcol, static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0), pm0, val);
-      if constexpr (HALF) {  // the same values (self pair zeroed by put) on the FP16 copy
-        typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
-        const floatx4 iv = *reinterpret_cast<const floatx4*>(ivcol + pm0);
-        halfx4 h;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) h[e] = static_cast<_Float16>(val[e] * iv[e]);
-        _Float16* dst = T16col + (static_cast<uint32_t>(u) * lp32 + static_cast<uint32_t>(pm0));
-        if (full || pm0 + 3 < npm) {
-          *reinterpret_cast<halfx4*>(dst) = h;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (pm0 + e < npm) dst[e] = h[e];
-        }
-      }
     }
     // velocity operator: Theta_0 (projection 1), then Phi_0 (projection 0).
     // Analytical form: its jobs start on the lanes after the system
@@ -592,12 +573,18 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     r0v = serving->V;
     npart = serving->npart;
   }
-  double out = 0.0;     // lane tid < ROWS: row tid of the off-diagonal product A x0
-  if (x0 != nullptr) {  // the stores above are visible to the workgroup after the barrier
-    constexpr int NACC = 1 + 3 * (N - 1);  // s0, then U_k (k >= 1) as in gemv_theta_block
-    double acc[NACC];
+  double out = 0.0;  // lane tid < ROWS: row tid of the off-diagonal product A x0
+  const bool have_x = x0 != nullptr;
+  // The workgroup reads back the rows it stored (visible to it after the
+  // barrier above): for the product with x0, and for the FP16 copy, which
+  // converts them with the block scales the sweeps stored (the values the
+  // sweeps wrote, so the copy is theirs bit for bit; the self pair's columns
+  // are zero already).
+  constexpr int NACC = 1 + 3 * (N - 1);  // s0, then U_k (k >= 1) as in gemv_theta_block
+  double acc[NACC];
 #pragma unroll
-    for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+  for (int a = 0; a < NACC; ++a) acc[a] = 0.0;
+  if (have_x || HALF) {
     // column (i-row block, e): s0 += Q0 . x_e, U_k += Theta_k x_e
     auto add_column = [&](const float* q, double xa, double xb, double xc) {
       acc[0] = fma(static_cast<double>(q[0]), xa,
@@ -619,14 +606,34 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
     // aligned), then the remaining column one per lane.
     const int ebeg = jb * N, eend = je * N;
     const int efull = ebeg + (eend - ebeg) / 2 * 2;
+    // FP16 copy: the inverse of column e's block scale (1 / sv exactly, sv a
+    // power of two; 0 for a zero or non-finite scale, as half_block_scale)
+    _Float16* const Hi = HALF ? T16 + static_cast<int64_t>(i) * NC * Lp : nullptr;
+    auto inv_scale = [&](int e) {
+      const unsigned es = (__float_as_uint(S16[static_cast<int64_t>(i) * M + e / N]) >> 23) & 0xffu;
+      return (es >= 1u && es < 255u) ? __uint_as_float((254u - es) << 23) : 0.0f;
+    };
     for (int e0 = ebeg + 2 * tid; e0 < efull; e0 += 2 * kPairThreads) {
+      const float* q = Ti + e0;
+      auto ld = [&](int comp) { return *reinterpret_cast<const floatx2*>(q + static_cast<int64_t>(comp) * Lp); };
+      if constexpr (HALF) {
+        typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+        const float iv0 = inv_scale(e0), iv1 = inv_scale(e0 + 1);
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp) {
+          const floatx2 v = ld(comp);
+          halfx2 h;
+          h[0] = static_cast<_Float16>(v[0] * iv0);
+          h[1] = static_cast<_Float16>(v[1] * iv1);
+          *reinterpret_cast<halfx2*>(Hi + static_cast<int64_t>(comp) * Lp + e0) = h;
+        }
+        if (!have_x) continue;
+      }
       double xv[2][3];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
 #pragma unroll
         for (int a = 0; a < 3; ++a) xv[c][a] = x0[3 * static_cast<int64_t>(e0 + c) + a];
-      const float* q = Ti + e0;
-      auto ld = [&](int comp) { return *reinterpret_cast<const floatx2*>(q + static_cast<int64_t>(comp) * Lp); };
       {
         const floatx2 q0 = ld(0), q1 = ld(1), q2 = ld(2);
 #pragma unroll
@@ -650,9 +657,19 @@ __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64
         }
       }
     }
-    for (int e = efull + tid; e < eend; e += kPairThreads)
-      add_column(Ti + e, x0[3 * static_cast<int64_t>(e)], x0[3 * static_cast<int64_t>(e) + 1],
-                 x0[3 * static_cast<int64_t>(e) + 2]);
+    for (int e = efull + tid; e < eend; e += kPairThreads) {
+      if constexpr (HALF) {
+        const float iv = inv_scale(e);
+#pragma unroll
+        for (int comp = 0; comp < NC; ++comp)
+          Hi[static_cast<int64_t>(comp) * Lp + e] = static_cast<_Float16>(Ti[static_cast<int64_t>(comp) * Lp + e] * iv);
+      }
+      if (have_x)
+        add_column(Ti + e, x0[3 * static_cast<int64_t>(e)], x0[3 * static_cast<int64_t>(e) + 1],
+                   x0[3 * static_cast<int64_t>(e) + 2]);
+    }
+  }
+  if (have_x) {  // uniform over the workgroup: the barrier below is reached by all or none
     constexpr int WAVES = kPairThreads / 64;
     static_assert(WAVES * NACC * sizeof(double) <= RP * Q * sizeof(float), "partials fit the freed G");
     double* red = reinterpret_cast<double*>(G);  // G is free after the last sweep
