@@ -337,6 +337,9 @@ __device__ __forceinline__ void half_block_scale(unsigned gbits, float& sv, floa
 // apart[gy n + row] (the theta GEMV's arithmetic: FP32 values times FP64
 // x0, FP64 sums).  The read is HBM traffic while the other workgroups of the
 // CU keep its VALUs busy, which is what makes it cheaper than a matvec.
+// HALF (GMRES-IR's FP16 copy T16, S16): phase 2 also takes each (target,
+// source) block's largest |projection|, phase 3 stores the block scales, and
+// the read-back (run then with or without x0) converts the stored values.
 template <int N, int NQI, bool ANALYTIC, bool HALF = false>
 __device__ __forceinline__ void assemble_theta_block(int i, int gy, int M, int64_t Lp, const double* __restrict__ pos,
                                                      const double* __restrict__ ori, float* __restrict__ T,
