@@ -52,7 +52,10 @@ constexpr int kGemmGroupRows = 4;
 constexpr int kStageBytes = 128;
 constexpr int kLdsRowBytes = kStageBytes + 16;
 
-template <typename T, int WM, int WN, int PF, bool STORE>
+// PERSIST: a grid smaller than the tile count; workgroup b takes the tiles of
+// the virtual workgroups b, b + gridDim.x, ... (gridDim.x a multiple of 8, so
+// that every one of them runs on b's XCD), with a barrier between tiles.
+template <typename T, int WM, int WN, int PF, bool STORE, bool PERSIST = false>
 __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, int n, int K,
                                                                        const T* __restrict__ At, int64_t ldak,
                                                                        const T* __restrict__ Bt, int64_t ldbk,
@@ -71,15 +74,24 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
   __shared__ __attribute__((aligned(16))) T As[2][kTile][RS];
   __shared__ __attribute__((aligned(16))) T Bs[2][kTile][RS];
 
+  static_assert(!PERSIST || kGemmGroupRows > 0, "the persistent grid walks the grouped order");
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave % WM) * (kTile / WM), wn = (wave / WM) * (kTile / WN);
+  for (int vw = static_cast<int>(blockIdx.x);; vw += static_cast<int>(gridDim.x)) {  // virtual workgroup
   int64_t m0, n0;
   if constexpr (kGemmGroupRows > 0) {
     const int mt = (m + kTile - 1) / kTile, nt = (n + kTile - 1) / kTile, tiles = mt * nt;
-    const int w = static_cast<int>(blockIdx.x), per = (tiles + 7) / 8;
-    const int v = (w & 7) * per + (w >> 3);  // this XCD's run of the grouped order
-    if (v >= tiles) return;                  // padding of the grid (the whole workgroup)
+    const int per = (tiles + 7) / 8;
+    if constexpr (PERSIST) {
+      if (vw >= 8 * per) return;           // every virtual workgroup done
+      if (vw != static_cast<int>(blockIdx.x)) __syncthreads();  // the previous tile's LDS reads are done
+    }
+    const int v = (vw & 7) * per + (vw >> 3);  // this XCD's run of the grouped order
+    if (v >= tiles) {                        // padding of the grid (the whole workgroup)
+      if constexpr (PERSIST) continue;
+      return;
+    }
     const int first = v / (kGemmGroupRows * nt) * kGemmGroupRows;
     const int rows = min(kGemmGroupRows, mt - first);
     const int within = v - first * nt;
@@ -263,6 +275,7 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
           }
         }
     }
+    if constexpr (PERSIST) continue;
     return;
   }
 #pragma unroll
@@ -307,6 +320,8 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
         }
       }
     }
+  if constexpr (!PERSIST) return;
+  }
 }
 
 // Launch shape per element type: 8 waves of 128x64 (two per SIMD, 128
@@ -332,9 +347,13 @@ int gemm_stage_k(LuMode mode) { return mode == LuMode::kFP32 ? kStageBytes / 4 :
 template <typename T>
 static void launch_gemm(dim3 grid, bool store, int m, int n, int K, const void* At, int64_t ldak, const void* Bt,
                         int64_t ldbk, float* C, int64_t ldc, int vec4, const float* sa, const float* sb,
-                        hipStream_t s) {
+                        hipStream_t s, bool persist) {
   using S = GemmShape<T>;
-  if (store)
+  if (persist)
+    hipLaunchKernelGGL((gemm_packed_sub_kernel<T, S::WM, S::WN, S::PF, false, true>), grid, dim3(64 * S::WM * S::WN),
+                       0, s, m, n, K, static_cast<const T*>(At), ldak, static_cast<const T*>(Bt), ldbk, C, ldc, vec4,
+                       sa, sb);
+  else if (store)
     hipLaunchKernelGGL((gemm_packed_sub_kernel<T, S::WM, S::WN, S::PF, true>), grid, dim3(64 * S::WM * S::WN), 0, s,
                        m, n, K, static_cast<const T*>(At), ldak, static_cast<const T*>(Bt), ldbk, C, ldc, vec4, sa,
                        sb);
@@ -346,13 +365,13 @@ static void launch_gemm(dim3 grid, bool store, int m, int n, int K, const void* 
 
 hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
                            const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s, const float* sa,
-                           const float* sb) {
-  return gemm_packed(mode, false, m, n, K, At, ldak, Bt, ldbk, C, ldc, s, sa, sb);
+                           const float* sb, int max_workgroups) {
+  return gemm_packed(mode, false, m, n, K, At, ldak, Bt, ldbk, C, ldc, s, sa, sb, max_workgroups);
 }
 
 hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
                        const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s, const float* sa,
-                       const float* sb) {
+                       const float* sb, int max_workgroups) {
   if (m <= 0 || n <= 0 || K <= 0) return hipSuccess;
   const int bk = gemm_stage_k(mode);
   const int esz = mode == LuMode::kFP32 ? 4 : 2;
@@ -364,18 +383,23 @@ hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K,
   const int vec4 = ((reinterpret_cast<uintptr_t>(C) & 15) == 0 && (ldc & 3) == 0) ? 1 : 0;
   // tile order: see kGemmGroupRows
   const unsigned mt = static_cast<unsigned>((m + kTile - 1) / kTile), nt = static_cast<unsigned>((n + kTile - 1) / kTile);
-  const dim3 grid = kGemmGroupRows > 0 ? dim3((mt * nt + 7) / 8 * 8) : dim3(mt, nt);
+  dim3 grid = kGemmGroupRows > 0 ? dim3((mt * nt + 7) / 8 * 8) : dim3(mt, nt);
+  // a cap below the tile count (subtracting form): the persistent form on a
+  // whole number of workgroups per XCD
+  const int cap = max_workgroups / 8 * 8;
+  const bool persist = !store && kGemmGroupRows > 0 && cap > 0 && grid.x > static_cast<unsigned>(cap);
+  if (persist) grid = dim3(static_cast<unsigned>(cap));
   const int mi = static_cast<int>(m), ni = static_cast<int>(n), ki = static_cast<int>(K);
   if ((sa == nullptr) != (sb == nullptr)) return hipErrorInvalidValue;
   switch (mode) {
     case LuMode::kFP32:
-      launch_gemm<float>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
+      launch_gemm<float>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s, persist);
       break;
     case LuMode::kBF16:
-      launch_gemm<__bf16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
+      launch_gemm<__bf16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s, persist);
       break;
     case LuMode::kFP16:
-      launch_gemm<_Float16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s);
+      launch_gemm<_Float16>(grid, store, mi, ni, ki, At, ldak, Bt, ldbk, C, ldc, vec4, sa, sb, s, persist);
       break;
   }
   return hipGetLastError();

@@ -22,6 +22,17 @@ constexpr int kLuPanel = 1024;  // outer panel
 // (799 / 902: the longer in-group chains on the look-ahead stream cost more
 // than the faster update saves).
 constexpr int kLuGroup = 3;
+// The trailing update beside the look-ahead's panels, once at most
+// kLuTailRows rows remain (where the panels' chain is the longer of the two):
+// kLuTailWorkgroups workgroups (one per CU) take its tiles, the other CUs run
+// the chain's small kernels as they come instead of waiting for a tile to end.
+// Measured at n = 75 000 (BF16 / FP16 ms per step, interleaved runs on two
+// boxes, against 448.6 / 409.5 and 455.7 / 414.0 without): 24 576 rows, 224
+// workgroups -2 / -2; 36 864, 224: -4 / -4.5; 49 152, 224: -4.4 / -2.9;
+// 36 864, 192: -8.1 / -7.8 and -8.5 / -8.7; 36 864, 160: -8.0 / -6.5;
+// 49 152, 192: -2.8 / -1.5.
+constexpr int64_t kLuTailRows = 36864;
+constexpr int kLuTailWorkgroups = 192;
 
 struct LuWorkspace {
   LuMode mode = LuMode::kFP32;
@@ -79,12 +90,15 @@ struct LuResult {
 // hold row block r / 32 of A times sa[r / 32] and column block c / 32 of B
 // times sb[c / 32] (powers of two), which the update divides out.
 int gemm_stage_k(LuMode mode);
+// max_workgroups > 0 (subtracting form): at most that many workgroups (rounded
+// down to a multiple of 8), each taking several tiles, so that the rest of the
+// CUs stay free for work on other streams.
 hipError_t gemm_packed(LuMode mode, bool store, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
                        const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s,
-                       const float* sa = nullptr, const float* sb = nullptr);
+                       const float* sa = nullptr, const float* sb = nullptr, int max_workgroups = 0);
 hipError_t gemm_packed_sub(LuMode mode, int64_t m, int64_t n, int64_t K, const void* At, int64_t ldak,
                            const void* Bt, int64_t ldbk, float* C, int64_t ldc, hipStream_t s,
-                           const float* sa = nullptr, const float* sb = nullptr);
+                           const float* sa = nullptr, const float* sb = nullptr, int max_workgroups = 0);
 
 hipError_t lu_workspace_create(LuMode mode, int64_t n, int64_t lda, LuWorkspace* ws);
 int64_t lu_workspace_bytes(const LuWorkspace& ws);

@@ -787,10 +787,10 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
   // C -= A B on stream st; the updates of the columns right of a group (main
   // stream, `count`) are timed and their flops counted
   auto sub = [&](int64_t m, int64_t nc, int64_t K, const T* A, const T* B, float* C, const float* sa,
-                 const float* sb, hipStream_t st, bool count) -> hipError_t {
+                 const float* sb, hipStream_t st, bool count, int cap = 0) -> hipError_t {
     const bool timed = count && nev < LuWorkspace::kEvents;
     if (timed) (void)hipEventRecord(ws.ev[2 * nev], s);
-    hipError_t ee = gemm_packed(mode, false, m, nc, K, A, K, B, K, C, lda, st, sa, sb);
+    hipError_t ee = gemm_packed(mode, false, m, nc, K, A, K, B, K, C, lda, st, sa, sb, cap);
     if (ee != hipSuccess) return ee;
     if (timed) {
       (void)hipEventRecord(ws.ev[2 * nev + 1], s);
@@ -881,9 +881,9 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     return hipGetLastError();
   };
   // A22[KG:n, c0:c1] -= L21 U12[c0:c1], K = G W
-  auto trailing = [&](int64_t KG, int64_t c0, int64_t c1) -> hipError_t {
+  auto trailing = [&](int64_t KG, int64_t c0, int64_t c1, int cap) -> hipError_t {
     return sub(n - KG, c1 - c0, GW, Lj(KG - GW), Uj + (c0 - KG) * GW, at(KG, c0), sc(sAj(KG - GW)),
-               scaled ? ws.sB + (c0 - KG) / 32 : nullptr, s, true);
+               scaled ? ws.sB + (c0 - KG) / 32 : nullptr, s, true, cap);
   };
   if ((e = group_panels(0, s)) != hipSuccess) return e;
   for (int64_t K0 = 0; K0 + GW < n; K0 += GW) {
@@ -892,12 +892,15 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     // the next group's columns, then (look-ahead) that group's panels on the
     // aux stream while the main stream updates the rest
     const int64_t NE = std::min<int64_t>(n, KG + GW);
-    if ((e = trailing(KG, KG, NE)) != hipSuccess) return e;
+    if ((e = trailing(KG, KG, NE, 0)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_cols, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(ws.aux, ws.next_cols, 0)) != hipSuccess) return e;
     if ((e = group_panels(KG, ws.aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
-    if (NE < n && (e = trailing(KG, NE, n)) != hipSuccess) return e;
+    // in the last groups the panels' chain outlasts this update: it leaves
+    // CUs free for the chain's small kernels
+    const int cap = (n - KG <= kLuTailRows) ? kLuTailWorkgroups : 0;
+    if (NE < n && (e = trailing(KG, NE, n, cap)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
   res->gemm_launches = nev;
