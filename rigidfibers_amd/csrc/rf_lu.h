@@ -22,8 +22,8 @@ constexpr int kLuPanel = 1024;  // outer panel
 // (799 / 902: the longer in-group chains on the look-ahead stream cost more
 // than the faster update saves).
 constexpr int kLuGroup = 3;
-// The trailing update beside the look-ahead's panels, once at most
-// kLuTailRows rows remain (where the panels' chain is the longer of the two):
+// The trailing update beside the look-ahead's panels (BF16 / FP16), once at
+// most kLuTailRows rows remain (where the panels' chain is the longer of the two):
 // kLuTailWorkgroups workgroups (one per CU) take its tiles, the other CUs run
 // the chain's small kernels as they come instead of waiting for a tile to end.
 // Measured at n = 75 000 (BF16 / FP16 ms per step, interleaved runs on two

@@ -898,8 +898,9 @@ static hipError_t lu_factor_t(LuWorkspace& ws, LuMode mode, hipStream_t s, LuRes
     if ((e = group_panels(KG, ws.aux)) != hipSuccess) return e;
     if ((e = hipEventRecord(ws.next_done, ws.aux)) != hipSuccess) return e;
     // in the last groups the panels' chain outlasts this update: it leaves
-    // CUs free for the chain's small kernels
-    const int cap = (n - KG <= kLuTailRows) ? kLuTailWorkgroups : 0;
+    // CUs free for the chain's small kernels (the mixed modes; the FP32
+    // update is several times longer than the chain)
+    const int cap = (mode != LuMode::kFP32 && n - KG <= kLuTailRows) ? kLuTailWorkgroups : 0;
     if (NE < n && (e = trailing(KG, NE, n, cap)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(s, ws.next_done, 0)) != hipSuccess) return e;
   }
