@@ -79,248 +79,248 @@ __global__ __launch_bounds__(64 * WM * WN) void gemm_packed_sub_kernel(int m, in
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = (wave % WM) * (kTile / WM), wn = (wave / WM) * (kTile / WN);
   for (int vw = static_cast<int>(blockIdx.x);; vw += static_cast<int>(gridDim.x)) {  // virtual workgroup
-  int64_t m0, n0;
-  if constexpr (kGemmGroupRows > 0) {
-    const int mt = (m + kTile - 1) / kTile, nt = (n + kTile - 1) / kTile, tiles = mt * nt;
-    const int per = (tiles + 7) / 8;
-    if constexpr (PERSIST) {
-      if (vw >= 8 * per) return;           // every virtual workgroup done
-      if (vw != static_cast<int>(blockIdx.x)) __syncthreads();  // the previous tile's LDS reads are done
-    }
-    const int v = (vw & 7) * per + (vw >> 3);  // this XCD's run of the grouped order
-    if (v >= tiles) {                        // padding of the grid (the whole workgroup)
-      if constexpr (PERSIST) continue;
-      return;
-    }
-    const int first = v / (kGemmGroupRows * nt) * kGemmGroupRows;
-    const int rows = min(kGemmGroupRows, mt - first);
-    const int within = v - first * nt;
-    m0 = static_cast<int64_t>(first + within % rows) * kTile;
-    n0 = static_cast<int64_t>(within / rows) * kTile;
-  } else {
-    m0 = static_cast<int64_t>(blockIdx.x) * kTile;
-    n0 = static_cast<int64_t>(blockIdx.y) * kTile;
-  }
-  const int srow = tid >> 3, spart = tid & 7;
-
-  v16f acc[TM][TN];
-#pragma unroll
-  for (int a = 0; a < TM; ++a)
-#pragma unroll
-    for (int b = 0; b < TN; ++b)
-#pragma unroll
-      for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
-
-  // Rows past m (columns past n) of a ragged edge tile load the last valid
-  // row (column) instead: they only feed accumulator rows (columns) that the
-  // epilogue does not store, and the loads stay branch-free.
-  // two register slots, named apart: a slot index that is a run-time value
-  // anywhere would put the slots in scratch memory
-  uint4 ra0[CH], rb0[CH], ra1[CH], rb1[CH];
-  const T* arow[CH];
-  const T* brow[CH];
-#pragma unroll
-  for (int i = 0; i < CH; ++i) {
-    const int64_t r = min(m0 + srow + RSTEP * i, static_cast<int64_t>(m) - 1);
-    const int64_t c = min(n0 + srow + RSTEP * i, static_cast<int64_t>(n) - 1);
-    arow[i] = At + r * ldak + spart * E;
-    brow[i] = Bt + c * ldbk + spart * E;
-  }
-  auto load = [&](auto slot_c, int k0) {
-#pragma unroll
-    for (int i = 0; i < CH; ++i) {
-      const uint4 a = *reinterpret_cast<const uint4*>(arow[i] + k0);
-      const uint4 b = *reinterpret_cast<const uint4*>(brow[i] + k0);
-      if constexpr (decltype(slot_c)::value == 0) {
-        ra0[i] = a;
-        rb0[i] = b;
-      } else {
-        ra1[i] = a;
-        rb1[i] = b;
+    int64_t m0, n0;
+    if constexpr (kGemmGroupRows > 0) {
+      const int mt = (m + kTile - 1) / kTile, nt = (n + kTile - 1) / kTile, tiles = mt * nt;
+      const int per = (tiles + 7) / 8;
+      if constexpr (PERSIST) {
+        if (vw >= 8 * per) return;           // every virtual workgroup done
+        if (vw != static_cast<int>(blockIdx.x)) __syncthreads();  // the previous tile's LDS reads are done
       }
+      const int v = (vw & 7) * per + (vw >> 3);  // this XCD's run of the grouped order
+      if (v >= tiles) {                        // padding of the grid (the whole workgroup)
+        if constexpr (PERSIST) continue;
+        return;
+      }
+      const int first = v / (kGemmGroupRows * nt) * kGemmGroupRows;
+      const int rows = min(kGemmGroupRows, mt - first);
+      const int within = v - first * nt;
+      m0 = static_cast<int64_t>(first + within % rows) * kTile;
+      n0 = static_cast<int64_t>(within / rows) * kTile;
+    } else {
+      m0 = static_cast<int64_t>(blockIdx.x) * kTile;
+      n0 = static_cast<int64_t>(blockIdx.y) * kTile;
     }
-  };
-  auto store = [&](auto slot_c, int buf) {
+    const int srow = tid >> 3, spart = tid & 7;
+
+    v16f acc[TM][TN];
+#pragma unroll
+    for (int a = 0; a < TM; ++a)
+#pragma unroll
+      for (int b = 0; b < TN; ++b)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.0f;
+
+    // Rows past m (columns past n) of a ragged edge tile load the last valid
+    // row (column) instead: they only feed accumulator rows (columns) that the
+    // epilogue does not store, and the loads stay branch-free.
+    // two register slots, named apart: a slot index that is a run-time value
+    // anywhere would put the slots in scratch memory
+    uint4 ra0[CH], rb0[CH], ra1[CH], rb1[CH];
+    const T* arow[CH];
+    const T* brow[CH];
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      constexpr bool first = decltype(slot_c)::value == 0;
-      *reinterpret_cast<uint4*>(&As[buf][srow + RSTEP * i][spart * E]) = first ? ra0[i] : ra1[i];
-      *reinterpret_cast<uint4*>(&Bs[buf][srow + RSTEP * i][spart * E]) = first ? rb0[i] : rb1[i];
+      const int64_t r = min(m0 + srow + RSTEP * i, static_cast<int64_t>(m) - 1);
+      const int64_t c = min(n0 + srow + RSTEP * i, static_cast<int64_t>(n) - 1);
+      arow[i] = At + r * ldak + spart * E;
+      brow[i] = Bt + c * ldbk + spart * E;
     }
-  };
-  auto compute = [&](int cur) {
-    if constexpr (std::is_same<T, float>::value) {
+    auto load = [&](auto slot_c, int k0) {
 #pragma unroll
-      for (int kg = 0; kg < BK; kg += 8) {
-        v4f a[TM], b[TN];
+      for (int i = 0; i < CH; ++i) {
+        const uint4 a = *reinterpret_cast<const uint4*>(arow[i] + k0);
+        const uint4 b = *reinterpret_cast<const uint4*>(brow[i] + k0);
+        if constexpr (decltype(slot_c)::value == 0) {
+          ra0[i] = a;
+          rb0[i] = b;
+        } else {
+          ra1[i] = a;
+          rb1[i] = b;
+        }
+      }
+    };
+    auto store = [&](auto slot_c, int buf) {
 #pragma unroll
-        for (int t = 0; t < TM; ++t)
-          a[t] = *reinterpret_cast<const v4f*>(&As[cur][wm + 32 * t + (lane & 31)][kg + 4 * (lane >> 5)]);
+      for (int i = 0; i < CH; ++i) {
+        constexpr bool first = decltype(slot_c)::value == 0;
+        *reinterpret_cast<uint4*>(&As[buf][srow + RSTEP * i][spart * E]) = first ? ra0[i] : ra1[i];
+        *reinterpret_cast<uint4*>(&Bs[buf][srow + RSTEP * i][spart * E]) = first ? rb0[i] : rb1[i];
+      }
+    };
+    auto compute = [&](int cur) {
+      if constexpr (std::is_same<T, float>::value) {
 #pragma unroll
-        for (int t = 0; t < TN; ++t)
-          b[t] = *reinterpret_cast<const v4f*>(&Bs[cur][wn + 32 * t + (lane & 31)][kg + 4 * (lane >> 5)]);
+        for (int kg = 0; kg < BK; kg += 8) {
+          v4f a[TM], b[TN];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+          for (int t = 0; t < TM; ++t)
+            a[t] = *reinterpret_cast<const v4f*>(&As[cur][wm + 32 * t + (lane & 31)][kg + 4 * (lane >> 5)]);
+#pragma unroll
+          for (int t = 0; t < TN; ++t)
+            b[t] = *reinterpret_cast<const v4f*>(&Bs[cur][wn + 32 * t + (lane & 31)][kg + 4 * (lane >> 5)]);
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+              for (int tb = 0; tb < TN; ++tb)
+                acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ta][j], b[tb][j], acc[ta][tb], 0, 0, 0);
+        }
+      } else {
+        using V8 = typename std::conditional<std::is_same<T, __bf16>::value, v8bf, v8h>::type;
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+          V8 a[TM], b[TN];
+#pragma unroll
+          for (int t = 0; t < TM; ++t)
+            a[t] = *reinterpret_cast<const V8*>(&As[cur][wm + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)]);
+#pragma unroll
+          for (int t = 0; t < TN; ++t)
+            b[t] = *reinterpret_cast<const V8*>(&Bs[cur][wn + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)]);
 #pragma unroll
           for (int ta = 0; ta < TM; ++ta)
 #pragma unroll
-            for (int tb = 0; tb < TN; ++tb)
-              acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ta][j], b[tb][j], acc[ta][tb], 0, 0, 0);
-      }
-    } else {
-      using V8 = typename std::conditional<std::is_same<T, __bf16>::value, v8bf, v8h>::type;
-#pragma unroll
-      for (int kk = 0; kk < BK; kk += 16) {
-        V8 a[TM], b[TN];
-#pragma unroll
-        for (int t = 0; t < TM; ++t)
-          a[t] = *reinterpret_cast<const V8*>(&As[cur][wm + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)]);
-#pragma unroll
-        for (int t = 0; t < TN; ++t)
-          b[t] = *reinterpret_cast<const V8*>(&Bs[cur][wn + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)]);
-#pragma unroll
-        for (int ta = 0; ta < TM; ++ta)
-#pragma unroll
-          for (int tb = 0; tb < TN; ++tb) {
-            if constexpr (std::is_same<T, __bf16>::value)
-              acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
-            else
-              acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
-          }
-      }
-    }
-  };
-
-  using Slot0 = std::integral_constant<int, 0>;
-  using Slot1 = std::integral_constant<int, 1>;
-  const int stages = K / BK;
-  load(Slot0{}, 0);
-  store(Slot0{}, 0);
-  if constexpr (PF == 1) {
-    __syncthreads();
-    for (int s = 0; s < stages; ++s) {
-      const int cur = s & 1;
-      if (s + 1 < stages) load(Slot0{}, (s + 1) * BK);
-      compute(cur);
-      if (s + 1 < stages) store(Slot0{}, cur ^ 1);
-      __syncthreads();
-    }
-  } else {
-    // register slot (s & 1) holds stage s until it is stored, then stage s + 2;
-    // the loop is unrolled by two so that every slot index is a constant
-    if (stages > 1) load(Slot1{}, BK);
-    __syncthreads();
-    auto step = [&](auto cur_c, int s) {
-      constexpr int cur = decltype(cur_c)::value;
-      if (s + 1 < stages) store(std::integral_constant<int, cur ^ 1>{}, cur ^ 1);  // stage s + 1
-      if (s + 2 < stages) load(cur_c, (s + 2) * BK);
-      compute(cur);
-      __syncthreads();
-    };
-    for (int s = 0; s < stages; s += 2) {
-      step(Slot0{}, s);
-      if (s + 1 < stages) step(Slot1{}, s + 1);
-    }
-  }
-
-  // C -= acc / (sa sb), or C = acc (STORE).  A whole tile with 16-byte C
-  // access (the same for every thread of the workgroup) reads half its
-  // wave's C blocks at a time, all loads in flight together, into the
-  // registers the K loop no longer needs; other tiles take the general path.
-  if (!STORE && vec4 && m0 + kTile <= m && n0 + kTile <= n) {
-    constexpr int TH = TN >= 4 ? 1 : 4 / TN;  // row blocks per round: 16 loads in flight
-    static_assert(TM % TH == 0, "whole rounds of the wave's row blocks");
-    // the wave's block scales first (uniform: scalar loads), so that no wait
-    // for them falls between the C loads
-    const int uw = __builtin_amdgcn_readfirstlane(wave);
-    const int64_t um = m0 + (uw % WM) * (kTile / WM), un = n0 + (uw / WM) * (kTile / WN);
-    float w[TM][TN];
-#pragma unroll
-    for (int ta = 0; ta < TM; ++ta)
-#pragma unroll
-      for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f;
-    if (sa) {
-      float ra[TM], rb[TN];
-#pragma unroll
-      for (int ta = 0; ta < TM; ++ta) ra[ta] = sa[(um + 32 * ta) >> 5];
-#pragma unroll
-      for (int tb = 0; tb < TN; ++tb) rb[tb] = sb[(un + 32 * tb) >> 5];
-#pragma unroll
-      for (int ta = 0; ta < TM; ++ta)
-#pragma unroll
-        for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f / (ra[ta] * rb[tb]);  // exact: powers of two
-    }
-    const float* cbase = C + (un + (lane & 31)) * ldc + um + 4 * (lane >> 5);
-#pragma unroll
-    for (int h = 0; h < TM / TH; ++h) {
-      v4f cv[TH][TN][4];
-#pragma unroll
-      for (int a = 0; a < TH; ++a)
-#pragma unroll
-        for (int tb = 0; tb < TN; ++tb)
-#pragma unroll
-          for (int g = 0; g < 4; ++g)
-            cv[a][tb][g] = *reinterpret_cast<const v4f*>(cbase + 32 * tb * ldc + 32 * (h * TH + a) + 8 * g);
-#pragma unroll
-      for (int a = 0; a < TH; ++a)
-#pragma unroll
-        for (int tb = 0; tb < TN; ++tb) {
-          const int ta = h * TH + a;
-#pragma unroll
-          for (int g = 0; g < 4; ++g) {
-            v4f c = cv[a][tb][g];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) c[i] = fmaf(-w[ta][tb], acc[ta][tb][4 * g + i], c[i]);
-            *reinterpret_cast<v4f*>(const_cast<float*>(cbase) + 32 * tb * ldc + 32 * ta + 8 * g) = c;
-          }
-        }
-    }
-    if constexpr (PERSIST) continue;
-    return;
-  }
-#pragma unroll
-  for (int ta = 0; ta < TM; ++ta)
-#pragma unroll
-    for (int tb = 0; tb < TN; ++tb) {
-      const int64_t col = n0 + wn + 32 * tb + (lane & 31);
-      if (col >= n) continue;
-      float* cc = C + col * ldc;
-      float w = 1.0f;
-      if (!STORE && sa) {  // exact: powers of two
-        const int64_t rb = min(m0 + wm + 32 * ta, static_cast<int64_t>(m) - 1) >> 5;  // rows past m: unused
-        w = 1.0f / (sa[rb] * sb[col >> 5]);
-      }
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int64_t row = m0 + wm + 32 * ta + 8 * g + 4 * (lane >> 5);
-        if (vec4 && row + 3 < m) {
-          v4f c;
-          if constexpr (STORE) {
-            c[0] = acc[ta][tb][4 * g + 0];
-            c[1] = acc[ta][tb][4 * g + 1];
-            c[2] = acc[ta][tb][4 * g + 2];
-            c[3] = acc[ta][tb][4 * g + 3];
-          } else {
-            c = *reinterpret_cast<const v4f*>(cc + row);
-            c[0] = fmaf(-w, acc[ta][tb][4 * g + 0], c[0]);
-            c[1] = fmaf(-w, acc[ta][tb][4 * g + 1], c[1]);
-            c[2] = fmaf(-w, acc[ta][tb][4 * g + 2], c[2]);
-            c[3] = fmaf(-w, acc[ta][tb][4 * g + 3], c[3]);
-          }
-          *reinterpret_cast<v4f*>(cc + row) = c;
-        } else {
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (row + i < m) {
-              if constexpr (STORE)
-                cc[row + i] = acc[ta][tb][4 * g + i];
+            for (int tb = 0; tb < TN; ++tb) {
+              if constexpr (std::is_same<T, __bf16>::value)
+                acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
               else
-                cc[row + i] = fmaf(-w, acc[ta][tb][4 * g + i], cc[row + i]);
+                acc[ta][tb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[ta], b[tb], acc[ta][tb], 0, 0, 0);
             }
         }
       }
+    };
+
+    using Slot0 = std::integral_constant<int, 0>;
+    using Slot1 = std::integral_constant<int, 1>;
+    const int stages = K / BK;
+    load(Slot0{}, 0);
+    store(Slot0{}, 0);
+    if constexpr (PF == 1) {
+      __syncthreads();
+      for (int s = 0; s < stages; ++s) {
+        const int cur = s & 1;
+        if (s + 1 < stages) load(Slot0{}, (s + 1) * BK);
+        compute(cur);
+        if (s + 1 < stages) store(Slot0{}, cur ^ 1);
+        __syncthreads();
+      }
+    } else {
+      // register slot (s & 1) holds stage s until it is stored, then stage s + 2;
+      // the loop is unrolled by two so that every slot index is a constant
+      if (stages > 1) load(Slot1{}, BK);
+      __syncthreads();
+      auto step = [&](auto cur_c, int s) {
+        constexpr int cur = decltype(cur_c)::value;
+        if (s + 1 < stages) store(std::integral_constant<int, cur ^ 1>{}, cur ^ 1);  // stage s + 1
+        if (s + 2 < stages) load(cur_c, (s + 2) * BK);
+        compute(cur);
+        __syncthreads();
+      };
+      for (int s = 0; s < stages; s += 2) {
+        step(Slot0{}, s);
+        if (s + 1 < stages) step(Slot1{}, s + 1);
+      }
     }
-  if constexpr (!PERSIST) return;
+
+    // C -= acc / (sa sb), or C = acc (STORE).  A whole tile with 16-byte C
+    // access (the same for every thread of the workgroup) reads half its
+    // wave's C blocks at a time, all loads in flight together, into the
+    // registers the K loop no longer needs; other tiles take the general path.
+    if (!STORE && vec4 && m0 + kTile <= m && n0 + kTile <= n) {
+      constexpr int TH = TN >= 4 ? 1 : 4 / TN;  // row blocks per round: 16 loads in flight
+      static_assert(TM % TH == 0, "whole rounds of the wave's row blocks");
+      // the wave's block scales first (uniform: scalar loads), so that no wait
+      // for them falls between the C loads
+      const int uw = __builtin_amdgcn_readfirstlane(wave);
+      const int64_t um = m0 + (uw % WM) * (kTile / WM), un = n0 + (uw / WM) * (kTile / WN);
+      float w[TM][TN];
+#pragma unroll
+      for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f;
+      if (sa) {
+        float ra[TM], rb[TN];
+#pragma unroll
+        for (int ta = 0; ta < TM; ++ta) ra[ta] = sa[(um + 32 * ta) >> 5];
+#pragma unroll
+        for (int tb = 0; tb < TN; ++tb) rb[tb] = sb[(un + 32 * tb) >> 5];
+#pragma unroll
+        for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+          for (int tb = 0; tb < TN; ++tb) w[ta][tb] = 1.0f / (ra[ta] * rb[tb]);  // exact: powers of two
+      }
+      const float* cbase = C + (un + (lane & 31)) * ldc + um + 4 * (lane >> 5);
+#pragma unroll
+      for (int h = 0; h < TM / TH; ++h) {
+        v4f cv[TH][TN][4];
+#pragma unroll
+        for (int a = 0; a < TH; ++a)
+#pragma unroll
+          for (int tb = 0; tb < TN; ++tb)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+              cv[a][tb][g] = *reinterpret_cast<const v4f*>(cbase + 32 * tb * ldc + 32 * (h * TH + a) + 8 * g);
+#pragma unroll
+        for (int a = 0; a < TH; ++a)
+#pragma unroll
+          for (int tb = 0; tb < TN; ++tb) {
+            const int ta = h * TH + a;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+              v4f c = cv[a][tb][g];
+#pragma unroll
+              for (int i = 0; i < 4; ++i) c[i] = fmaf(-w[ta][tb], acc[ta][tb][4 * g + i], c[i]);
+              *reinterpret_cast<v4f*>(const_cast<float*>(cbase) + 32 * tb * ldc + 32 * ta + 8 * g) = c;
+            }
+          }
+      }
+      if constexpr (PERSIST) continue;
+      return;
+    }
+#pragma unroll
+    for (int ta = 0; ta < TM; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < TN; ++tb) {
+        const int64_t col = n0 + wn + 32 * tb + (lane & 31);
+        if (col >= n) continue;
+        float* cc = C + col * ldc;
+        float w = 1.0f;
+        if (!STORE && sa) {  // exact: powers of two
+          const int64_t rb = min(m0 + wm + 32 * ta, static_cast<int64_t>(m) - 1) >> 5;  // rows past m: unused
+          w = 1.0f / (sa[rb] * sb[col >> 5]);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int64_t row = m0 + wm + 32 * ta + 8 * g + 4 * (lane >> 5);
+          if (vec4 && row + 3 < m) {
+            v4f c;
+            if constexpr (STORE) {
+              c[0] = acc[ta][tb][4 * g + 0];
+              c[1] = acc[ta][tb][4 * g + 1];
+              c[2] = acc[ta][tb][4 * g + 2];
+              c[3] = acc[ta][tb][4 * g + 3];
+            } else {
+              c = *reinterpret_cast<const v4f*>(cc + row);
+              c[0] = fmaf(-w, acc[ta][tb][4 * g + 0], c[0]);
+              c[1] = fmaf(-w, acc[ta][tb][4 * g + 1], c[1]);
+              c[2] = fmaf(-w, acc[ta][tb][4 * g + 2], c[2]);
+              c[3] = fmaf(-w, acc[ta][tb][4 * g + 3], c[3]);
+            }
+            *reinterpret_cast<v4f*>(cc + row) = c;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (row + i < m) {
+                if constexpr (STORE)
+                  cc[row + i] = acc[ta][tb][4 * g + i];
+                else
+                  cc[row + i] = fmaf(-w, acc[ta][tb][4 * g + i], cc[row + i]);
+              }
+          }
+        }
+      }
+    if constexpr (!PERSIST) return;
   }
 }
 
