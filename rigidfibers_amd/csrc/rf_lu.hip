@@ -581,11 +581,13 @@ static_assert(kLuPanel % kBgRows == 0 && kBgRows % 4 == 0 && kLuPanel % 256 == 0
 // y[r0:r1] -= F[r0:r1, c0:c0+w] x[0:w] (w <= W; the rows disjoint from the
 // columns' block; r0 and lda multiples of 4).  Workgroup = kSgRows rows x all
 // w columns: wavefront v takes columns [v W / kSgWaves, (v + 1) W / kSgWaves),
-// lane 4 consecutive rows (one 16-byte load per column), FP64 sums; the
-// wavefronts' partial sums are added in LDS in a fixed order.  With
-// copy != nullptr, workgroup 0 also writes x to copy[0:w] (the solved block
-// back into the vector).
-constexpr int kSgWaves = 16, kSgRows = 256;
+// lane 2 consecutive rows (one 8-byte load per column), FP64 sums; the
+// wavefronts' partial sums are added in LDS in a fixed order.  128 rows per
+// workgroup (256 before, lane 4 rows: the same sums in the same order) give a
+// panel with r rows below it r / 128 workgroups, more of the chip for the
+// panels past the first third.  With copy != nullptr, workgroup 0 also
+// writes x to copy[0:w] (the solved block back into the vector).
+constexpr int kSgWaves = 16, kSgRows = 128;
 __global__ __launch_bounds__(64 * kSgWaves) void lu_solve_gemv_kernel(int64_t lda, const float* __restrict__ F,
                                                                        int64_t c0, int w, int64_t r0, int64_t r1,
                                                                        const double* __restrict__ x,
@@ -602,38 +604,33 @@ __global__ __launch_bounds__(64 * kSgWaves) void lu_solve_gemv_kernel(int64_t ld
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
   const int64_t rt = r0 + static_cast<int64_t>(blockIdx.x) * kSgRows;  // the workgroup's first row
-  const int64_t rb = rt + 4 * lane;                                     // the lane's first row
+  const int64_t rb = rt + 2 * lane;                                     // the lane's first row
   const int cb = wave * CW, ce = min(w, cb + CW);
-  double s[4] = {0.0, 0.0, 0.0, 0.0};
-  if (rb + 4 <= r1) {
+  double s[2] = {0.0, 0.0};
+  if (rb + 2 <= r1) {
     const float* p = F + c0 * lda + rb;
     int c = cb;
     for (; c + U <= ce; c += U) {
-      float4 v[U];
+      float2 v[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const float4*>(p + (c + u) * lda);
+      for (int u = 0; u < U; ++u) v[u] = *reinterpret_cast<const float2*>(p + (c + u) * lda);
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const double xc = xs[c + u];
         s[0] = fma(static_cast<double>(v[u].x), xc, s[0]);
         s[1] = fma(static_cast<double>(v[u].y), xc, s[1]);
-        s[2] = fma(static_cast<double>(v[u].z), xc, s[2]);
-        s[3] = fma(static_cast<double>(v[u].w), xc, s[3]);
       }
     }
     for (; c < ce; ++c) {
-      const float4 v = *reinterpret_cast<const float4*>(p + c * lda);
+      const float2 v = *reinterpret_cast<const float2*>(p + c * lda);
       s[0] = fma(static_cast<double>(v.x), xs[c], s[0]);
       s[1] = fma(static_cast<double>(v.y), xs[c], s[1]);
-      s[2] = fma(static_cast<double>(v.z), xs[c], s[2]);
-      s[3] = fma(static_cast<double>(v.w), xs[c], s[3]);
     }
   } else if (rb < r1) {
-    for (int i = 0; i < 4 && rb + i < r1; ++i)
-      for (int c = cb; c < ce; ++c) s[i] = fma(static_cast<double>(F[(c0 + c) * lda + rb + i]), xs[c], s[i]);
+    for (int c = cb; c < ce; ++c) s[0] = fma(static_cast<double>(F[(c0 + c) * lda + rb]), xs[c], s[0]);
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) part[wave][4 * lane + i] = s[i];
+  for (int i = 0; i < 2; ++i) part[wave][2 * lane + i] = s[i];
   __syncthreads();
   if (tid < kSgRows && rt + tid < r1) {
     double acc = part[0][tid];
@@ -642,7 +639,7 @@ __global__ __launch_bounds__(64 * kSgWaves) void lu_solve_gemv_kernel(int64_t ld
     y[rt + tid] -= acc;
   }
 }
-static_assert(kSgRows == 4 * 64 && kLuPanel % kSgWaves == 0 && 64 * kSgWaves >= kSgRows, "solve GEMV tiling");
+static_assert(kSgRows == 2 * 64 && kLuPanel % kSgWaves == 0 && 64 * kSgWaves >= kSgRows, "solve GEMV tiling");
 
 // The inverses of a factored W x W diagonal block D = F[K0:K0+W, K0:K0+W]:
 // the panel solves above applied to unit vectors, kInvCols of them per
